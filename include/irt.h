@@ -1,0 +1,339 @@
+/*
+ * irt.h — C ABI of the MI355X (gfx950) per-pixel lighting + denoising path.
+ *
+ * Replaces the wgpu compute passes that src/renderer.rs schedules for the
+ * ReSTIR direct/VPL reservoirs, the SDF soft-shadow resolve and SVGF
+ * (reference: src/renderer.rs:1873-1941, src/svgf.rs:140-261).  The caller
+ * (renderer.rs, or any host) keeps packing the same POD records it uploads to
+ * wgpu today and hands them over through plain pointers; the library owns the
+ * device memory and all cross-frame state (reservoirs, SVGF moments/feedback/
+ * history, previous G-buffer, VPL archive ring).
+ *
+ * Conventions
+ *   - Every entry point returns IRT_OK (0) or a negative irt_status.  The text
+ *     of the last failure of a context is returned by irt_last_error(); errors
+ *     raised before a context exists go to irt_global_error().
+ *   - Matrices are 4x4 column-major float (glam Mat4 memory order).
+ *   - Pixel buffers exchanged through the ABI are row-major, id = y*W + x, in
+ *     the reference's texture formats: Depth32Float = float, Rgba16F = four
+ *     IEEE binary16 bit patterns (uint16_t), Rg16F = two, Rgba32F = four float.
+ *   - Calls on one context are serialised on the context's HIP stream and are
+ *     asynchronous unless stated; irt_read* and irt_sync block.  Different
+ *     contexts are independent (one per view / session).
+ *   - No pointer passed in is retained after a call returns, except device
+ *     pointers handed to irt_set_gbuffer_device (read during that call only).
+ */
+#ifndef IRT_H
+#define IRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRT_ABI_VERSION 1u
+
+/* ---------------------------------------------------------------- status */
+typedef enum irt_status {
+  IRT_OK = 0,
+  IRT_ERR_INVALID_ARGUMENT = -1, /* null pointer, wrong size, bad enum       */
+  IRT_ERR_INVALID_SETTINGS = -2, /* Settings::validate failure (same text)   */
+  IRT_ERR_LIMIT = -3,            /* adapter-limit style checks               */
+  IRT_ERR_STATE = -4,            /* call order (e.g. no scene/frame yet)     */
+  IRT_ERR_DEVICE = -5,           /* HIP runtime failure or no gfx950 device  */
+  IRT_ERR_OUT_OF_MEMORY = -6
+} irt_status;
+
+/* ------------------------------------------------------------- POD records
+ * Byte-for-byte the layouts of src/renderer.rs:8-66 / src/shaders/common.wgsl:3-16
+ * (sizes asserted by the reference at src/renderer.rs:2254-2261). */
+
+/* Frame uniform, 384 B (renderer.rs:51-66, packing at renderer.rs:1687-1735). */
+typedef struct irt_frame {
+  float vp[16];
+  float inv_vp[16];
+  float previous_vp[16];
+  float previous_inv_vp[16];
+  float camera[4];     /* xyz, 0 */
+  uint32_t size[4];    /* width, height, frame_index, scene generation */
+  uint32_t counts[4];  /* lights, vpl_count, vpl_max_bounces, objects */
+  uint32_t restir[4];  /* direct candidates, VPL candidates, spatial neighbours, spatial radius */
+  float temporal[4];   /* history weight, depth threshold, normal threshold, SVGF max history */
+  float sdf[4];        /* max steps, hit epsilon, soft-shadow k, 0 */
+  float rr[4];         /* roulette min, roulette max, ReSTIR max history, 0 */
+  uint32_t flags[4];   /* history enabled, previous generation, debug mode, 0 */
+} irt_frame;
+
+/* GpuObject, 208 B (renderer.rs:8-15, built by objects() renderer.rs:359-384). */
+typedef struct irt_object {
+  float model[16];
+  float previous[16];
+  float inverse[16];
+  uint32_t info[4]; /* mesh index, material index, bits(scale), bits(voxel diagonal * scale) */
+} irt_object;
+
+/* GpuMesh, 128 B (renderer.rs:16-28, filled at renderer.rs:846-885). */
+typedef struct irt_mesh {
+  float lo[4];               /* SDF grid bounds (node-centred samples) */
+  float hi[4];
+  uint32_t geometry[4];      /* vertex offset, vertex count, index offset, index count */
+  uint32_t field[4];         /* atlas origin x, y, z, resolution */
+  float grid[4];             /* 1/spacing xyz, |spacing| */
+  float geometry_lo[4];      /* vertex AABB xyz, w = 1 if certified box */
+  float geometry_hi[4];      /* vertex AABB xyz, w = 1/Lipschitz bound */
+  float enclosing_sphere[4]; /* centre xyz, radius */
+} irt_mesh;
+
+/* GpuLight, 48 B (renderer.rs:37-43, renderer.rs:1743-1755). */
+typedef struct irt_light {
+  float position_strength[4]; /* xyz, W/sr */
+  float direction_angle[4];   /* spot axis xyz, full cone angle (rad) */
+  float color_type[4];        /* rgb, w = 1 for spot */
+} irt_light;
+
+/* BVH node, 48 B, preorder with escape links (renderer.rs:44-50, 385-455). */
+typedef struct irt_node {
+  float lo[4];
+  float hi[4];
+  uint32_t info[4]; /* leaf: object id / inner: left, inner: right, is_leaf, escape index */
+} irt_node;
+
+/* Virtual point light, 48 B (common.wgsl:15). */
+typedef struct irt_vpl {
+  float position[4];
+  float normal[4];
+  float flux[4];
+} irt_vpl;
+
+/* Reservoir, 32 B (common.wgsl:16). */
+typedef struct irt_reservoir {
+  uint32_t id;
+  uint32_t source; /* frame index the sample was drawn in */
+  uint32_t count;  /* M */
+  uint32_t age;
+  float sum;             /* w_sum */
+  float selected_target; /* p-hat of the selected sample */
+  float pdf;
+  uint32_t generation;
+} irt_reservoir;
+
+/* Mirror of the reference Settings record (src/settings.rs:3-28). */
+typedef struct irt_settings {
+  uint32_t render_width;
+  uint32_t render_height;
+  uint32_t vpl_count;
+  uint32_t vpl_max_bounces;
+  float vpl_rr_min_survival;
+  float vpl_rr_max_survival;
+  uint32_t restir_direct_candidates;
+  uint32_t restir_vpl_candidates;
+  uint32_t restir_spatial_neighbors;
+  uint32_t restir_spatial_radius;
+  float depth_threshold;
+  float normal_threshold;
+  float temporal_history_weight;
+  uint32_t temporal_max_history;
+  uint32_t restir_max_history;
+  uint32_t svgf_iterations;
+  uint32_t sdf_max_steps;
+  float sdf_hit_epsilon;
+  float sdf_soft_shadow_k;
+  float target_frame_ms;
+} irt_settings;
+
+/* Storage profile of a context. */
+typedef enum irt_precision {
+  /* FP32 arithmetic; every buffer the reference keeps in FP32 stays FP32 and
+   * the G-buffer is widened to FP32.  Values that the reference writes to an
+   * Rgba16F target are rounded to binary16 at the same point, so results
+   * follow the reference's storage rounding. */
+  IRT_PRECISION_FP32 = 0,
+  /* FP16 G-buffer / radiance / SVGF radiance, FP16 first moment with FP32
+   * second moment, 16-bit SDF samples; FP32 march, FP32 reservoir W/M. */
+  IRT_PRECISION_MIXED = 1
+} irt_precision;
+
+/* Schedules of irt_encode / irt_lighting / irt_svgf. */
+typedef enum irt_schedule {
+  IRT_SCHEDULE_PASSES = 0, /* one kernel per reference pass (11 + copies) */
+  IRT_SCHEDULE_FUSED = 1   /* fused kernels: reservoirs A/B, SVGF chain */
+} irt_schedule;
+
+/* Single passes, in the reference's entry-point order (renderer.rs:1477-1486, svgf.rs:100). */
+typedef enum irt_pass_id {
+  IRT_PASS_CANDIDATES_DIRECT = 0,
+  IRT_PASS_CANDIDATES_INDIRECT = 1,
+  IRT_PASS_TEMPORAL_DIRECT = 2,
+  IRT_PASS_TEMPORAL_INDIRECT = 3,
+  IRT_PASS_SPATIAL_DIRECT = 4,
+  IRT_PASS_SPATIAL_INDIRECT = 5,
+  IRT_PASS_RESOLVE_DIRECT = 6,
+  IRT_PASS_RESOLVE_INDIRECT = 7,
+  IRT_PASS_SVGF_TEMPORAL = 8,
+  IRT_PASS_SVGF_VARIANCE = 9,
+  IRT_PASS_SVGF_ATROUS = 10, /* stride argument of irt_pass_ex selects 1<<i */
+  IRT_PASS_COUNT = 11
+} irt_pass_id;
+
+/* Buffers visible to irt_read / irt_write (the reference's test hooks,
+ * src/renderer.rs:457-712).  Formats are the reference's. */
+typedef enum irt_buffer_id {
+  IRT_BUF_DIRECT0 = 0,      /* irt_reservoir[W*H] */
+  IRT_BUF_DIRECT1 = 1,
+  IRT_BUF_DIRECT2 = 2,      /* cross-frame direct reservoirs */
+  IRT_BUF_INDIRECT0 = 3,
+  IRT_BUF_INDIRECT1 = 4,
+  IRT_BUF_INDIRECT2 = 5,    /* cross-frame indirect reservoirs */
+  IRT_BUF_DIRECT_LIGHTING = 6, /* Rgba32F direct HDR */
+  IRT_BUF_LIGHTING = 7,     /* Rgba16F noisy HDR (resolve output, SVGF input) */
+  IRT_BUF_HISTORY = 8,      /* Rgba16F denoised histories[current] */
+  IRT_BUF_MOMENTS = 9,      /* Rgba32F moments[current] (m1, m2, age, 0) */
+  IRT_BUF_MOMENTS_OLD = 10, /* Rgba32F moments[old] */
+  IRT_BUF_FEEDBACK = 11,    /* Rgba16F feedback[current] */
+  IRT_BUF_FEEDBACK_OLD = 12,/* Rgba16F feedback[old] */
+  IRT_BUF_TEMPORAL = 13,    /* Rgba16F SVGF temporal radiance */
+  IRT_BUF_DEPTH = 14,       /* Depth32Float */
+  IRT_BUF_NORMAL = 15,      /* Rgba16F */
+  IRT_BUF_COLOR = 16,       /* Rgba16F */
+  IRT_BUF_PARAMS = 17,      /* Rg16F */
+  IRT_BUF_MOTION = 18,      /* Rg16F */
+  IRT_BUF_PREVIOUS_DEPTH = 19,  /* Depth32Float */
+  IRT_BUF_PREVIOUS_NORMAL = 20, /* Rgba16F */
+  IRT_BUF_VPL_ARCHIVE = 21, /* irt_vpl[(restir_max_history+1)*vpl_count] */
+  IRT_BUF_SCRATCH0 = 22,    /* Rgba16F SVGF scratch[0] (variance output) */
+  IRT_BUF_SCRATCH1 = 23,    /* Rgba16F SVGF scratch[1] */
+  IRT_BUF_COUNT = 24
+} irt_buffer_id;
+
+typedef struct irt_ctx irt_ctx;
+
+/* Version of this ABI (IRT_ABI_VERSION). */
+uint32_t irt_abi_version(void);
+
+/* Text of the last failure that happened without a context (thread-local). */
+const char* irt_global_error(void);
+
+/* Settings::validate (src/settings.rs:42-106): same checks, same messages
+ * ("invalid render size", "invalid VPL budget", ...). */
+int irt_settings_validate(const irt_settings* settings);
+
+/* Renderer::new's hot-path part (src/renderer.rs:713-766, 1205-1333,
+ * src/svgf.rs:16-138): validates settings and the reservoir/VPL-archive
+ * budget checks of renderer.rs:750-766, allocates all per-view state on the
+ * current HIP device.  `precision` is an irt_precision. */
+int irt_create(const irt_settings* settings, uint32_t precision, irt_ctx** out);
+void irt_destroy(irt_ctx* ctx);
+const char* irt_last_error(const irt_ctx* ctx);
+
+/* Selects IRT_SCHEDULE_PASSES or IRT_SCHEDULE_FUSED for irt_encode/irt_lighting/irt_svgf. */
+int irt_set_schedule(irt_ctx* ctx, uint32_t schedule);
+
+/* Scene (group 0 bindings, renderer.rs:1115-1204).  The SDF atlas is the
+ * reference's R32Float 3-D texture, x fastest: texel(x,y,z) = texels[x + w*(y + h*z)]
+ * (renderer.rs:776-845). */
+int irt_set_sdf_atlas(irt_ctx* ctx, const float* texels, uint32_t w, uint32_t h, uint32_t d);
+int irt_set_meshes(irt_ctx* ctx, const irt_mesh* meshes, uint32_t count);
+/* objects() and hierarchy() uploads of renderer.rs:1736-1782 (every frame in the reference). */
+int irt_set_objects(irt_ctx* ctx, const irt_object* objects, uint32_t count,
+                    const irt_node* nodes, uint32_t node_count);
+int irt_set_lights(irt_ctx* ctx, const irt_light* lights, uint32_t count);
+
+/* Frame uniform (renderer.rs:1687-1737).  size[0..1] must equal the context's size. */
+int irt_set_frame(irt_ctx* ctx, const irt_frame* frame);
+
+/* G-buffer of the frame being rendered, host memory, reference formats
+ * (renderer.rs:1207-1262): depth float[W*H]; normal/color uint16[4*W*H]
+ * binary16; params/motion uint16[2*W*H]. */
+int irt_set_gbuffer(irt_ctx* ctx, const float* depth, const uint16_t* normal,
+                    const uint16_t* color, const uint16_t* params, const uint16_t* motion);
+/* Same, from device memory already resident in HBM (read on the context stream). */
+int irt_set_gbuffer_device(irt_ctx* ctx, const float* depth, const uint16_t* normal,
+                           const uint16_t* color, const uint16_t* params, const uint16_t* motion);
+
+/* This frame's VPL pool (vpl.wgsl output).  Written to archive ring slot
+ * frame_index % (restir_max_history + 1) — the copy of renderer.rs:1873-1884.
+ * `count` must equal settings.vpl_count.  Call after irt_set_frame. */
+int irt_set_vpls(irt_ctx* ctx, const irt_vpl* vpls, uint32_t count);
+int irt_set_vpls_device(irt_ctx* ctx, const irt_vpl* vpls, uint32_t count);
+
+/* The six reservoir passes and the two resolves (renderer.rs:1889-1928). */
+int irt_lighting(irt_ctx* ctx);
+/* Svgf::encode (svgf.rs:140-261): temporal, variance, `iterations` a-trous,
+ * feedback and final copy; debug >= 3 copies the noisy image (svgf.rs:154-161). */
+int irt_svgf(irt_ctx* ctx, uint32_t iterations, uint32_t debug);
+/* End-of-frame history (renderer.rs:1946-1960): current depth/normal become previous. */
+int irt_end_frame(irt_ctx* ctx);
+/* irt_lighting + irt_svgf(settings.svgf_iterations, flags.z) + irt_end_frame. */
+int irt_encode(irt_ctx* ctx);
+
+/* One reference pass with the reference's bindings for the frame schedule
+ * (pass_group at renderer.rs:1617-1674; svgf.rs:163-253).  For
+ * IRT_PASS_SVGF_ATROUS `arg` is the iteration index i (stride 1<<i, reads
+ * scratch[i%2], writes scratch[1-i%2]); ignored otherwise. */
+int irt_pass(irt_ctx* ctx, uint32_t pass, uint32_t arg);
+
+/* Read-back / injection of internal state in reference formats (blocking).
+ * `bytes` must equal the buffer's size in that format. */
+size_t irt_buffer_bytes(const irt_ctx* ctx, uint32_t buffer);
+int irt_read(irt_ctx* ctx, uint32_t buffer, void* dst, size_t bytes);
+int irt_write(irt_ctx* ctx, uint32_t buffer, const void* src, size_t bytes);
+
+/* Block until every call enqueued on the context has finished. */
+int irt_sync(irt_ctx* ctx);
+
+/* HIP stream of the context (hipStream_t as void*). */
+void* irt_stream(irt_ctx* ctx);
+
+/* GPU time of every pass recorded since timing was enabled or last read
+ * (read_gpu_time, renderer.rs:2066-2108), one entry per launch in launch
+ * order; reading clears the record.  Names: candidates_direct,
+ * candidates_indirect, temporal_direct, temporal_indirect, spatial_direct,
+ * spatial_indirect, visibility_direct, visibility_indirect, svgf; in the fused
+ * schedule the fused kernels report under reservoirs_a, reservoirs_b_resolve,
+ * svgf.  Returns the number of entries written. */
+int irt_enable_timing(irt_ctx* ctx, int enable);
+int irt_pass_times(irt_ctx* ctx, char names[][32], float* ms, int capacity);
+
+/* Event counters of the next irt_encode / irt_lighting (instrumented
+ * kernels; slower).  counts[] order: brdf, node tests, box object
+ * evaluations, grid object evaluations, march steps, SVGF geometry taps,
+ * march calls.  Reset on enable. */
+#define IRT_COUNTER_COUNT 7
+int irt_enable_counters(irt_ctx* ctx, int enable);
+int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
+
+/* ------------------------------------------------ host precompute (a23)
+ * The scene-side helpers renderer.rs runs on the CPU before upload. */
+
+/* is_box_geometry (renderer.rs:287-329). positions: xyz stride `stride_floats`. */
+int irt_host_is_box_geometry(const float* positions, uint32_t vertex_count, uint32_t stride_floats,
+                             const uint32_t* indices, uint32_t index_count,
+                             const float lo[3], const float hi[3]);
+/* field_lipschitz_bound (renderer.rs:330-358). */
+float irt_host_field_lipschitz_bound(const float* samples, uint32_t resolution, const float spacing[3]);
+/* hierarchy (renderer.rs:385-455): world AABBs of the objects' SDF boxes
+ * (lo/hi as xyz triples) → preorder nodes with escape links.  `nodes` must
+ * hold 2*count-1 records.  Returns the node count. */
+int irt_host_hierarchy(const float* box_lo, const float* box_hi, uint32_t count, irt_node* nodes);
+/* Object-space SDF box → world AABB of its 8 transformed corners (renderer.rs:387-424). */
+void irt_host_transform_box(const float model[16], const float lo[3], const float hi[3],
+                            float out_lo[3], float out_hi[3]);
+/* SDF atlas placement of renderer.rs:778-817: grid (gx, gy, gz); mesh m's tile
+ * origin is ((m%gx)*tile, ((m/gx)%gy)*tile, (m/(gx*gy))*tile). */
+void irt_host_atlas_grid(uint32_t mesh_count, uint32_t tile, uint32_t grid[3]);
+/* Frame packing of Renderer::encode (renderer.rs:1687-1735).  inv_vp and
+ * previous_inv_vp are the inverses of vp / previous_vp; history_enabled is
+ * flags.x (frame_index > 0 && previous debug mode == debug mode). */
+int irt_host_pack_frame(const irt_settings* settings, const float vp[16], const float previous_vp[16],
+                        const float camera[3], uint32_t frame_index, uint32_t generation,
+                        uint32_t previous_generation, uint32_t history_enabled, uint32_t debug_mode,
+                        uint32_t light_count, uint32_t object_count, irt_frame* out);
+/* 4x4 column-major inverse (double internally, rounded to float). Returns 0 if singular. */
+int irt_host_inverse(const float m[16], float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IRT_H */

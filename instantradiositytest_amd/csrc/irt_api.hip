@@ -1,0 +1,893 @@
+// C ABI of the lighting + denoising path (include/irt.h): per-view context,
+// device buffers, pass schedules, format conversion for read-back/injection.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "irt_host.h"
+#include "irt_kernels.h"
+
+using namespace irt;
+
+namespace {
+
+enum Fmt {
+  FMT_RESERVOIR,      // irt_reservoir (same inside and outside)
+  FMT_F32,            // float (Depth32Float)
+  FMT_F32X4,          // float4 <-> Rgba32F
+  FMT_F32X4_AS_F16X4, // internal float4 holding binary16 values <-> Rgba16F
+  FMT_F16X4,          // internal binary16x4 <-> Rgba16F
+  FMT_F32X2_AS_F16X2, // internal float2 <-> Rg16F
+  FMT_F16X2,          // internal binary16x2 <-> Rg16F
+  FMT_F16X4_AS_F32X4, // internal binary16x4 <-> Rgba32F
+  FMT_VPL             // irt_vpl
+};
+
+size_t internal_bytes(Fmt f) {
+  switch (f) {
+    case FMT_RESERVOIR: return 32;
+    case FMT_F32: return 4;
+    case FMT_F32X4: case FMT_F32X4_AS_F16X4: return 16;
+    case FMT_F16X4: case FMT_F16X4_AS_F32X4: return 8;
+    case FMT_F32X2_AS_F16X2: return 8;
+    case FMT_F16X2: return 4;
+    case FMT_VPL: return 48;
+  }
+  return 0;
+}
+size_t external_bytes(Fmt f) {
+  switch (f) {
+    case FMT_RESERVOIR: return 32;
+    case FMT_F32: return 4;
+    case FMT_F32X4: case FMT_F16X4_AS_F32X4: return 16;
+    case FMT_F32X4_AS_F16X4: case FMT_F16X4: return 8;
+    case FMT_F32X2_AS_F16X2: case FMT_F16X2: return 4;
+    case FMT_VPL: return 48;
+  }
+  return 0;
+}
+
+struct Timer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct irt_ctx {
+  irt_settings s{};
+  uint32_t precision = IRT_PRECISION_FP32;
+  uint32_t schedule = IRT_SCHEDULE_PASSES;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t W = 0, H = 0, V = 0, slots = 0;
+  size_t npx = 0;
+  // scene
+  irt_object* objects = nullptr;
+  uint32_t n_objects = 0, cap_objects = 0;
+  irt_node* nodes = nullptr;
+  uint32_t n_nodes = 0, cap_nodes = 0;
+  irt_mesh* meshes = nullptr;
+  uint32_t n_meshes = 0, cap_meshes = 0;
+  irt_light* lights = nullptr;
+  uint32_t n_lights = 0, cap_lights = 0;
+  float* atlas = nullptr;
+  uint32_t atlas_dims[3] = {0, 0, 0};
+  irt_frame frame{};
+  bool have_frame = false;
+  std::vector<irt_mesh> host_meshes;     // host copies for launch-time validation
+  uint32_t max_mesh_ref = 0;             // largest mesh index referenced by an object
+  // G-buffer (internal formats) and history
+  float* depth = nullptr;
+  void* normal = nullptr;
+  void* color = nullptr;
+  void* params = nullptr;
+  void* motion = nullptr;
+  float* prev_depth = nullptr;
+  void* prev_normal = nullptr;
+  uint16_t* stage[4] = {nullptr, nullptr, nullptr, nullptr};  // binary16 staging (FP32 profile ingest)
+  irt_reservoir* direct[3] = {nullptr, nullptr, nullptr};
+  irt_reservoir* indirect[3] = {nullptr, nullptr, nullptr};
+  irt_vpl* ring = nullptr;
+  void* direct_lighting = nullptr;
+  void* lighting = nullptr;
+  void* temporal = nullptr;
+  void* scratch[2] = {nullptr, nullptr};
+  void* feedback[2] = {nullptr, nullptr};
+  void* history = nullptr;
+  float4* moments[2] = {nullptr, nullptr};
+  unsigned long long* counters = nullptr;
+  bool counting = false;
+  bool timing = false;
+  std::vector<Timer> timers;
+  size_t n_timers_used = 0;
+  std::vector<void*> allocs;
+  std::string err;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+#define HIP_CHECK(c, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return (c)->fail(IRT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+int dev_alloc(irt_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return c->fail(IRT_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  e = hipMemsetAsync(*p, 0, bytes, c->stream);
+  if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return IRT_OK;
+}
+
+// Grow-only device array for small scene records.
+template <class T>
+int ensure_array(irt_ctx* c, T** p, uint32_t* cap, uint32_t count) {
+  uint32_t need = count > 0 ? count : 1;
+  if (*p && *cap >= need) return IRT_OK;
+  if (*p) {
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (auto& a : c->allocs)
+      if (a == *p) a = nullptr;
+    HIP_CHECK(c, hipFree(*p));
+    *p = nullptr;
+  }
+  int rc = dev_alloc(c, (void**)p, sizeof(T) * need);
+  if (rc) return rc;
+  *cap = need;
+  return IRT_OK;
+}
+
+bool mixed(const irt_ctx* c) { return c->precision == IRT_PRECISION_MIXED; }
+
+struct BufDesc {
+  void* ptr;
+  Fmt fmt;
+  size_t count;
+};
+
+BufDesc describe(irt_ctx* c, uint32_t b) {
+  const bool m = mixed(c);
+  const uint32_t cur = c->frame.size[2] % 2u, old = 1u - cur;
+  const Fmt n4 = m ? FMT_F16X4 : FMT_F32X4_AS_F16X4;
+  const Fmt p2 = m ? FMT_F16X2 : FMT_F32X2_AS_F16X2;
+  const Fmt r4 = m ? FMT_F16X4 : FMT_F32X4_AS_F16X4;
+  const size_t n = c->npx;
+  switch (b) {
+    case IRT_BUF_DIRECT0: case IRT_BUF_DIRECT1: case IRT_BUF_DIRECT2:
+      return {c->direct[b - IRT_BUF_DIRECT0], FMT_RESERVOIR, n};
+    case IRT_BUF_INDIRECT0: case IRT_BUF_INDIRECT1: case IRT_BUF_INDIRECT2:
+      return {c->indirect[b - IRT_BUF_INDIRECT0], FMT_RESERVOIR, n};
+    case IRT_BUF_DIRECT_LIGHTING: return {c->direct_lighting, m ? FMT_F16X4_AS_F32X4 : FMT_F32X4, n};
+    case IRT_BUF_LIGHTING: return {c->lighting, r4, n};
+    case IRT_BUF_HISTORY: return {c->history, r4, n};
+    case IRT_BUF_MOMENTS: return {c->moments[cur], FMT_F32X4, n};
+    case IRT_BUF_MOMENTS_OLD: return {c->moments[old], FMT_F32X4, n};
+    case IRT_BUF_FEEDBACK: return {c->feedback[cur], r4, n};
+    case IRT_BUF_FEEDBACK_OLD: return {c->feedback[old], r4, n};
+    case IRT_BUF_TEMPORAL: return {c->temporal, r4, n};
+    case IRT_BUF_DEPTH: return {c->depth, FMT_F32, n};
+    case IRT_BUF_NORMAL: return {c->normal, n4, n};
+    case IRT_BUF_COLOR: return {c->color, n4, n};
+    case IRT_BUF_PARAMS: return {c->params, p2, n};
+    case IRT_BUF_MOTION: return {c->motion, p2, n};
+    case IRT_BUF_PREVIOUS_DEPTH: return {c->prev_depth, FMT_F32, n};
+    case IRT_BUF_PREVIOUS_NORMAL: return {c->prev_normal, n4, n};
+    case IRT_BUF_VPL_ARCHIVE: return {c->ring, FMT_VPL, (size_t)c->slots * c->V};
+    case IRT_BUF_SCRATCH0: return {c->scratch[0], r4, n};
+    case IRT_BUF_SCRATCH1: return {c->scratch[1], r4, n};
+  }
+  return {nullptr, FMT_F32, 0};
+}
+
+// internal -> external (reference format), host side.
+void to_external(Fmt f, const uint8_t* in, uint8_t* out, size_t count) {
+  switch (f) {
+    case FMT_F32X4_AS_F16X4: {
+      const float* s = (const float*)in;
+      uint16_t* d = (uint16_t*)out;
+      for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f32_to_f16(s[i]);
+      break;
+    }
+    case FMT_F32X2_AS_F16X2: {
+      const float* s = (const float*)in;
+      uint16_t* d = (uint16_t*)out;
+      for (size_t i = 0; i < count * 2; ++i) d[i] = irt_host::f32_to_f16(s[i]);
+      break;
+    }
+    case FMT_F16X4_AS_F32X4: {
+      const uint16_t* s = (const uint16_t*)in;
+      float* d = (float*)out;
+      for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f16_to_f32(s[i]);
+      break;
+    }
+    default:
+      std::memcpy(out, in, count * internal_bytes(f));
+  }
+}
+void to_internal(Fmt f, const uint8_t* in, uint8_t* out, size_t count) {
+  switch (f) {
+    case FMT_F32X4_AS_F16X4: {
+      const uint16_t* s = (const uint16_t*)in;
+      float* d = (float*)out;
+      for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f16_to_f32(s[i]);
+      break;
+    }
+    case FMT_F32X2_AS_F16X2: {
+      const uint16_t* s = (const uint16_t*)in;
+      float* d = (float*)out;
+      for (size_t i = 0; i < count * 2; ++i) d[i] = irt_host::f16_to_f32(s[i]);
+      break;
+    }
+    case FMT_F16X4_AS_F32X4: {
+      const float* s = (const float*)in;
+      uint16_t* d = (uint16_t*)out;
+      for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f32_to_f16(s[i]);
+      break;
+    }
+    default:
+      std::memcpy(out, in, count * internal_bytes(f));
+  }
+}
+
+// ------------------------------------------------------------- timing
+void timer_begin(irt_ctx* c, const char* name) {
+  if (!c->timing) return;
+  if (c->n_timers_used >= c->timers.size()) {
+    Timer t;
+    t.name = name;
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+    c->timers.push_back(t);
+  }
+  Timer& t = c->timers[c->n_timers_used];
+  t.name = name;
+  (void)hipEventRecord(t.a, c->stream);
+}
+void timer_end(irt_ctx* c) {
+  if (!c->timing) return;
+  (void)hipEventRecord(c->timers[c->n_timers_used].b, c->stream);
+  c->n_timers_used++;
+}
+
+// ------------------------------------------------------------- launches
+dim3 grid_of(const irt_ctx* c) { return dim3((c->W + TILE - 1) / TILE, (c->H + TILE - 1) / TILE, 1); }
+
+template <class P>
+LightArgs<P> light_args(irt_ctx* c, int in, int out, int hist) {
+  LightArgs<P> a;
+  std::memset(&a, 0, sizeof(a));
+  a.F = c->frame;
+  a.sc = SceneView{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1]};
+  a.depth = c->depth;
+  a.normal = (const typename P::N4*)c->normal;
+  a.color = (const typename P::N4*)c->color;
+  a.params = (const typename P::P2*)c->params;
+  a.motion = (const typename P::P2*)c->motion;
+  a.prev_depth = c->prev_depth;
+  a.prev_normal = (const typename P::N4*)c->prev_normal;
+  a.direct_in = c->direct[in];
+  a.indirect_in = c->indirect[in];
+  a.direct_out = c->direct[out];
+  a.indirect_out = c->indirect[out];
+  a.direct_history = c->direct[hist];
+  a.indirect_history = c->indirect[hist];
+  a.vpl_ring = c->ring;
+  a.direct_lighting = (typename P::L4*)c->direct_lighting;
+  a.lighting = (typename P::R4*)c->lighting;
+  a.counters = c->counters;
+  return a;
+}
+
+template <class P>
+SvgfArgs<P> svgf_args(irt_ctx* c) {
+  SvgfArgs<P> a;
+  std::memset(&a, 0, sizeof(a));
+  a.F = c->frame;
+  a.depth = c->depth;
+  a.normal = (const typename P::N4*)c->normal;
+  a.color = (const typename P::N4*)c->color;
+  a.params = (const typename P::P2*)c->params;
+  a.motion = (const typename P::P2*)c->motion;
+  a.old_depth = c->prev_depth;
+  a.old_normal = (const typename P::N4*)c->prev_normal;
+  a.noisy = (const typename P::R4*)c->lighting;
+  a.counters = c->counters;
+  a.stride = 1;
+  return a;
+}
+
+#define LAUNCH(c, kern, P, ...)                                                                          \
+  do {                                                                                                   \
+    if ((c)->counting)                                                                                   \
+      hipLaunchKernelGGL((kern<P, true>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+    else                                                                                                 \
+      hipLaunchKernelGGL((kern<P, false>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+    HIP_CHECK(c, hipGetLastError());                                                                     \
+  } while (0)
+
+int check_ready(irt_ctx* c) {
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
+  if (!c->objects || c->n_objects == 0) return c->fail(IRT_ERR_STATE, "no objects set (irt_set_objects)");
+  if (!c->meshes || c->n_meshes == 0) return c->fail(IRT_ERR_STATE, "no meshes set (irt_set_meshes)");
+  if (!c->atlas) return c->fail(IRT_ERR_STATE, "no SDF atlas set (irt_set_sdf_atlas)");
+  if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "an object references a missing mesh");
+  for (const irt_mesh& m : c->host_meshes) {
+    const uint32_t res = m.field[3];
+    if (m.field[0] + res > c->atlas_dims[0] || m.field[1] + res > c->atlas_dims[1] ||
+        m.field[2] + res > c->atlas_dims[2])
+      return c->fail(IRT_ERR_STATE, "mesh SDF tile lies outside the atlas");
+  }
+  if (c->frame.counts[0] > c->n_lights) return c->fail(IRT_ERR_STATE, "frame light count exceeds uploaded lights");
+  if (c->frame.counts[3] != c->n_objects) return c->fail(IRT_ERR_STATE, "frame object count differs from uploaded objects");
+  return IRT_OK;
+}
+
+// One reservoir/resolve pass with the reference's bindings (pass_group, renderer.rs:1617-1674).
+template <class P>
+int lighting_pass(irt_ctx* c, uint32_t pass) {
+  switch (pass) {
+    case IRT_PASS_CANDIDATES_DIRECT: LAUNCH(c, k_candidates_direct, P, light_args<P>(c, 2, 0, 2)); break;
+    case IRT_PASS_CANDIDATES_INDIRECT: LAUNCH(c, k_candidates_indirect, P, light_args<P>(c, 2, 0, 2)); break;
+    case IRT_PASS_TEMPORAL_DIRECT: LAUNCH(c, k_temporal_direct, P, light_args<P>(c, 0, 1, 2)); break;
+    case IRT_PASS_TEMPORAL_INDIRECT: LAUNCH(c, k_temporal_indirect, P, light_args<P>(c, 0, 1, 2)); break;
+    case IRT_PASS_SPATIAL_DIRECT: LAUNCH(c, k_spatial_direct, P, light_args<P>(c, 1, 2, 0)); break;
+    case IRT_PASS_SPATIAL_INDIRECT: LAUNCH(c, k_spatial_indirect, P, light_args<P>(c, 1, 2, 0)); break;
+    case IRT_PASS_RESOLVE_DIRECT: LAUNCH(c, k_resolve_direct, P, light_args<P>(c, 2, 0, 2)); break;
+    case IRT_PASS_RESOLVE_INDIRECT: LAUNCH(c, k_resolve_indirect, P, light_args<P>(c, 2, 0, 2)); break;
+    default: return c->fail(IRT_ERR_INVALID_ARGUMENT, "not a lighting pass");
+  }
+  return IRT_OK;
+}
+
+template <class P>
+int run_lighting(irt_ctx* c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->schedule == IRT_SCHEDULE_FUSED) {
+    timer_begin(c, "reservoirs_a");
+    LAUNCH(c, k_reservoirs_a, P, light_args<P>(c, 2, 1, 2), c->direct[0], c->indirect[0]);
+    timer_end(c);
+    timer_begin(c, "reservoirs_b_resolve");
+    LAUNCH(c, k_reservoirs_b, P, light_args<P>(c, 1, 2, 0));
+    timer_end(c);
+    return IRT_OK;
+  }
+  static const char* names[8] = {"candidates_direct", "candidates_indirect", "temporal_direct",
+                                 "temporal_indirect", "spatial_direct", "spatial_indirect",
+                                 "visibility_direct", "visibility_indirect"};
+  for (uint32_t p = 0; p < 8; ++p) {
+    timer_begin(c, names[p]);
+    rc = lighting_pass<P>(c, p);
+    if (rc) return rc;
+    timer_end(c);
+  }
+  return IRT_OK;
+}
+
+template <class P>
+int svgf_temporal_pass(irt_ctx* c) {
+  const uint32_t cur = c->frame.size[2] % 2u, old = 1u - cur;
+  SvgfArgs<P> a = svgf_args<P>(c);
+  a.old_radiance = (const typename P::R4*)c->feedback[old];
+  a.old_moments = c->moments[old];
+  a.input = (const typename P::R4*)c->feedback[old];
+  a.moments = c->moments[old];
+  a.out = (typename P::R4*)c->temporal;
+  a.moments_out = c->moments[cur];
+  LAUNCH(c, k_svgf_temporal, P, a);
+  return IRT_OK;
+}
+template <class P>
+int svgf_variance_pass(irt_ctx* c) {
+  const uint32_t cur = c->frame.size[2] % 2u;
+  SvgfArgs<P> a = svgf_args<P>(c);
+  a.input = (const typename P::R4*)c->temporal;
+  a.moments = c->moments[cur];
+  a.out = (typename P::R4*)c->scratch[0];
+  LAUNCH(c, k_svgf_variance, P, a);
+  return IRT_OK;
+}
+template <class P>
+int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2) {
+  const uint32_t cur = c->frame.size[2] % 2u;
+  SvgfArgs<P> a = svgf_args<P>(c);
+  a.input = (const typename P::R4*)in;
+  a.moments = c->moments[cur];
+  a.out = (typename P::R4*)out;
+  a.out2 = (typename P::R4*)out2;
+  a.stride = 1u << i;
+  LAUNCH(c, k_svgf_atrous, P, a);
+  return IRT_OK;
+}
+
+size_t r4_bytes(const irt_ctx* c) { return mixed(c) ? 8 : 16; }
+
+// Svgf::encode — svgf.rs:140-261.
+template <class P>
+int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
+  if (iterations == 0 || iterations > 5) return c->fail(IRT_ERR_INVALID_ARGUMENT, "SVGF iterations must be 1..5");
+  const uint32_t cur = c->frame.size[2] % 2u;
+  const size_t rb = r4_bytes(c) * c->npx;
+  timer_begin(c, "svgf");
+  if (debug >= 3) {
+    HIP_CHECK(c, hipMemcpyAsync(c->history, c->lighting, rb, hipMemcpyDeviceToDevice, c->stream));
+    timer_end(c);
+    return IRT_OK;
+  }
+  int rc = svgf_temporal_pass<P>(c);
+  if (!rc) rc = svgf_variance_pass<P>(c);
+  if (rc) return rc;
+  if (c->schedule == IRT_SCHEDULE_FUSED) {
+    // Feedback and final copies folded into the a-trous stores.
+    void* in = c->scratch[0];
+    for (uint32_t i = 0; i < iterations; ++i) {
+      const bool last = i + 1 == iterations;
+      void* out;
+      void* out2 = nullptr;
+      if (i == 0) {
+        out = c->feedback[cur];
+        if (last) out2 = c->history;
+      } else if (last) {
+        out = c->history;
+      } else {
+        out = in == c->scratch[1] ? c->scratch[0] : c->scratch[1];
+      }
+      rc = svgf_atrous_pass<P>(c, i, in, out, out2);
+      if (rc) return rc;
+      in = out;
+    }
+  } else {
+    for (uint32_t i = 0; i < iterations; ++i) {
+      rc = svgf_atrous_pass<P>(c, i, c->scratch[i % 2], c->scratch[1 - i % 2], nullptr);
+      if (rc) return rc;
+      if (i == 0)
+        HIP_CHECK(c, hipMemcpyAsync(c->feedback[cur], c->scratch[1], rb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_CHECK(c, hipMemcpyAsync(c->history, c->scratch[iterations % 2], rb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  timer_end(c);
+  return IRT_OK;
+}
+
+template <class P>
+int run_pass(irt_ctx* c, uint32_t pass, uint32_t arg) {
+  if (pass < 8) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    return lighting_pass<P>(c, pass);
+  }
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
+  if (pass == IRT_PASS_SVGF_TEMPORAL) return svgf_temporal_pass<P>(c);
+  if (pass == IRT_PASS_SVGF_VARIANCE) return svgf_variance_pass<P>(c);
+  if (pass == IRT_PASS_SVGF_ATROUS) {
+    if (arg > 4) return c->fail(IRT_ERR_INVALID_ARGUMENT, "a-trous iteration must be 0..4");
+    int rc = svgf_atrous_pass<P>(c, arg, c->scratch[arg % 2], c->scratch[1 - arg % 2], nullptr);
+    if (rc) return rc;
+    if (arg == 0) {
+      const uint32_t cur = c->frame.size[2] % 2u;
+      HIP_CHECK(c, hipMemcpyAsync(c->feedback[cur], c->scratch[1], r4_bytes(c) * c->npx, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    return IRT_OK;
+  }
+  return c->fail(IRT_ERR_INVALID_ARGUMENT, "unknown pass id");
+}
+
+template <class P>
+int ingest(irt_ctx* c, const float* depth, const uint16_t* n, const uint16_t* col, const uint16_t* prm,
+           const uint16_t* mot) {
+  const uint32_t count = (uint32_t)c->npx;
+  hipLaunchKernelGGL((k_ingest_gbuffer<P>), dim3((count + 255) / 256), dim3(256), 0, c->stream, count, depth, n, col,
+                     prm, mot, c->depth, (typename P::N4*)c->normal, (typename P::N4*)c->color,
+                     (typename P::P2*)c->params, (typename P::P2*)c->motion);
+  HIP_CHECK(c, hipGetLastError());
+  return IRT_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+uint32_t irt_abi_version(void) { return IRT_ABI_VERSION; }
+const char* irt_global_error(void) { return irt_host::global_error(); }
+
+int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
+  if (!s || !out) {
+    irt_host::set_global_error("null argument");
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  std::string msg;
+  int rc = irt_host::validate_settings(*s, &msg);
+  if (rc) {
+    irt_host::set_global_error(msg);
+    return rc;
+  }
+  if (precision > IRT_PRECISION_MIXED) {
+    irt_host::set_global_error("unknown precision profile");
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  const uint64_t pixels = (uint64_t)s->render_width * s->render_height;
+  // Adapter-limit checks of renderer.rs:750-766 against a 4 GiB binding.
+  const uint64_t max_binding = 1ull << 32;
+  if ((uint64_t)(s->restir_max_history + 1) * s->vpl_count * 48 > max_binding) {
+    irt_host::set_global_error("VPL history budget exceeds adapter storage binding limit");
+    return IRT_ERR_LIMIT;
+  }
+  if (pixels * 32 > max_binding) {
+    irt_host::set_global_error("render size exceeds adapter reservoir binding limit");
+    return IRT_ERR_LIMIT;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    irt_host::set_global_error("no HIP device available");
+    return IRT_ERR_DEVICE;
+  }
+  irt_ctx* c = new irt_ctx();
+  c->s = *s;
+  c->precision = precision;
+  (void)hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    irt_host::set_global_error("hipStreamCreate failed");
+    delete c;
+    return IRT_ERR_DEVICE;
+  }
+  c->W = s->render_width;
+  c->H = s->render_height;
+  c->npx = pixels;
+  c->V = s->vpl_count;
+  c->slots = s->restir_max_history + 1;
+  const bool m = precision == IRT_PRECISION_MIXED;
+  const size_t n = c->npx;
+  const size_t n4 = m ? 8 : 16, p2 = m ? 4 : 8, r4 = m ? 8 : 16, l4 = m ? 8 : 16;
+  rc = IRT_OK;
+  auto A = [&](void** p, size_t bytes) {
+    if (!rc) rc = dev_alloc(c, p, bytes);
+  };
+  A((void**)&c->depth, n * 4);
+  A(&c->normal, n * n4);
+  A(&c->color, n * n4);
+  A(&c->params, n * p2);
+  A(&c->motion, n * p2);
+  A((void**)&c->prev_depth, n * 4);
+  A(&c->prev_normal, n * n4);
+  if (!m) {
+    A((void**)&c->stage[0], n * 8);
+    A((void**)&c->stage[1], n * 8);
+    A((void**)&c->stage[2], n * 4);
+    A((void**)&c->stage[3], n * 4);
+  }
+  for (int i = 0; i < 3; ++i) {
+    A((void**)&c->direct[i], n * 32);
+    A((void**)&c->indirect[i], n * 32);
+  }
+  A((void**)&c->ring, (size_t)c->slots * c->V * sizeof(irt_vpl));
+  A(&c->direct_lighting, n * l4);
+  A(&c->lighting, n * r4);
+  A(&c->temporal, n * r4);
+  A(&c->scratch[0], n * r4);
+  A(&c->scratch[1], n * r4);
+  A(&c->feedback[0], n * r4);
+  A(&c->feedback[1], n * r4);
+  A(&c->history, n * r4);
+  A((void**)&c->moments[0], n * 16);
+  A((void**)&c->moments[1], n * 16);
+  A((void**)&c->counters, 8 * sizeof(unsigned long long));
+  // Depth targets start cleared to 1 (far plane) as the reference's depth pass clears them.
+  if (!rc) {
+    std::vector<float> ones(n, 1.0f);
+    if (hipMemcpyAsync(c->depth, ones.data(), n * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->prev_depth, ones.data(), n * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = c->fail(IRT_ERR_DEVICE, "initial depth clear failed");
+  }
+  if (rc) {
+    irt_host::set_global_error(c->err);
+    irt_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return IRT_OK;
+}
+
+void irt_destroy(irt_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs)
+    if (p) (void)hipFree(p);
+  for (auto& t : c->timers) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* irt_last_error(const irt_ctx* c) { return c ? c->err.c_str() : irt_host::global_error(); }
+
+int irt_set_schedule(irt_ctx* c, uint32_t schedule) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (schedule > IRT_SCHEDULE_FUSED) return c->fail(IRT_ERR_INVALID_ARGUMENT, "unknown schedule");
+  c->schedule = schedule;
+  return IRT_OK;
+}
+
+int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, uint32_t d) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!texels || w < 2 || h < 2 || d < 2) return c->fail(IRT_ERR_INVALID_ARGUMENT, "invalid SDF atlas");
+  const size_t bytes = (size_t)w * h * d * 4;
+  if (c->atlas) {
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (auto& a : c->allocs)
+      if (a == c->atlas) a = nullptr;
+    HIP_CHECK(c, hipFree(c->atlas));
+    c->atlas = nullptr;
+  }
+  int rc = dev_alloc(c, (void**)&c->atlas, bytes);
+  if (rc) return rc;
+  HIP_CHECK(c, hipMemcpyAsync(c->atlas, texels, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->atlas_dims[0] = w;
+  c->atlas_dims[1] = h;
+  c->atlas_dims[2] = d;
+  return IRT_OK;
+}
+
+int irt_set_meshes(irt_ctx* c, const irt_mesh* meshes, uint32_t count) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!meshes || count == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "no meshes");
+  for (uint32_t i = 0; i < count; ++i) {
+    const irt_mesh& m = meshes[i];
+    const uint32_t res = m.field[3];
+    if (res < 2) return c->fail(IRT_ERR_INVALID_ARGUMENT, "mesh SDF resolution must be >= 2");
+    if (c->atlas && (m.field[0] + res > c->atlas_dims[0] || m.field[1] + res > c->atlas_dims[1] ||
+                     m.field[2] + res > c->atlas_dims[2]))
+      return c->fail(IRT_ERR_INVALID_ARGUMENT, "mesh SDF tile lies outside the atlas");
+  }
+  int rc = ensure_array(c, &c->meshes, &c->cap_meshes, count);
+  if (rc) return rc;
+  HIP_CHECK(c, hipMemcpyAsync(c->meshes, meshes, sizeof(irt_mesh) * count, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->n_meshes = count;
+  c->host_meshes.assign(meshes, meshes + count);
+  return IRT_OK;
+}
+
+int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const irt_node* nodes,
+                    uint32_t node_count) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!objects || !nodes || count == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "no objects");
+  if (node_count != 2 * count - 1) return c->fail(IRT_ERR_INVALID_ARGUMENT, "node count must be 2*objects-1");
+  uint32_t max_ref = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (c->n_meshes && objects[i].info[0] >= c->n_meshes)
+      return c->fail(IRT_ERR_INVALID_ARGUMENT, "object references a missing mesh");
+    max_ref = std::max(max_ref, objects[i].info[0]);
+  }
+  for (uint32_t i = 0; i < node_count; ++i) {
+    const irt_node& nd = nodes[i];
+    if (nd.info[3] > node_count || (nd.info[2] == 1u && nd.info[0] >= count) ||
+        (nd.info[2] != 1u && (nd.info[0] >= node_count || nd.info[1] >= node_count)))
+      return c->fail(IRT_ERR_INVALID_ARGUMENT, "malformed hierarchy node");
+    if (nd.info[3] <= i) return c->fail(IRT_ERR_INVALID_ARGUMENT, "hierarchy escape link must point forward");
+  }
+  int rc = ensure_array(c, &c->objects, &c->cap_objects, count);
+  if (!rc) rc = ensure_array(c, &c->nodes, &c->cap_nodes, node_count);
+  if (rc) return rc;
+  HIP_CHECK(c, hipMemcpyAsync(c->objects, objects, sizeof(irt_object) * count, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->nodes, nodes, sizeof(irt_node) * node_count, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->n_objects = count;
+  c->n_nodes = node_count;
+  c->max_mesh_ref = max_ref;
+  return IRT_OK;
+}
+
+int irt_set_lights(irt_ctx* c, const irt_light* lights, uint32_t count) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (count && !lights) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null lights");
+  int rc = ensure_array(c, &c->lights, &c->cap_lights, count);
+  if (rc) return rc;
+  if (count) {
+    HIP_CHECK(c, hipMemcpyAsync(c->lights, lights, sizeof(irt_light) * count, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
+  c->n_lights = count;
+  return IRT_OK;
+}
+
+int irt_set_frame(irt_ctx* c, const irt_frame* f) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!f) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null frame");
+  if (f->size[0] != c->W || f->size[1] != c->H) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame size differs from context size");
+  if (f->counts[1] != c->V) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame VPL count differs from settings");
+  if (f->rr[2] != (float)c->s.restir_max_history)
+    return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame ReSTIR history differs from settings");
+  if (f->counts[3] == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame must have at least one object");
+  c->frame = *f;
+  c->have_frame = true;
+  return IRT_OK;
+}
+
+int irt_set_gbuffer(irt_ctx* c, const float* depth, const uint16_t* normal, const uint16_t* color,
+                    const uint16_t* params, const uint16_t* motion) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
+  const size_t n = c->npx;
+  HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyHostToDevice, c->stream));
+  if (mixed(c)) {
+    HIP_CHECK(c, hipMemcpyAsync(c->normal, normal, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->color, color, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->params, params, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->motion, motion, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return IRT_OK;
+  }
+  HIP_CHECK(c, hipMemcpyAsync(c->stage[0], normal, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->stage[1], color, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->stage[2], params, n * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->stage[3], motion, n * 4, hipMemcpyHostToDevice, c->stream));
+  int rc = ingest<ProfileF32>(c, c->depth, c->stage[0], c->stage[1], c->stage[2], c->stage[3]);
+  if (rc) return rc;
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+
+int irt_set_gbuffer_device(irt_ctx* c, const float* depth, const uint16_t* normal, const uint16_t* color,
+                           const uint16_t* params, const uint16_t* motion) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
+  if (mixed(c)) {
+    const size_t n = c->npx;
+    HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->normal, normal, n * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->color, color, n * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->params, params, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->motion, motion, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    return IRT_OK;
+  }
+  return ingest<ProfileF32>(c, depth, normal, color, params, motion);
+}
+
+static int set_vpls_impl(irt_ctx* c, const irt_vpl* v, uint32_t count, hipMemcpyKind kind) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!v) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null VPL pool");
+  if (count != c->V) return c->fail(IRT_ERR_INVALID_ARGUMENT, "VPL pool size differs from settings.vpl_count");
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "set the frame before its VPL pool");
+  const uint32_t slot = c->frame.size[2] % c->slots;
+  HIP_CHECK(c, hipMemcpyAsync(c->ring + (size_t)slot * c->V, v, sizeof(irt_vpl) * c->V, kind, c->stream));
+  if (kind == hipMemcpyHostToDevice) HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+int irt_set_vpls(irt_ctx* c, const irt_vpl* v, uint32_t count) { return set_vpls_impl(c, v, count, hipMemcpyHostToDevice); }
+int irt_set_vpls_device(irt_ctx* c, const irt_vpl* v, uint32_t count) {
+  return set_vpls_impl(c, v, count, hipMemcpyDeviceToDevice);
+}
+
+int irt_lighting(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  return mixed(c) ? run_lighting<ProfileMixed>(c) : run_lighting<ProfileF32>(c);
+}
+
+int irt_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  return mixed(c) ? run_svgf<ProfileMixed>(c, iterations, debug) : run_svgf<ProfileF32>(c, iterations, debug);
+}
+
+int irt_end_frame(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  const size_t n = c->npx;
+  HIP_CHECK(c, hipMemcpyAsync(c->prev_depth, c->depth, n * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->prev_normal, c->normal, n * (mixed(c) ? 8 : 16), hipMemcpyDeviceToDevice, c->stream));
+  return IRT_OK;
+}
+
+int irt_encode(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  int rc = irt_lighting(c);
+  if (!rc) rc = irt_svgf(c, c->s.svgf_iterations, c->frame.flags[2]);
+  if (!rc) rc = irt_end_frame(c);
+  return rc;
+}
+
+int irt_pass(irt_ctx* c, uint32_t pass, uint32_t arg) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (pass >= IRT_PASS_COUNT) return c->fail(IRT_ERR_INVALID_ARGUMENT, "unknown pass id");
+  return mixed(c) ? run_pass<ProfileMixed>(c, pass, arg) : run_pass<ProfileF32>(c, pass, arg);
+}
+
+size_t irt_buffer_bytes(const irt_ctx* c, uint32_t b) {
+  if (!c || b >= IRT_BUF_COUNT) return 0;
+  BufDesc d = describe(const_cast<irt_ctx*>(c), b);
+  return d.count * external_bytes(d.fmt);
+}
+
+int irt_read(irt_ctx* c, uint32_t b, void* dst, size_t bytes) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (b >= IRT_BUF_COUNT || !dst) return c->fail(IRT_ERR_INVALID_ARGUMENT, "bad buffer id or null destination");
+  BufDesc d = describe(c, b);
+  if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "read size mismatch");
+  std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
+  HIP_CHECK(c, hipMemcpyAsync(tmp.data(), d.ptr, tmp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  to_external(d.fmt, tmp.data(), (uint8_t*)dst, d.count);
+  return IRT_OK;
+}
+
+int irt_write(irt_ctx* c, uint32_t b, const void* src, size_t bytes) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (b >= IRT_BUF_COUNT || !src) return c->fail(IRT_ERR_INVALID_ARGUMENT, "bad buffer id or null source");
+  BufDesc d = describe(c, b);
+  if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "write size mismatch");
+  std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
+  to_internal(d.fmt, (const uint8_t*)src, tmp.data(), d.count);
+  HIP_CHECK(c, hipMemcpyAsync(d.ptr, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+
+int irt_sync(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+
+void* irt_stream(irt_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int irt_enable_timing(irt_ctx* c, int enable) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  c->timing = enable != 0;
+  c->n_timers_used = 0;
+  return IRT_OK;
+}
+
+int irt_pass_times(irt_ctx* c, char names[][32], float* ms, int capacity) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  int n = 0;
+  for (size_t i = 0; i < c->n_timers_used && n < capacity; ++i, ++n) {
+    float t = 0.f;
+    HIP_CHECK(c, hipEventElapsedTime(&t, c->timers[i].a, c->timers[i].b));
+    if (names) {
+      std::snprintf(names[n], 32, "%s", c->timers[i].name.c_str());
+    }
+    if (ms) ms[n] = t;
+  }
+  c->n_timers_used = 0;
+  return n;
+}
+
+int irt_enable_counters(irt_ctx* c, int enable) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  c->counting = enable != 0;
+  HIP_CHECK(c, hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
+  return IRT_OK;
+}
+
+int irt_read_counters(irt_ctx* c, uint64_t counts[IRT_COUNTER_COUNT]) {
+  if (!c || !counts) return IRT_ERR_INVALID_ARGUMENT;
+  unsigned long long tmp[8];
+  HIP_CHECK(c, hipMemcpyAsync(tmp, c->counters, sizeof(tmp), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < IRT_COUNTER_COUNT; ++i) counts[i] = tmp[i];
+  return IRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,436 @@
+// Device-side per-pixel math of the lighting + denoising path (gfx950).
+//
+// Each function restates the behaviour of one WGSL function of the reference;
+// the file:line of that function is cited beside it.  Semantics that matter
+// for parity are kept exactly: the PCG stream and its consumption points,
+// half-to-even round(), saturating float->int conversions, the reservoir
+// add() rule that lets NaN weights through, the order of the BVH walk and the
+// march's early-outs.  Everything else is written for the hardware: scalar
+// (kernarg) frame constants, LDS-resident scene records where a kernel stages
+// them, and explicit FP32 arithmetic (no packed-FP32 forms, see the build).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/irt.h"
+
+namespace irt {
+
+#define IRT_PI 3.14159265359f
+
+// ------------------------------------------------------------------ vectors
+struct v3 {
+  float x, y, z;
+};
+__device__ __forceinline__ v3 mk3(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return v3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 operator-(v3 a) { return v3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, v3 b) { return v3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ v3 operator*(float s, v3 a) { return v3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ v3 operator/(v3 a, float s) { return v3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float length(v3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ v3 normalize(v3 a) {
+  float s = 1.0f / sqrtf(dot(a, a));
+  return a * s;
+}
+__device__ __forceinline__ v3 vmax(v3 a, v3 b) { return v3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+__device__ __forceinline__ v3 vmin(v3 a, v3 b) { return v3{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ v3 vmaxs(v3 a, float s) { return v3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
+__device__ __forceinline__ v3 ld3(const float* p) { return v3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ v3 xyz(float4 v) { return v3{v.x, v.y, v.z}; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+// Column-major 4x4 times (x, y, z, w).
+__device__ __forceinline__ float4 mat_mul(const float* m, float x, float y, float z, float w) {
+  float4 r;
+  r.x = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
+  r.y = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
+  r.z = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+  r.w = m[3] * x + m[7] * y + m[11] * z + m[15] * w;
+  return r;
+}
+
+// ------------------------------------------------------------ conversions
+// WGSL f32 -> i32 / u32 conversions saturate (NaN -> 0); C casts do not.
+__device__ __forceinline__ int32_t f2i_sat(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (-2147483647 - 1);
+  return (int32_t)x;
+}
+__device__ __forceinline__ uint32_t f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967295.0f) return 4294967295u;
+  return (uint32_t)x;
+}
+// Rgba16F storage rounding (round to nearest even), as a texture write does.
+__device__ __forceinline__ float round_f16(float x) { return (float)(_Float16)x; }
+__device__ __forceinline__ float4 round_f16(float4 v) {
+  return make_float4(round_f16(v.x), round_f16(v.y), round_f16(v.z), round_f16(v.w));
+}
+
+// -------------------------------------------------------- storage formats
+// 16-bit channel storage (binary16 bit patterns) for the mixed profile.
+struct h4 {
+  _Float16 x, y, z, w;
+};
+struct h2 {
+  _Float16 x, y;
+};
+__device__ __forceinline__ float4 ld4(const float4* p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float4 ld4(const h4* p, uint32_t i) {
+  h4 v = p[i];
+  return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+}
+__device__ __forceinline__ float2 ld2(const float2* p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float2 ld2(const h2* p, uint32_t i) {
+  h2 v = p[i];
+  return make_float2((float)v.x, (float)v.y);
+}
+// Store a value the reference writes to an Rgba16F target.
+__device__ __forceinline__ void st_r16(float4* p, uint32_t i, float4 v) { p[i] = round_f16(v); }
+__device__ __forceinline__ void st_r16(h4* p, uint32_t i, float4 v) {
+  p[i] = h4{(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+}
+__device__ __forceinline__ void st4(float4* p, uint32_t i, float4 v) { p[i] = v; }
+__device__ __forceinline__ void st4(h4* p, uint32_t i, float4 v) { st_r16(p, i, v); }
+
+// Storage profiles.
+struct ProfileF32 {
+  typedef float4 N4;  // normal, colour (xyz/rgb, w)
+  typedef float2 P2;  // params (rough, metal), motion
+  typedef float4 R4;  // radiance targets (values binary16-rounded as the reference stores them)
+  typedef float4 L4;  // direct lighting (Rgba32F in the reference)
+};
+struct ProfileMixed {
+  typedef h4 N4;
+  typedef h2 P2;
+  typedef h4 R4;
+  typedef h4 L4;
+};
+
+// ------------------------------------------------------------- counters
+// Event counters for the algorithmic FLOP model (instrumented launches only).
+enum { C_BRDF = 0, C_NODE, C_OBJ_BOX, C_OBJ_GRID, C_STEP, C_TAP, C_MARCH, C_N };
+template <bool ON>
+struct Count {
+  __device__ __forceinline__ void add(int, uint32_t) {}
+  __device__ __forceinline__ void flush(unsigned long long*) {}
+};
+template <>
+struct Count<true> {
+  uint32_t c[C_N] = {0, 0, 0, 0, 0, 0, 0};
+  __device__ __forceinline__ void add(int k, uint32_t n) { c[k] += n; }
+  __device__ void flush(unsigned long long* out) {
+    for (int k = 0; k < C_N; ++k) {
+      unsigned long long v = c[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0 && v) atomicAdd(out + k, v);
+    }
+  }
+};
+
+// ------------------------------------------------------------------ RNG
+// random(), seed() — src/shaders/common.wgsl:39-44.
+__device__ __forceinline__ float rnd(uint32_t& s) {
+  s = s * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (float)((w >> 22u) ^ w) * (1.0f / 4294967296.0f);
+}
+__device__ __forceinline__ uint32_t seed(uint32_t i, uint32_t salt, uint32_t frame_index) {
+  return i * 1973u + frame_index * 9277u + salt * 26699u + 89173u;
+}
+
+// luminance, world, pixel_uv — common.wgsl:45-47.
+__device__ __forceinline__ float luminance(v3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+__device__ __forceinline__ v3 world(float u, float v, float depth, const float* m) {
+  float4 h = mat_mul(m, u * 2.0f - 1.0f, 1.0f - v * 2.0f, depth, 1.0f);
+  return v3{h.x / h.w, h.y / h.w, h.z / h.w};
+}
+__device__ __forceinline__ float pixel_u(int32_t x, uint32_t w) { return ((float)x + 0.5f) / (float)w; }
+
+// ------------------------------------------------------------------ BRDF
+struct Surface {
+  v3 p, n, color;
+  float rough, metal;
+};
+
+// x^n for the two fixed exponents used by the path.
+__device__ __forceinline__ float pow5(float x) {
+  float x2 = x * x;
+  return x2 * x2 * x;
+}
+__device__ __forceinline__ float pow64(float x) {
+  float y = x * x;  // 2
+  y = y * y;        // 4
+  y = y * y;        // 8
+  y = y * y;        // 16
+  y = y * y;        // 32
+  return y * y;     // 64
+}
+
+// brdf — common.wgsl:67-74.
+__device__ __forceinline__ v3 brdf(const Surface& s, v3 wo, v3 wi) {
+  float nv = fmaxf(dot(s.n, wo), 0.00001f);
+  float nl = fmaxf(dot(s.n, wi), 0.0f);
+  if (nl <= 0.0f) return v3{0.f, 0.f, 0.f};
+  v3 h = normalize(wo + wi);
+  float nh = fmaxf(dot(s.n, h), 0.0f);
+  float vh = fmaxf(dot(wo, h), 0.0f);
+  float a = s.rough * s.rough;
+  float a2 = a * a;
+  float q = nh * nh * (a2 - 1.0f) + 1.0f;
+  float d = a2 / (IRT_PI * q * q);
+  float g1v = 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
+  float g1l = 2.0f * nl / (nl + sqrtf(a2 + (1.0f - a2) * nl * nl));
+  float im = 1.0f - s.metal;
+  v3 f0 = v3{0.04f * im + s.color.x * s.metal, 0.04f * im + s.color.y * s.metal, 0.04f * im + s.color.z * s.metal};
+  float p5 = pow5(1.0f - vh);
+  v3 f = v3{f0.x + (1.0f - f0.x) * p5, f0.y + (1.0f - f0.y) * p5, f0.z + (1.0f - f0.z) * p5};
+  // (1-f)*(1-metal)*color/PI + f*d*g1v*g1l/max(4*nv*nl, 1e-5), evaluated left to right.
+  float den = fmaxf(4.0f * nv * nl, 0.00001f);
+  v3 diff = v3{(1.0f - f.x) * im * s.color.x / IRT_PI, (1.0f - f.y) * im * s.color.y / IRT_PI,
+               (1.0f - f.z) * im * s.color.z / IRT_PI};
+  v3 spec = v3{f.x * d * g1v * g1l / den, f.y * d * g1v * g1l / den, f.z * d * g1v * g1l / den};
+  return diff + spec;
+}
+
+// direct — common.wgsl:75-79.
+template <class C>
+__device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, v3 camera, C& cnt) {
+  v3 lp = v3{l.position_strength[0], l.position_strength[1], l.position_strength[2]};
+  v3 delta = lp - s.p;
+  float r2 = fmaxf(dot(delta, delta), 0.0001f);
+  v3 wi = delta * rsqrtf(r2);
+  if (l.color_type[3] > 0.5f) {
+    v3 axis = normalize(v3{l.direction_angle[0], l.direction_angle[1], l.direction_angle[2]});
+    if (dot(-wi, axis) < cosf(l.direction_angle[3] * 0.5f)) return v3{0.f, 0.f, 0.f};
+  }
+  cnt.add(C_BRDF, 1);
+  v3 b = brdf(s, normalize(camera - s.p), wi);
+  float k = fmaxf(dot(s.n, wi), 0.0f);
+  float strength = l.position_strength[3];
+  return v3{b.x * k * l.color_type[0] * strength / r2, b.y * k * l.color_type[1] * strength / r2,
+            b.z * k * l.color_type[2] * strength / r2};
+}
+
+// indirect — common.wgsl:80-83.
+template <class C>
+__device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, v3 camera, C& cnt) {
+  v3 delta = ld3(v.position) - s.p;
+  float r2 = fmaxf(dot(delta, delta), 0.0025f);
+  v3 wi = delta * rsqrtf(r2);
+  cnt.add(C_BRDF, 1);
+  v3 b = brdf(s, normalize(camera - s.p), wi);
+  float k1 = fmaxf(dot(s.n, wi), 0.0f);
+  float k2 = fmaxf(dot(ld3(v.normal), -wi), 0.0f);
+  float den = 4.0f * IRT_PI * r2;
+  return v3{b.x * k1 * k2 * v.flux[0] / den, b.y * k1 * k2 * v.flux[1] / den, b.z * k1 * k2 * v.flux[2] / den};
+}
+
+// ------------------------------------------------------------- reservoirs
+// empty_reservoir, normalization, add — common.wgsl:98-106; cap — lighting.wgsl:51.
+__device__ __forceinline__ irt_reservoir empty_reservoir(uint32_t frame_index, uint32_t generation) {
+  irt_reservoir r;
+  r.id = 0u;
+  r.source = frame_index;
+  r.count = 0u;
+  r.age = 0u;
+  r.sum = 0.0f;
+  r.selected_target = 0.0f;
+  r.pdf = 0.0f;
+  r.generation = generation;
+  return r;
+}
+__device__ __forceinline__ float normalization(const irt_reservoir& r) {
+  if (r.count == 0u || r.selected_target <= 0.0f) return 0.0f;
+  return r.sum / ((float)r.count * r.selected_target);
+}
+// A NaN weight passes the range test (as in the reference) and poisons `sum`.
+__device__ __forceinline__ bool add(irt_reservoir& r, uint32_t id, uint32_t source, float target, float pdf,
+                                    float weight, uint32_t count, uint32_t age, uint32_t& rng) {
+  r.count += count;
+  if (weight <= 0.0f || weight > 1e30f) return false;
+  r.sum += weight;
+  if (rnd(rng) * r.sum < weight) {
+    r.id = id;
+    r.source = source;
+    r.selected_target = target;
+    r.pdf = pdf;
+    r.age = age;
+    return true;
+  }
+  return false;
+}
+__device__ __forceinline__ irt_reservoir cap(irt_reservoir a, uint32_t limit) {
+  if (a.count > limit) {
+    a.sum *= (float)limit / (float)a.count;
+    a.count = limit;
+  }
+  return a;
+}
+
+// --------------------------------------------------------------- SDF
+// Scene records as the kernels see them.
+struct SceneView {
+  const irt_object* objects;
+  const irt_mesh* meshes;
+  const irt_node* nodes;
+  const irt_light* lights;
+  const float* atlas;  // reference atlas layout: x + w*(y + h*z)
+  uint32_t atlas_w, atlas_h;
+};
+
+// signed_box_distance — sdf.wgsl:1-4.
+__device__ __forceinline__ float signed_box_distance(v3 p, v3 lo, v3 hi) {
+  v3 delta = vmax(lo - p, p - hi);
+  return length(vmaxs(delta, 0.0f)) + fminf(fmaxf(delta.x, fmaxf(delta.y, delta.z)), 0.0f);
+}
+
+// sample_field (manual 8-tap FP32 variant) — common.wgsl:31-38.
+__device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_t* origin, v3 q, uint32_t res) {
+  uint32_t cx = min(f2u_sat(floorf(q.x)), res - 2u);
+  uint32_t cy = min(f2u_sat(floorf(q.y)), res - 2u);
+  uint32_t cz = min(f2u_sat(floorf(q.z)), res - 2u);
+  float fx = q.x - (float)cx, fy = q.y - (float)cy, fz = q.z - (float)cz;
+  const uint32_t w = sc.atlas_w, h = sc.atlas_h;
+  const float* base = sc.atlas + ((size_t)(origin[2] + cz) * h + (origin[1] + cy)) * w + (origin[0] + cx);
+  const size_t sy = w, sz = (size_t)w * h;
+  float t000 = base[0], t100 = base[1], t010 = base[sy], t110 = base[sy + 1];
+  float t001 = base[sz], t101 = base[sz + 1], t011 = base[sz + sy], t111 = base[sz + sy + 1];
+  float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+  float value = 0.0f;
+  value += t000 * (gx * gy * gz);
+  value += t100 * (fx * gy * gz);
+  value += t010 * (gx * fy * gz);
+  value += t110 * (fx * fy * gz);
+  value += t001 * (gx * gy * fz);
+  value += t101 * (fx * gy * fz);
+  value += t011 * (gx * fy * fz);
+  value += t111 * (fx * fy * fz);
+  return value;
+}
+
+// mesh_distance — sdf.wgsl:5-33.
+template <class C>
+__device__ __forceinline__ float mesh_distance(const SceneView& sc, v3 p, const irt_mesh& mesh, C& cnt) {
+  v3 glo = ld3(mesh.geometry_lo), ghi = ld3(mesh.geometry_hi);
+  float geometry_bound = signed_box_distance(p, glo, ghi);
+  if (mesh.geometry_lo[3] > 0.5f) {
+    cnt.add(C_OBJ_BOX, 1);
+    return geometry_bound;
+  }
+  cnt.add(C_OBJ_GRID, 1);
+  uint32_t res = mesh.field[3];
+  v3 lo = ld3(mesh.lo), hi = ld3(mesh.hi);
+  v3 clamped = vmin(vmax(p, lo), hi);
+  v3 q = (clamped - lo) * ld3(mesh.grid);
+  float value = sample_field(sc, mesh.field, q, res);
+  float bound = value - mesh.grid[3] * (0.5f + 1.0f / 128.0f);
+  v3 outside = vmaxs(vmax(lo - p, p - hi), 0.0f);
+  float outside_squared = dot(outside, outside);
+  if (outside_squared > 0.0f) {
+    float positive = fmaxf(bound, 0.0f);
+    bound = sqrtf(positive * positive + outside_squared);
+  }
+  if (bound > 0.0f) bound *= mesh.geometry_hi[3];
+  float sphere_bound = length(p - ld3(mesh.enclosing_sphere)) - mesh.enclosing_sphere[3];
+  return fmaxf(bound, fmaxf(geometry_bound, sphere_bound));
+}
+
+// object_distance — sdf.wgsl:35-41.  Returns the distance; *voxel gets the spacing term.
+template <class C>
+__device__ __forceinline__ float object_distance(const SceneView& sc, v3 p, uint32_t id, bool need_spacing,
+                                                 float hit_eps, float* voxel, C& cnt) {
+  const irt_object& obj = sc.objects[id];
+  const irt_mesh& mesh = sc.meshes[obj.info[0]];
+  float4 l = mat_mul(obj.inverse, p.x, p.y, p.z, 1.0f);
+  float d = mesh_distance(sc, v3{l.x, l.y, l.z}, mesh, cnt) * __uint_as_float(obj.info[2]);
+  if (d > 0.0f) d *= 0.9999f;
+  *voxel = need_spacing ? __uint_as_float(obj.info[3]) : hit_eps;
+  return d;
+}
+
+struct Distance {
+  float distance, voxel;
+  uint32_t object_id;
+};
+
+// scene_distance — sdf.wgsl:42-60.  Stackless preorder walk with escape links.
+template <class C>
+__device__ __forceinline__ Distance scene_distance(const SceneView& sc, uint32_t n_objects, float hit_eps, v3 p,
+                                                   bool need_spacing, uint32_t hint, C& cnt) {
+  float best = 1e20f, voxel = hit_eps;
+  uint32_t closest = hint;
+  if (hint < n_objects) best = object_distance(sc, p, hint, need_spacing, hit_eps, &voxel, cnt);
+  const uint32_t n_nodes = n_objects * 2u - 1u;
+  uint32_t cursor = 0u;
+  while (cursor < n_nodes) {
+    const irt_node& node = sc.nodes[cursor];
+    cnt.add(C_NODE, 1);
+    v3 delta = vmaxs(vmax(ld3(node.lo) - p, p - ld3(node.hi)), 0.0f);
+    v3 lb = delta * 0.9998f;
+    float bp = fmaxf(best, 0.0f);
+    if (dot(lb, lb) > bp * bp) {
+      cursor = node.info[3];
+      continue;
+    }
+    if (node.info[2] == 1u && node.info[0] != hint) {
+      float vx;
+      float d = object_distance(sc, p, node.info[0], need_spacing, hit_eps, &vx, cnt);
+      if (d < best) {
+        best = d;
+        voxel = vx;
+        closest = node.info[0];
+      }
+    }
+    cursor++;
+  }
+  return Distance{best, voxel, closest};
+}
+
+// visibility — sdf.wgsl:61-81.  Conservative sphere-traced soft shadow.
+template <class C>
+__device__ __forceinline__ float visibility(const SceneView& sc, const irt_frame& F, v3 p, v3 n, v3 target,
+                                            v3 end_normal, C& cnt) {
+  const uint32_t n_objects = F.counts[3];
+  const float eps0 = F.sdf[1];
+  cnt.add(C_MARCH, 1);
+  Distance clearance = scene_distance(sc, n_objects, eps0, p, true, 0xffffffffu, cnt);
+  float epsilon = fmaxf(eps0, clearance.voxel * 0.65f);
+  uint32_t closest = clearance.object_id;
+  v3 origin = p + n * epsilon;
+  v3 endpoint = target;
+  if (dot(end_normal, end_normal) > 0.5f) {
+    Distance e = scene_distance(sc, n_objects, eps0, target, true, 0xffffffffu, cnt);
+    endpoint = endpoint + end_normal * fmaxf(eps0, e.voxel * 0.65f);
+  }
+  v3 delta = endpoint - origin;
+  float limit = length(delta) - eps0;
+  if (limit <= 0.0f) return 1.0f;
+  v3 direction = normalize(delta);
+  float t = eps0, result = 1.0f, previous = 1e20f;
+  const uint32_t steps = f2u_sat(F.sdf[0]);
+  const float k = F.sdf[2];
+  for (uint32_t i = 0; i < steps; ++i) {
+    if (t >= limit) return clampf(result, 0.0f, 1.0f);
+    cnt.add(C_STEP, 1);
+    Distance qd = scene_distance(sc, n_objects, eps0, origin + direction * t, false, closest, cnt);
+    float h = qd.distance;
+    closest = qd.object_id;
+    if (h < eps0) return 0.0f;
+    float y = h * h / (2.0f * previous);
+    float d = sqrtf(fmaxf(h * h - y * y, 0.0f));
+    if (t > y) result = fminf(result, k * d / (t - y));
+    if (result < 0.001f) return 0.0f;
+    previous = h;
+    t += h;
+  }
+  return 0.0f;
+}
+
+}  // namespace irt

@@ -1,0 +1,518 @@
+// Per-pixel bodies of the eight ReSTIR/resolve passes and the three SVGF
+// passes.  Kernels (irt_kernels.hip) call these one per launch (the
+// reference's pass schedule) or several per launch (fused schedule).
+#pragma once
+
+#include "irt_device.h"
+
+namespace irt {
+
+// Everything a lighting pass reads or writes.  The frame record travels in
+// the kernel-argument segment, so its fields are scalar loads.
+template <class P>
+struct LightArgs {
+  irt_frame F;
+  SceneView sc;
+  const float* depth;
+  const typename P::N4* normal;
+  const typename P::N4* color;
+  const typename P::P2* params;
+  const typename P::P2* motion;
+  const float* prev_depth;
+  const typename P::N4* prev_normal;
+  const irt_reservoir* direct_in;
+  const irt_reservoir* indirect_in;
+  irt_reservoir* direct_out;
+  irt_reservoir* indirect_out;
+  const irt_reservoir* direct_history;
+  const irt_reservoir* indirect_history;
+  const irt_vpl* vpl_ring;  // (restir_max_history + 1) pools of vpl_count records
+  typename P::L4* direct_lighting;
+  typename P::R4* lighting;
+  unsigned long long* counters;
+};
+
+template <class P>
+__device__ __forceinline__ Surface load_surface(const LightArgs<P>& a, int32_t x, int32_t y) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  float2 rm = ld2(a.params, id);
+  Surface s;
+  s.p = world(pixel_u(x, W), pixel_u(y, H), a.depth[id], a.F.inv_vp);
+  s.n = normalize(xyz(ld4(a.normal, id)));
+  s.color = xyz(ld4(a.color, id));
+  s.rough = rm.x;
+  s.metal = rm.y;
+  return s;
+}
+
+// vpl_sample_valid — lighting.wgsl:22-24.
+__device__ __forceinline__ bool vpl_sample_valid(const irt_frame& F, const irt_reservoir& r) {
+  return r.generation == F.size[3] && r.selected_target > 0.0f && r.id < F.counts[1] &&
+         F.size[2] - r.source <= f2u_sat(F.rr[2]);
+}
+// selected_vpl — lighting.wgsl:25-29.  The current and previous pools are
+// the ring slots of this and the previous frame, so one ring lookup serves
+// all three cases of the reference.
+__device__ __forceinline__ const irt_vpl& selected_vpl(const irt_frame& F, const irt_vpl* ring,
+                                                      const irt_reservoir& r) {
+  const uint32_t slots = f2u_sat(F.rr[2]) + 1u;
+  return ring[(r.source % slots) * F.counts[1] + r.id];
+}
+
+// compatible — lighting.wgsl:36-40.
+template <class P>
+__device__ __forceinline__ bool compatible(const LightArgs<P>& a, const Surface& s, int32_t qx, int32_t qy) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+  float d = a.depth[q];
+  if (d >= 1.0f) return false;
+  v3 p = world(pixel_u(qx, W), pixel_u(qy, H), d, a.F.inv_vp);
+  v3 n = normalize(xyz(ld4(a.normal, q)));
+  v3 cam = ld3(a.F.camera);
+  return dot(s.n, n) > a.F.temporal[2] && length(s.p - p) < a.F.temporal[1] * fmaxf(length(s.p - cam), 1.0f);
+}
+
+// reproject — lighting.wgsl:41-44.
+template <class P>
+__device__ __forceinline__ void reproject(const LightArgs<P>& a, int32_t x, int32_t y, int32_t* qx, int32_t* qy) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  float2 m = ld2(a.motion, (uint32_t)y * W + (uint32_t)x);
+  float u = pixel_u(x, W) - m.x, v = pixel_u(y, H) - m.y;
+  *qx = f2i_sat(floorf(u * (float)W));
+  *qy = f2i_sat(floorf(v * (float)H));
+}
+
+// history_valid — lighting.wgsl:45-50.
+template <class P>
+__device__ __forceinline__ bool history_valid(const LightArgs<P>& a, const Surface& s, int32_t qx, int32_t qy) {
+  const irt_frame& F = a.F;
+  const int32_t W = (int32_t)F.size[0], H = (int32_t)F.size[1];
+  if (F.flags[0] == 0u || F.flags[1] != F.size[3] || qx < 0 || qy < 0 || qx >= W || qy >= H) return false;
+  const uint32_t q = (uint32_t)qy * (uint32_t)W + (uint32_t)qx;
+  float d = a.prev_depth[q];
+  if (d >= 1.0f) return false;
+  v3 oldp = world(pixel_u(qx, W), pixel_u(qy, H), d, F.previous_inv_vp);
+  v3 cam = ld3(F.camera);
+  return dot(s.n, xyz(ld4(a.prev_normal, q))) > F.temporal[2] &&
+         length(s.p - oldp) < F.temporal[1] * fmaxf(length(s.p - cam), 1.0f);
+}
+
+// candidates_direct — lighting.wgsl:52-58.
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir candidates_direct(const LightArgs<P>& a, uint32_t id, bool valid,
+                                                            const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  irt_reservoir r = empty_reservoir(F.size[2], F.size[3]);
+  uint32_t rng = seed(id, 11u, F.size[2]);
+  if (valid && F.counts[0] > 0u) {
+    const v3 cam = ld3(F.camera);
+    const float pdf = 1.0f / (float)F.counts[0];
+    for (uint32_t i = 0; i < F.restir[0]; ++i) {
+      uint32_t light_id = min(f2u_sat(rnd(rng) * (float)F.counts[0]), F.counts[0] - 1u);
+      float target = luminance(direct(s, a.sc.lights[light_id], cam, cnt));
+      add(r, light_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
+    }
+  }
+  return r;
+}
+
+// candidates_indirect — lighting.wgsl:59-65.
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir candidates_indirect(const LightArgs<P>& a, uint32_t id, bool valid,
+                                                              const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  irt_reservoir r = empty_reservoir(F.size[2], F.size[3]);
+  uint32_t rng = seed(id, 13u, F.size[2]);
+  if (valid) {
+    const v3 cam = ld3(F.camera);
+    const float pdf = 1.0f / (float)F.counts[1];
+    const uint32_t slots = f2u_sat(F.rr[2]) + 1u;
+    const irt_vpl* current = a.vpl_ring + (size_t)(F.size[2] % slots) * F.counts[1];
+    for (uint32_t i = 0; i < F.restir[1]; ++i) {
+      uint32_t vpl_id = min(f2u_sat(rnd(rng) * (float)F.counts[1]), F.counts[1] - 1u);
+      float target = luminance(indirect(s, current[vpl_id], cam, cnt));
+      add(r, vpl_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
+    }
+  }
+  return r;
+}
+
+// temporal_direct — lighting.wgsl:66-74.  `hv`: valid && history_valid, `q`: reprojected id.
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir temporal_direct(const LightArgs<P>& a, irt_reservoir r, uint32_t id,
+                                                          bool hv, uint32_t q, const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  uint32_t rng = seed(id, 17u, F.size[2]);
+  if (hv) {
+    const uint32_t hist = f2u_sat(F.rr[2]);
+    irt_reservoir prev = cap(a.direct_history[q], hist * F.restir[0]);
+    if (prev.generation == F.size[3] && prev.selected_target > 0.0f && prev.id < F.counts[0] && prev.age < hist) {
+      float target = luminance(direct(s, a.sc.lights[prev.id], ld3(F.camera), cnt));
+      add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
+          prev.age + 1u, rng);
+    }
+  }
+  return r;
+}
+
+// temporal_indirect — lighting.wgsl:75-85.
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir temporal_indirect(const LightArgs<P>& a, irt_reservoir r, uint32_t id,
+                                                            bool hv, uint32_t q, const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  uint32_t rng = seed(id, 19u, F.size[2]);
+  if (hv) {
+    irt_reservoir prev = cap(a.indirect_history[q], f2u_sat(F.rr[2]) * F.restir[1]);
+    if (vpl_sample_valid(F, prev)) {
+      const irt_vpl& v = selected_vpl(F, a.vpl_ring, prev);
+      float target = luminance(indirect(s, v, ld3(F.camera), cnt));
+      add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
+          prev.age + 1u, rng);
+    }
+  }
+  return r;
+}
+
+// neighbor — lighting.wgsl:86-89 (round() is half-to-even: rintf).
+__device__ __forceinline__ void neighbor(const irt_frame& F, int32_t x, int32_t y, uint32_t& rng, int32_t* qx,
+                                         int32_t* qy) {
+  float angle = (2.0f * IRT_PI) * rnd(rng);
+  float radius = sqrtf(rnd(rng)) * (float)F.restir[3];
+  float sn, cs;
+  sincosf(angle, &sn, &cs);
+  *qx = x + f2i_sat(rintf(cs * radius));
+  *qy = y + f2i_sat(rintf(sn * radius));
+}
+
+// spatial_direct — lighting.wgsl:90-97.  `r` is direct_in[id].
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir spatial_direct(const LightArgs<P>& a, irt_reservoir r, int32_t x, int32_t y,
+                                                         bool valid, const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  const int32_t W = (int32_t)F.size[0], H = (int32_t)F.size[1];
+  const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+  uint32_t rng = seed(id, 23u, F.size[2]);
+  if (valid) {
+    const v3 cam = ld3(F.camera);
+    for (uint32_t i = 0; i < F.restir[2]; ++i) {
+      int32_t qx, qy;
+      neighbor(F, x, y, rng, &qx, &qy);
+      if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+      if (!compatible(a, s, qx, qy)) continue;
+      irt_reservoir prev = a.direct_in[(uint32_t)qy * (uint32_t)W + (uint32_t)qx];
+      // The reference indexes lights[prev.id] unchecked; ids written this frame
+      // are always < counts.x, and an out-of-range id (injected data) is skipped.
+      if (prev.selected_target > 0.0f && prev.id < F.counts[0]) {
+        float target = luminance(direct(s, a.sc.lights[prev.id], cam, cnt));
+        add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
+            prev.age, rng);
+      }
+    }
+  }
+  return cap(r, f2u_sat(F.rr[2]) * F.restir[0] * 8u);
+}
+
+// spatial_indirect — lighting.wgsl:98-105.
+template <class P, class C>
+__device__ __forceinline__ irt_reservoir spatial_indirect(const LightArgs<P>& a, irt_reservoir r, int32_t x,
+                                                           int32_t y, bool valid, const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  const int32_t W = (int32_t)F.size[0], H = (int32_t)F.size[1];
+  const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+  uint32_t rng = seed(id, 29u, F.size[2]);
+  if (valid) {
+    const v3 cam = ld3(F.camera);
+    for (uint32_t i = 0; i < F.restir[2]; ++i) {
+      int32_t qx, qy;
+      neighbor(F, x, y, rng, &qx, &qy);
+      if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+      if (!compatible(a, s, qx, qy)) continue;
+      irt_reservoir prev = a.indirect_in[(uint32_t)qy * (uint32_t)W + (uint32_t)qx];
+      if (vpl_sample_valid(F, prev)) {
+        float target = luminance(indirect(s, selected_vpl(F, a.vpl_ring, prev), cam, cnt));
+        add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
+            prev.age, rng);
+      }
+    }
+  }
+  return cap(r, f2u_sat(F.rr[2]) * F.restir[1] * 8u);
+}
+
+// resolve_direct — lighting.wgsl:106-122.  `d` is the final direct reservoir.
+template <class P, class C>
+__device__ __forceinline__ v3 resolve_direct(const LightArgs<P>& a, const irt_reservoir& d, uint32_t id, bool valid,
+                                             const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  v3 result = v3{0.f, 0.f, 0.f};
+  if (valid) {
+    const uint32_t mode = F.flags[2];
+    if (d.selected_target > 0.0f && d.id < F.counts[0] && mode != 2u && mode < 3u) {
+      const irt_light& l = a.sc.lights[d.id];
+      v3 value = direct(s, l, ld3(F.camera), cnt);
+      if (luminance(value) > 0.0f) {
+        float w = normalization(d);
+        float vis = visibility(a.sc, F, s.p, s.n, ld3(l.position_strength), v3{0.f, 0.f, 0.f}, cnt);
+        result = result + value * w * vis;
+      }
+    }
+    if (mode == 3u) result = s.color;
+    if (mode == 4u) result = s.n * 0.5f + v3{0.5f, 0.5f, 0.5f};
+    if (mode == 5u) {
+      float2 m = ld2(a.motion, id);
+      result = v3{clampf(0.5f + m.x * 50.0f, 0.0f, 1.0f), clampf(0.5f + m.y * 50.0f, 0.0f, 1.0f), 0.5f};
+    }
+  }
+  return result;
+}
+
+// resolve_indirect — lighting.wgsl:123-134.  `base` is direct_lighting[id].rgb.
+template <class P, class C>
+__device__ __forceinline__ v3 resolve_indirect(const LightArgs<P>& a, const irt_reservoir& v, v3 base, bool valid,
+                                               const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  v3 result = base;
+  if (valid && F.flags[2] != 1u && F.flags[2] < 3u) {
+    if (vpl_sample_valid(F, v)) {
+      const irt_vpl& smp = selected_vpl(F, a.vpl_ring, v);
+      v3 value = indirect(s, smp, ld3(F.camera), cnt);
+      if (luminance(value) > 0.0f) {
+        float w = normalization(v);
+        float vis = visibility(a.sc, F, s.p, s.n, ld3(smp.position), ld3(smp.normal), cnt);
+        result = result + value * w * vis;
+      }
+    }
+  }
+  return vmin(result, v3{65000.f, 65000.f, 65000.f});
+}
+
+// ----------------------------------------------------------------- SVGF
+template <class P>
+struct SvgfArgs {
+  irt_frame F;
+  const float* depth;
+  const typename P::N4* normal;
+  const typename P::N4* color;
+  const typename P::P2* params;
+  const typename P::P2* motion;
+  const float* old_depth;
+  const typename P::N4* old_normal;
+  const typename P::R4* noisy;
+  const typename P::R4* old_radiance;
+  const float4* old_moments;
+  const typename P::R4* input;
+  const float4* moments;
+  typename P::R4* out;
+  typename P::R4* out2;  // optional second destination (feedback / final copy)
+  float4* moments_out;
+  uint32_t stride;
+  unsigned long long* counters;
+};
+
+// inside, position, history_ok, guide, geometry_weight — svgf.wgsl:18-44.
+struct Guide {
+  v3 position, normal, color;
+  float m0, m1, pixel_scale;
+};
+
+template <class P>
+__device__ __forceinline__ bool inside(const SvgfArgs<P>& a, int32_t x, int32_t y) {
+  return x >= 0 && y >= 0 && x < (int32_t)a.F.size[0] && y < (int32_t)a.F.size[1];
+}
+template <class P>
+__device__ __forceinline__ v3 position(const SvgfArgs<P>& a, int32_t x, int32_t y) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  return world(pixel_u(x, W), pixel_u(y, H), a.depth[(uint32_t)y * W + (uint32_t)x], a.F.inv_vp);
+}
+template <class P>
+__device__ __forceinline__ bool history_ok(const SvgfArgs<P>& a, int32_t px, int32_t py, int32_t qx, int32_t qy) {
+  const irt_frame& F = a.F;
+  if (F.flags[0] == 0u || F.flags[1] != F.size[3] || !inside(a, qx, qy)) return false;
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+  float d = a.old_depth[q];
+  if (d >= 1.0f) return false;
+  v3 pa = position(a, px, py);
+  v3 pb = world(pixel_u(qx, W), pixel_u(qy, H), d, F.previous_inv_vp);
+  v3 cam = ld3(F.camera);
+  return length(pa - pb) < F.temporal[1] * fmaxf(length(pa - cam), 1.0f) &&
+         dot(xyz(ld4(a.normal, (uint32_t)py * W + (uint32_t)px)), xyz(ld4(a.old_normal, q))) > F.temporal[2];
+}
+template <class P>
+__device__ __forceinline__ Guide guide(const SvgfArgs<P>& a, int32_t x, int32_t y) {
+  const uint32_t W = a.F.size[0];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  Guide g;
+  g.position = position(a, x, y);
+  g.normal = xyz(ld4(a.normal, id));
+  g.color = xyz(ld4(a.color, id));
+  float2 rm = ld2(a.params, id);
+  g.m0 = rm.x;
+  g.m1 = rm.y;
+  g.pixel_scale = length(g.position - ld3(a.F.camera)) / (float)a.F.size[1];
+  return g;
+}
+template <class P, class C>
+__device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Guide& g, int32_t qx, int32_t qy,
+                                                 float stride, C& cnt) {
+  cnt.add(C_TAP, 1);
+  if (!inside(a, qx, qy)) return 0.0f;
+  const uint32_t W = a.F.size[0];
+  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+  float d = a.depth[q];
+  if (d >= 1.0f) return 0.0f;
+  v3 nq = xyz(ld4(a.normal, q));
+  float wn = pow64(fmaxf(dot(g.normal, nq), 0.0f));
+  v3 wq = world(pixel_u(qx, W), pixel_u(qy, a.F.size[1]), d, a.F.inv_vp);
+  float tolerance = fmaxf(0.0001f, g.pixel_scale * stride * 0.5f);
+  float wz = __expf(-fabsf(dot(wq - g.position, g.normal)) / tolerance);
+  v3 cq = xyz(ld4(a.color, q));
+  float2 rm = ld2(a.params, q);
+  float dm0 = g.m0 - rm.x, dm1 = g.m1 - rm.y;
+  float wc = __expf(-length(g.color - cq) * 8.0f - sqrtf(dm0 * dm0 + dm1 * dm1) * 8.0f);
+  return wn * wz * wc;
+}
+
+// svgf_temporal — svgf.wgsl:45-76.
+template <class P, class C>
+__device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, int32_t y, float4* rad_out,
+                                              float4* mom_out, C& cnt) {
+  const irt_frame& F = a.F;
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  v3 c = xyz(ld4(a.noisy, id));
+  float l = luminance(c);
+  v3 history = v3{0.f, 0.f, 0.f};
+  float hm0 = 0.f, hm1 = 0.f, length_history = 0.f, total = 0.f;
+  if (a.depth[id] < 1.0f) {
+    float2 m = ld2(a.motion, id);
+    float px = (pixel_u(x, W) - m.x) * (float)W - 0.5f;
+    float py = (pixel_u(y, H) - m.y) * (float)H - 0.5f;
+    float flx = floorf(px), fly = floorf(py);
+    int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    float fx = px - flx, fy = py - fly;
+    for (int32_t j = 0; j < 2; ++j) {
+      for (int32_t i = 0; i < 2; ++i) {
+        int32_t qx = bx + i, qy = by + j;
+        if (!history_ok(a, x, y, qx, qy)) continue;
+        const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+        float4 mo = a.old_moments[q];
+        if (mo.z <= 0.0f) continue;
+        float wx = i == 1 ? fx : 1.0f - fx;
+        float wy = j == 1 ? fy : 1.0f - fy;
+        float w = wx * wy;
+        history = history + xyz(ld4(a.old_radiance, q)) * w;
+        hm0 += mo.x * w;
+        hm1 += mo.y * w;
+        length_history += mo.z * w;
+        total += w;
+      }
+    }
+  }
+  float age = 1.0f;
+  v3 result = c;
+  float m0 = l, m1 = l * l;
+  if (total > 0.001f) {
+    history = history / total;
+    hm0 /= total;
+    hm1 /= total;
+    age = fminf(length_history / total + 1.0f, F.temporal[3]);
+    float lo = 1e20f, hi = 0.0f;
+    Guide g = guide(a, x, y);
+    for (int32_t j = -1; j <= 1; ++j) {
+      for (int32_t i = -1; i <= 1; ++i) {
+        int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+        int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+        if (geometry_weight(a, g, qx, qy, 1.0f, cnt) > 0.01f) {
+          float v = luminance(xyz(ld4(a.noisy, (uint32_t)qy * W + (uint32_t)qx)));
+          lo = fminf(lo, v);
+          hi = fmaxf(hi, v);
+        }
+      }
+    }
+    float lh = luminance(history);
+    history = history * (clampf(lh, lo * 0.5f, hi * 1.5f + 0.001f) / fmaxf(lh, 0.000001f));
+    float alpha = fmaxf(1.0f - F.temporal[0], 1.0f / age);
+    result = history * (1.0f - alpha) + c * alpha;
+    float beta = fmaxf(0.2f, 1.0f / age);
+    m0 = hm0 * (1.0f - beta) + m0 * beta;
+    m1 = hm1 * (1.0f - beta) + m1 * beta;
+  }
+  float variance = fmaxf(m1 - m0 * m0, 0.0f);
+  *rad_out = make_float4(result.x, result.y, result.z, fminf(sqrtf(variance), 65000.0f));
+  *mom_out = make_float4(m0, m1, age, 0.0f);
+}
+
+// svgf_variance — svgf.wgsl:77-89.
+template <class P, class C>
+__device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x, int32_t y, C& cnt) {
+  const uint32_t W = a.F.size[0];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  float4 c = ld4(a.input, id);
+  float age = a.moments[id].z;
+  float sigma = c.w;
+  if (age < 4.0f && a.depth[id] < 1.0f) {
+    float s0 = 0.f, s1 = 0.f, total = 0.f;
+    Guide g = guide(a, x, y);
+    for (int32_t j = -3; j <= 3; ++j) {
+      for (int32_t i = -3; i <= 3; ++i) {
+        int32_t qx = x + i, qy = y + j;
+        float w = geometry_weight(a, g, qx, qy, 1.0f, cnt);
+        if (w <= 0.0f) continue;
+        float4 mo = a.moments[(uint32_t)qy * W + (uint32_t)qx];
+        s0 += mo.x * w;
+        s1 += mo.y * w;
+        total += w;
+      }
+    }
+    float t = fmaxf(total, 0.00001f);
+    s0 /= t;
+    s1 /= t;
+    sigma = sqrtf(fmaxf(s1 - s0 * s0, 0.0f) * 4.0f / fmaxf(age, 1.0f));
+  }
+  return make_float4(c.x, c.y, c.z, fminf(sigma, 65000.0f));
+}
+
+// kernel, svgf_atrous — svgf.wgsl:90-110.
+__device__ __forceinline__ float bspline(int32_t x) { return x == 0 ? 1.0f : ((x == 1 || x == -1) ? 2.0f / 3.0f : 1.0f / 6.0f); }
+
+template <class P, class C>
+__device__ __forceinline__ float4 svgf_atrous(const SvgfArgs<P>& a, int32_t x, int32_t y, C& cnt) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  float4 c = ld4(a.input, id);
+  if (a.depth[id] >= 1.0f) return c;
+  float variance = 0.f, guide_weight = 0.f;
+  Guide g = guide(a, x, y);
+  for (int32_t j = -1; j <= 1; ++j) {
+    for (int32_t i = -1; i <= 1; ++i) {
+      int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+      int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+      float w = (i == 0 ? 2.0f : 1.0f) * (j == 0 ? 2.0f : 1.0f);
+      float s = ld4(a.input, (uint32_t)qy * W + (uint32_t)qx).w;
+      variance += s * s * w;
+      guide_weight += w;
+    }
+  }
+  float phi = 4.0f * sqrtf(variance / fmaxf(guide_weight, 0.00001f)) + 0.0001f;
+  float lc = luminance(xyz(c));
+  v3 sum = v3{0.f, 0.f, 0.f};
+  float total = 0.f, v = 0.f;
+  const int32_t step = (int32_t)a.stride;
+  for (int32_t j = -2; j <= 2; ++j) {
+    for (int32_t i = -2; i <= 2; ++i) {
+      int32_t qx = x + i * step, qy = y + j * step;
+      float geom = geometry_weight(a, g, qx, qy, (float)step, cnt);
+      if (geom <= 0.0f) continue;
+      float4 smp = ld4(a.input, (uint32_t)qy * W + (uint32_t)qx);
+      float w = bspline(i) * bspline(j) * geom * __expf(-fabsf(luminance(xyz(smp)) - lc) / phi);
+      sum = sum + xyz(smp) * w;
+      v += smp.w * smp.w * w * w;
+      total += w;
+    }
+  }
+  float t = fmaxf(total, 0.00001f);
+  return make_float4(sum.x / t, sum.y / t, sum.z / t, fminf(sqrtf(v) / t, 65000.0f));
+}
+
+}  // namespace irt

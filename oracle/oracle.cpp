@@ -1,0 +1,973 @@
+// CPU restatement of the reference WGSL (test infrastructure; see oracle.h).
+//
+// Every function follows one WGSL function of the reference line by line;
+// the file:line is given beside it.  Scalar code, no vector intrinsics, built
+// with -ffp-contract=off so FP32 arithmetic is one IEEE rounding per WGSL
+// operation in source order.
+#include "oracle.h"
+
+#include <omp.h>
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+template <class R>
+struct Vec {
+  R x, y, z;
+};
+template <class R>
+Vec<R> V(R x, R y, R z) {
+  return Vec<R>{x, y, z};
+}
+template <class R>
+Vec<R> add3(Vec<R> a, Vec<R> b) { return V<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R>
+Vec<R> sub3(Vec<R> a, Vec<R> b) { return V<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R>
+Vec<R> mul3(Vec<R> a, Vec<R> b) { return V<R>(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class R>
+Vec<R> scale3(Vec<R> a, R s) { return V<R>(a.x * s, a.y * s, a.z * s); }
+template <class R>
+Vec<R> div3(Vec<R> a, R s) { return V<R>(a.x / s, a.y / s, a.z / s); }
+template <class R>
+R dot3(Vec<R> a, Vec<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R>
+R length3(Vec<R> a) { return std::sqrt(dot3(a, a)); }
+template <class R>
+Vec<R> normalize3(Vec<R> a) { return div3(a, length3(a)); }  // WGSL: e / length(e)
+template <class R>
+R maxr(R a, R b) { return a > b ? a : b; }  // operands never NaN where used
+template <class R>
+R minr(R a, R b) { return a < b ? a : b; }
+template <class R>
+Vec<R> max3(Vec<R> a, Vec<R> b) { return V<R>(maxr(a.x, b.x), maxr(a.y, b.y), maxr(a.z, b.z)); }
+template <class R>
+Vec<R> min3(Vec<R> a, Vec<R> b) { return V<R>(minr(a.x, b.x), minr(a.y, b.y), minr(a.z, b.z)); }
+template <class R>
+R clampr(R e, R lo, R hi) { return minr(maxr(e, lo), hi); }  // WGSL clamp = min(max(e, lo), hi)
+template <class R>
+Vec<R> F3(const float* p) { return V<R>((R)p[0], (R)p[1], (R)p[2]); }
+template <class R>
+Vec<R> D3(const double* p) { return V<R>((R)p[0], (R)p[1], (R)p[2]); }
+
+// WGSL saturating conversions.
+template <class R>
+uint32_t to_u32(R x) {
+  if (!(x > R(0))) return 0u;
+  if (x >= R(4294967295.0)) return 4294967295u;
+  return (uint32_t)x;
+}
+template <class R>
+int32_t to_i32(R x) {
+  if (x != x) return 0;
+  if (x >= R(2147483647.0)) return 2147483647;
+  if (x <= R(-2147483648.0)) return (int32_t)(-2147483647 - 1);
+  return (int32_t)x;
+}
+
+uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+  if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
+  if (ax < 0x38800000u) {
+    if (ax < 0x33000000u) return (uint16_t)sign;
+    const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+    const int shift = 126 - (int)(ax >> 23);
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t q = (((ax >> 23) - 112u) << 10) | ((ax & 0x7fffffu) >> 13);
+  const uint32_t rem = ax & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
+  return (uint16_t)(sign | q);
+}
+float f16_to_f32(uint16_t h) {
+  const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float v;
+  if (e == 0)
+    v = std::ldexp((float)m, -24);
+  else if (e == 31)
+    v = m ? NAN : INFINITY;
+  else
+    v = std::ldexp((float)(m | 0x400u), (int)e - 25);
+  return (h & 0x8000u) ? -v : v;
+}
+
+// Storage rounding of an Rgba16F target: binary16 for FP32, none for FP64.
+inline float store16(float v) { return f16_to_f32(f32_to_f16(v)); }
+inline double store16(double v) { return v; }
+
+// random — common.wgsl:39-43 (the FP32 value, also used by the FP64 instance).
+float random_f32(uint32_t* s) {
+  *s = *s * 747796405u + 2891336453u;
+  const uint32_t w = ((*s >> ((*s >> 28u) + 4u)) ^ *s) * 277803737u;
+  return (float)((w >> 22u) ^ w) * (1.0f / 4294967296.0f);
+}
+// seed — common.wgsl:44.
+uint32_t seed_of(uint32_t i, uint32_t salt, uint32_t frame) { return i * 1973u + frame * 9277u + salt * 26699u + 89173u; }
+
+template <class R>
+struct Res {
+  uint32_t id, source, count, age;
+  R sum, selected_target, pdf;
+  uint32_t generation;
+};
+template <class R>
+Res<R> from_io(const oracle_reservoir& o) {
+  return Res<R>{o.id, o.source, o.count, o.age, (R)o.sum, (R)o.selected_target, (R)o.pdf, o.generation};
+}
+template <class R>
+oracle_reservoir to_io(const Res<R>& r) {
+  oracle_reservoir o;
+  o.id = r.id;
+  o.source = r.source;
+  o.count = r.count;
+  o.age = r.age;
+  o.sum = (double)r.sum;
+  o.selected_target = (double)r.selected_target;
+  o.pdf = (double)r.pdf;
+  o.generation = r.generation;
+  o.pad = 0;
+  return o;
+}
+
+template <class R>
+struct Surface {
+  Vec<R> p, n, color;
+  R rough, metal;
+};
+
+template <class R>
+struct Oracle {
+  const irt_frame& F;
+  const oracle_scene* sc;
+  const oracle_gbuffer* gb;
+  oracle_io* io;
+  uint64_t cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  const R PI = R(3.14159265359f);  // const PI: f32 (common.wgsl:2)
+
+  Oracle(const irt_frame& f, const oracle_scene* s, const oracle_gbuffer* g, oracle_io* i) : F(f), sc(s), gb(g), io(i) {}
+
+  uint32_t W() const { return F.size[0]; }
+  uint32_t H() const { return F.size[1]; }
+  R random(uint32_t* s) { return (R)random_f32(s); }
+
+  // mat4 * vec4, column-major, row by row in column order.
+  void mat_vec(const float* m, R x, R y, R z, R w, R out[4]) const {
+    for (int r = 0; r < 4; ++r) out[r] = (R)m[r] * x + (R)m[4 + r] * y + (R)m[8 + r] * z + (R)m[12 + r] * w;
+  }
+  R luminance(Vec<R> c) const { return c.x * R(0.2126f) + c.y * R(0.7152f) + c.z * R(0.0722f); }  // common.wgsl:45
+  Vec<R> world(R u, R v, R depth, const float* m) const {  // common.wgsl:46
+    R h[4];
+    mat_vec(m, u * R(2) - R(1), R(1) - v * R(2), depth, R(1), h);
+    return V<R>(h[0] / h[3], h[1] / h[3], h[2] / h[3]);
+  }
+  R pixel_u(int64_t x) const { return ((R)x + R(0.5f)) / (R)W(); }  // common.wgsl:47
+  R pixel_v(int64_t y) const { return ((R)y + R(0.5f)) / (R)H(); }
+  Vec<R> camera() const { return F3<R>(F.camera); }
+
+  // brdf — common.wgsl:67-74.
+  Vec<R> brdf(const Surface<R>& s, Vec<R> wo, Vec<R> wi) {
+    cnt[0]++;
+    const R nv = maxr(dot3(s.n, wo), R(0.00001f));
+    const R nl = maxr(dot3(s.n, wi), R(0));
+    if (nl <= R(0)) return V<R>(0, 0, 0);
+    const Vec<R> h = normalize3(add3(wo, wi));
+    const R nh = maxr(dot3(s.n, h), R(0));
+    const R vh = maxr(dot3(wo, h), R(0));
+    const R a = s.rough * s.rough;
+    const R a2 = a * a;
+    const R q = nh * nh * (a2 - R(1)) + R(1);
+    const R d = a2 / (PI * q * q);
+    const R g1v = R(2) * nv / (nv + std::sqrt(a2 + (R(1) - a2) * nv * nv));
+    const R g1l = R(2) * nl / (nl + std::sqrt(a2 + (R(1) - a2) * nl * nl));
+    // mix(vec3(.04), color, metal) = .04*(1-metal) + color*metal
+    const R c04 = R(0.04f);
+    const Vec<R> f0 = V<R>(c04 * (R(1) - s.metal) + s.color.x * s.metal, c04 * (R(1) - s.metal) + s.color.y * s.metal,
+                           c04 * (R(1) - s.metal) + s.color.z * s.metal);
+    const R p5 = std::pow(R(1) - vh, R(5));
+    const Vec<R> f = V<R>(f0.x + (R(1) - f0.x) * p5, f0.y + (R(1) - f0.y) * p5, f0.z + (R(1) - f0.z) * p5);
+    const R den = maxr(R(4) * nv * nl, R(0.00001f));
+    Vec<R> out;
+    const R fs[3] = {f.x, f.y, f.z}, cs[3] = {s.color.x, s.color.y, s.color.z};
+    R o[3];
+    for (int k = 0; k < 3; ++k) o[k] = (R(1) - fs[k]) * (R(1) - s.metal) * cs[k] / PI + fs[k] * d * g1v * g1l / den;
+    out = V<R>(o[0], o[1], o[2]);
+    return out;
+  }
+  // direct — common.wgsl:75-79.
+  Vec<R> direct(const Surface<R>& s, uint32_t id) {
+    const irt_light& l = sc->lights[id];
+    const Vec<R> delta = sub3(F3<R>(l.position_strength), s.p);
+    const R r2 = maxr(dot3(delta, delta), R(0.0001f));
+    const Vec<R> wi = scale3(delta, R(1) / std::sqrt(r2));
+    if ((R)l.color_type[3] > R(0.5f) &&
+        dot3(scale3(wi, R(-1)), normalize3(F3<R>(l.direction_angle))) < std::cos((R)l.direction_angle[3] * R(0.5f)))
+      return V<R>(0, 0, 0);
+    const Vec<R> b = brdf(s, normalize3(sub3(camera(), s.p)), wi);
+    const R k = maxr(dot3(s.n, wi), R(0));
+    const R c[3] = {(R)l.color_type[0], (R)l.color_type[1], (R)l.color_type[2]};
+    const R bb[3] = {b.x, b.y, b.z};
+    R o[3];
+    for (int i = 0; i < 3; ++i) o[i] = bb[i] * k * c[i] * (R)l.position_strength[3] / r2;
+    return V<R>(o[0], o[1], o[2]);
+  }
+  // indirect — common.wgsl:80-83.
+  Vec<R> indirect(const Surface<R>& s, const irt_vpl& v) {
+    const Vec<R> delta = sub3(F3<R>(v.position), s.p);
+    const R r2 = maxr(dot3(delta, delta), R(0.0025f));
+    const Vec<R> wi = scale3(delta, R(1) / std::sqrt(r2));
+    const Vec<R> b = brdf(s, normalize3(sub3(camera(), s.p)), wi);
+    const R k1 = maxr(dot3(s.n, wi), R(0));
+    const R k2 = maxr(dot3(F3<R>(v.normal), scale3(wi, R(-1))), R(0));
+    const R den = R(4) * PI * r2;
+    const R bb[3] = {b.x, b.y, b.z}, fl[3] = {(R)v.flux[0], (R)v.flux[1], (R)v.flux[2]};
+    R o[3];
+    for (int i = 0; i < 3; ++i) o[i] = bb[i] * k1 * k2 * fl[i] / den;
+    return V<R>(o[0], o[1], o[2]);
+  }
+
+  // Reservoir helpers — common.wgsl:98-106, lighting.wgsl:51.
+  Res<R> empty_reservoir() const { return Res<R>{0u, F.size[2], 0u, 0u, R(0), R(0), R(0), F.size[3]}; }
+  static R normalization(const Res<R>& r) {
+    if (r.count == 0u || r.selected_target <= R(0)) return R(0);
+    return r.sum / ((R)(float)r.count * r.selected_target);
+  }
+  bool add(Res<R>* r, uint32_t id, uint32_t source, R target, R pdf, R weight, uint32_t count, uint32_t age,
+           uint32_t* rng) {
+    r->count += count;
+    if (weight <= R(0) || weight > R(1e30f)) return false;  // NaN passes, as in WGSL
+    r->sum += weight;
+    if (random(rng) * r->sum < weight) {
+      r->id = id;
+      r->source = source;
+      r->selected_target = target;
+      r->pdf = pdf;
+      r->age = age;
+      return true;
+    }
+    return false;
+  }
+  static Res<R> cap(Res<R> a, uint32_t limit) {
+    if (a.count > limit) {
+      a.sum *= (R)(float)limit / (R)(float)a.count;
+      a.count = limit;
+    }
+    return a;
+  }
+
+  // ------------------------------------------------------------- SDF
+  static R signed_box_distance(Vec<R> p, Vec<R> lo, Vec<R> hi) {  // sdf.wgsl:1-4
+    const Vec<R> delta = max3(sub3(lo, p), sub3(p, hi));
+    return length3(max3(delta, V<R>(0, 0, 0))) + minr(maxr(delta.x, maxr(delta.y, delta.z)), R(0));
+  }
+  // sample_field — common.wgsl:31-38 (manual 8-tap variant).
+  R sample_field(const uint32_t origin[3], Vec<R> q, uint32_t res) const {
+    const uint32_t cell[3] = {minr(to_u32(std::floor(q.x)), res - 2u), minr(to_u32(std::floor(q.y)), res - 2u),
+                              minr(to_u32(std::floor(q.z)), res - 2u)};
+    const R f[3] = {q.x - (R)(float)cell[0], q.y - (R)(float)cell[1], q.z - (R)(float)cell[2]};
+    R value = R(0);
+    for (uint32_t z = 0; z < 2; ++z)
+      for (uint32_t y = 0; y < 2; ++y)
+        for (uint32_t x = 0; x < 2; ++x) {
+          const R wx = x == 1 ? f[0] : R(1) - f[0];
+          const R wy = y == 1 ? f[1] : R(1) - f[1];
+          const R wz = z == 1 ? f[2] : R(1) - f[2];
+          const size_t tx = origin[0] + cell[0] + x, ty = origin[1] + cell[1] + y, tz = origin[2] + cell[2] + z;
+          const R t = (R)sc->atlas[tx + (size_t)sc->atlas_w * (ty + (size_t)sc->atlas_h * tz)];
+          value += t * wx * wy * wz;
+        }
+    return value;
+  }
+  // mesh_distance — sdf.wgsl:5-33.
+  R mesh_distance(Vec<R> p, const irt_mesh& mesh) {
+    const R geometry_bound = signed_box_distance(p, F3<R>(mesh.geometry_lo), F3<R>(mesh.geometry_hi));
+    if ((R)mesh.geometry_lo[3] > R(0.5f)) {
+      cnt[2]++;
+      return geometry_bound;
+    }
+    cnt[3]++;
+    const uint32_t res = mesh.field[3];
+    const Vec<R> lo = F3<R>(mesh.lo), hi = F3<R>(mesh.hi);
+    const Vec<R> clamped = min3(max3(p, lo), hi);  // clamp(p, lo, hi)
+    const Vec<R> q = mul3(sub3(clamped, lo), F3<R>(mesh.grid));
+    const R value = sample_field(mesh.field, q, res);
+    R bound = value - (R)mesh.grid[3] * (R(0.5f) + R(1.0f / 128.0f));
+    const Vec<R> outside = max3(max3(sub3(lo, p), sub3(p, hi)), V<R>(0, 0, 0));
+    const R outside_squared = dot3(outside, outside);
+    if (outside_squared > R(0)) {
+      const R positive = maxr(bound, R(0));
+      bound = std::sqrt(positive * positive + outside_squared);
+    }
+    if (bound > R(0)) bound *= (R)mesh.geometry_hi[3];
+    const R sphere_bound = length3(sub3(p, F3<R>(mesh.enclosing_sphere))) - (R)mesh.enclosing_sphere[3];
+    return maxr(bound, maxr(geometry_bound, sphere_bound));
+  }
+  // object_distance — sdf.wgsl:35-41.
+  void object_distance(Vec<R> p, uint32_t id, bool need_spacing, R* d_out, R* voxel_out) {
+    const irt_object& obj = sc->objects[id];
+    const irt_mesh& mesh = sc->meshes[obj.info[0]];
+    R l[4];
+    mat_vec(obj.inverse, p.x, p.y, p.z, R(1), l);
+    float scale, voxel;
+    std::memcpy(&scale, &obj.info[2], 4);
+    std::memcpy(&voxel, &obj.info[3], 4);
+    R d = mesh_distance(V<R>(l[0], l[1], l[2]), mesh) * (R)scale;
+    if (d > R(0)) d *= R(0.9999f);
+    *d_out = d;
+    *voxel_out = need_spacing ? (R)voxel : (R)F.sdf[1];
+  }
+  // scene_distance — sdf.wgsl:42-60.
+  void scene_distance(Vec<R> p, bool need_spacing, uint32_t hint, R* best_out, R* voxel_out, uint32_t* closest_out) {
+    R best = R(1e20f), voxel = (R)F.sdf[1];
+    uint32_t closest = hint, cursor = 0u;
+    const uint32_t n = F.counts[3];
+    if (hint < n) object_distance(p, hint, need_spacing, &best, &voxel);
+    while (cursor < n * 2u - 1u) {
+      const irt_node& node = sc->nodes[cursor];
+      cnt[1]++;
+      const Vec<R> delta = max3(max3(sub3(F3<R>(node.lo), p), sub3(p, F3<R>(node.hi))), V<R>(0, 0, 0));
+      const Vec<R> lower_bound = scale3(delta, R(0.9998f));
+      if (dot3(lower_bound, lower_bound) > maxr(best, R(0)) * maxr(best, R(0))) {
+        cursor = node.info[3];
+        continue;
+      }
+      if (node.info[2] == 1u) {
+        if (node.info[0] != hint) {
+          R d, vx;
+          object_distance(p, node.info[0], need_spacing, &d, &vx);
+          if (d < best) {
+            best = d;
+            voxel = vx;
+            closest = node.info[0];
+          }
+        }
+      }
+      cursor++;
+    }
+    *best_out = best;
+    *voxel_out = voxel;
+    *closest_out = closest;
+  }
+  // visibility — sdf.wgsl:61-81.
+  R visibility(Vec<R> p, Vec<R> n, Vec<R> target, Vec<R> end_normal) {
+    cnt[6]++;
+    const R eps = (R)F.sdf[1];
+    R cd, cv;
+    uint32_t closest;
+    scene_distance(p, true, 0xffffffffu, &cd, &cv, &closest);
+    const R epsilon = maxr(eps, cv * R(0.65f));
+    const Vec<R> origin = add3(p, scale3(n, epsilon));
+    Vec<R> endpoint = target;
+    if (dot3(end_normal, end_normal) > R(0.5f)) {
+      R ed, ev;
+      uint32_t ec;
+      scene_distance(target, true, 0xffffffffu, &ed, &ev, &ec);
+      endpoint = add3(endpoint, scale3(end_normal, maxr(eps, ev * R(0.65f))));
+    }
+    const Vec<R> delta = sub3(endpoint, origin);
+    const R limit = length3(delta) - eps;
+    if (limit <= R(0)) return R(1);
+    const Vec<R> direction = normalize3(delta);
+    R t = eps, result = R(1), previous = R(1e20f);
+    const uint32_t steps = to_u32((R)F.sdf[0]);
+    for (uint32_t i = 0; i < steps; ++i) {
+      if (t >= limit) return clampr(result, R(0), R(1));
+      cnt[4]++;
+      R h, hv;
+      scene_distance(add3(origin, scale3(direction, t)), false, closest, &h, &hv, &closest);
+      if (h < eps) return R(0);
+      result = shadow_step(result, h, previous, t, (R)F.sdf[2]);
+      if (result < R(0.001f)) return R(0);
+      previous = h;
+      t += h;
+    }
+    return R(0);
+  }
+  static R shadow_step(R result, R h, R previous, R t, R k) {  // sdf.wgsl:180-181
+    const R y = h * h / (R(2) * previous);
+    const R d = std::sqrt(maxr(h * h - y * y, R(0)));
+    if (t > y) result = minr(result, k * d / (t - y));
+    return result;
+  }
+
+  // ---------------------------------------------------------- lighting
+  size_t idx(int64_t x, int64_t y) const { return (size_t)y * W() + (size_t)x; }
+  bool valid(int64_t x, int64_t y) const { return (R)gb->depth[idx(x, y)] < R(1); }  // lighting.wgsl:35
+  Surface<R> surface(int64_t x, int64_t y) const {                                    // lighting.wgsl:30-34
+    const size_t i = idx(x, y);
+    Surface<R> s;
+    s.p = world(pixel_u(x), pixel_v(y), (R)gb->depth[i], F.inv_vp);
+    s.n = normalize3(D3<R>(gb->normal + 4 * i));
+    s.color = D3<R>(gb->color + 4 * i);
+    s.rough = (R)gb->params[2 * i];
+    s.metal = (R)gb->params[2 * i + 1];
+    return s;
+  }
+  bool compatible(const Surface<R>& a, int64_t qx, int64_t qy) const {  // lighting.wgsl:36-40
+    const size_t q = idx(qx, qy);
+    const R d = (R)gb->depth[q];
+    if (d >= R(1)) return false;
+    const Vec<R> p = world(pixel_u(qx), pixel_v(qy), d, F.inv_vp);
+    const Vec<R> n = normalize3(D3<R>(gb->normal + 4 * q));
+    return dot3(a.n, n) > (R)F.temporal[2] &&
+           length3(sub3(a.p, p)) < (R)F.temporal[1] * maxr(length3(sub3(a.p, camera())), R(1));
+  }
+  void reproject(int64_t x, int64_t y, int32_t* qx, int32_t* qy) const {  // lighting.wgsl:41-44
+    const size_t i = idx(x, y);
+    const R u = pixel_u(x) - (R)gb->motion[2 * i], v = pixel_v(y) - (R)gb->motion[2 * i + 1];
+    *qx = to_i32(std::floor(u * (R)W()));
+    *qy = to_i32(std::floor(v * (R)H()));
+  }
+  bool history_valid(int64_t x, int64_t y, int32_t qx, int32_t qy) const {  // lighting.wgsl:45-50
+    if (F.flags[0] == 0u || F.flags[1] != F.size[3] || qx < 0 || qy < 0 || qx >= (int64_t)W() || qy >= (int64_t)H())
+      return false;
+    const size_t q = idx(qx, qy);
+    const R d = (R)gb->previous_depth[q];
+    if (d >= R(1)) return false;
+    const Surface<R> a = surface(x, y);
+    const Vec<R> oldp = world(pixel_u(qx), pixel_v(qy), d, F.previous_inv_vp);
+    return dot3(a.n, D3<R>(gb->previous_normal + 4 * q)) > (R)F.temporal[2] &&
+           length3(sub3(a.p, oldp)) < (R)F.temporal[1] * maxr(length3(sub3(a.p, camera())), R(1));
+  }
+  bool vpl_sample_valid(const Res<R>& r) const {  // lighting.wgsl:22-24
+    return r.generation == F.size[3] && r.selected_target > R(0) && r.id < F.counts[1] &&
+           F.size[2] - r.source <= to_u32((R)F.rr[2]);
+  }
+  const irt_vpl& selected_vpl(const Res<R>& r) const {  // lighting.wgsl:25-29
+    if (r.source == F.size[2]) return io->current_vpls[r.id];
+    if (r.source + 1u == F.size[2]) return io->previous_vpls[r.id];
+    return io->archived_vpls[(size_t)(r.source % (to_u32((R)F.rr[2]) + 1u)) * F.counts[1] + r.id];
+  }
+  void neighbor(int64_t x, int64_t y, uint32_t* rng, int64_t* qx, int64_t* qy) {  // lighting.wgsl:86-89
+    const R two_pi = (R)(2.0f * 3.14159265359f);  // 2.*PI folded in f32
+    const R angle = two_pi * random(rng);
+    const R radius = std::sqrt(random(rng)) * (R)(float)F.restir[3];
+    *qx = x + to_i32(std::nearbyint(std::cos(angle) * radius));  // round(): half to even
+    *qy = y + to_i32(std::nearbyint(std::sin(angle) * radius));
+  }
+
+  void candidates_direct(int64_t x, int64_t y) {  // lighting.wgsl:52-58
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = empty_reservoir();
+    uint32_t rng = seed_of(id, 11u, F.size[2]);
+    if (valid(x, y) && F.counts[0] > 0u) {
+      const Surface<R> s = surface(x, y);
+      const R pdf = R(1) / (R)(float)F.counts[0];
+      for (uint32_t i = 0; i < F.restir[0]; ++i) {
+        const uint32_t light_id = minr(to_u32(random(&rng) * (R)(float)F.counts[0]), F.counts[0] - 1u);
+        const R target = luminance(direct(s, light_id));
+        add(&r, light_id, F.size[2], target, pdf, target / pdf, 1u, 0u, &rng);
+      }
+    }
+    io->direct_out[id] = to_io(r);
+  }
+  void candidates_indirect(int64_t x, int64_t y) {  // lighting.wgsl:59-65
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = empty_reservoir();
+    uint32_t rng = seed_of(id, 13u, F.size[2]);
+    if (valid(x, y)) {
+      const Surface<R> s = surface(x, y);
+      const R pdf = R(1) / (R)(float)F.counts[1];
+      for (uint32_t i = 0; i < F.restir[1]; ++i) {
+        const uint32_t vpl_id = minr(to_u32(random(&rng) * (R)(float)F.counts[1]), F.counts[1] - 1u);
+        const R target = luminance(indirect(s, io->current_vpls[vpl_id]));
+        add(&r, vpl_id, F.size[2], target, pdf, target / pdf, 1u, 0u, &rng);
+      }
+    }
+    io->indirect_out[id] = to_io(r);
+  }
+  void temporal_direct(int64_t x, int64_t y) {  // lighting.wgsl:66-74
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = from_io<R>(io->direct_in[id]);
+    int32_t qx, qy;
+    reproject(x, y, &qx, &qy);
+    uint32_t rng = seed_of(id, 17u, F.size[2]);
+    if (valid(x, y) && history_valid(x, y, qx, qy)) {
+      const uint32_t hist = to_u32((R)F.rr[2]);
+      const Res<R> previous = cap(from_io<R>(io->direct_history[idx(qx, qy)]), hist * F.restir[0]);
+      if (previous.generation == F.size[3] && previous.selected_target > R(0) && previous.id < F.counts[0] &&
+          previous.age < hist) {
+        const R target = luminance(direct(surface(x, y), previous.id));
+        add(&r, previous.id, previous.source, target, previous.pdf,
+            target * normalization(previous) * (R)(float)previous.count, previous.count, previous.age + 1u, &rng);
+      }
+    }
+    io->direct_out[id] = to_io(r);
+  }
+  void temporal_indirect(int64_t x, int64_t y) {  // lighting.wgsl:75-85
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = from_io<R>(io->indirect_in[id]);
+    int32_t qx, qy;
+    reproject(x, y, &qx, &qy);
+    uint32_t rng = seed_of(id, 19u, F.size[2]);
+    if (valid(x, y) && history_valid(x, y, qx, qy)) {
+      const Res<R> previous = cap(from_io<R>(io->indirect_history[idx(qx, qy)]), to_u32((R)F.rr[2]) * F.restir[1]);
+      if (vpl_sample_valid(previous)) {
+        const irt_vpl& v = selected_vpl(previous);
+        const R target = luminance(indirect(surface(x, y), v));
+        add(&r, previous.id, previous.source, target, previous.pdf,
+            target * normalization(previous) * (R)(float)previous.count, previous.count, previous.age + 1u, &rng);
+      }
+    }
+    io->indirect_out[id] = to_io(r);
+  }
+  void spatial_direct(int64_t x, int64_t y) {  // lighting.wgsl:90-97
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = from_io<R>(io->direct_in[id]);
+    uint32_t rng = seed_of(id, 23u, F.size[2]);
+    if (valid(x, y)) {
+      const Surface<R> s = surface(x, y);
+      for (uint32_t i = 0; i < F.restir[2]; ++i) {
+        int64_t qx, qy;
+        neighbor(x, y, &rng, &qx, &qy);
+        if (qx < 0 || qy < 0 || qx >= (int64_t)W() || qy >= (int64_t)H()) continue;
+        if (!compatible(s, qx, qy)) continue;
+        const Res<R> previous = from_io<R>(io->direct_in[idx(qx, qy)]);
+        if (previous.selected_target > R(0) && previous.id < F.counts[0]) {
+          const R target = luminance(direct(s, previous.id));
+          add(&r, previous.id, previous.source, target, previous.pdf,
+              target * normalization(previous) * (R)(float)previous.count, previous.count, previous.age, &rng);
+        }
+      }
+    }
+    io->direct_out[id] = to_io(cap(r, to_u32((R)F.rr[2]) * F.restir[0] * 8u));
+  }
+  void spatial_indirect(int64_t x, int64_t y) {  // lighting.wgsl:98-105
+    const uint32_t id = (uint32_t)idx(x, y);
+    Res<R> r = from_io<R>(io->indirect_in[id]);
+    uint32_t rng = seed_of(id, 29u, F.size[2]);
+    if (valid(x, y)) {
+      const Surface<R> s = surface(x, y);
+      for (uint32_t i = 0; i < F.restir[2]; ++i) {
+        int64_t qx, qy;
+        neighbor(x, y, &rng, &qx, &qy);
+        if (qx < 0 || qy < 0 || qx >= (int64_t)W() || qy >= (int64_t)H()) continue;
+        if (!compatible(s, qx, qy)) continue;
+        const Res<R> previous = from_io<R>(io->indirect_in[idx(qx, qy)]);
+        if (vpl_sample_valid(previous)) {
+          const R target = luminance(indirect(s, selected_vpl(previous)));
+          add(&r, previous.id, previous.source, target, previous.pdf,
+              target * normalization(previous) * (R)(float)previous.count, previous.count, previous.age, &rng);
+        }
+      }
+    }
+    io->indirect_out[id] = to_io(cap(r, to_u32((R)F.rr[2]) * F.restir[1] * 8u));
+  }
+  void resolve_direct(int64_t x, int64_t y) {  // lighting.wgsl:106-122
+    const size_t id = idx(x, y);
+    Vec<R> result = V<R>(0, 0, 0);
+    if (valid(x, y)) {
+      const Surface<R> s = surface(x, y);
+      const Res<R> d = from_io<R>(io->direct_in[id]);
+      if (d.selected_target > R(0) && d.id < F.counts[0] && F.flags[2] != 2u && F.flags[2] < 3u) {
+        const Vec<R> value = direct(s, d.id);
+        if (luminance(value) > R(0)) {
+          const R w = normalization(d);
+          const R vis = visibility(s.p, s.n, F3<R>(sc->lights[d.id].position_strength), V<R>(0, 0, 0));
+          result = add3(result, scale3(scale3(value, w), vis));
+        }
+      }
+      if (F.flags[2] == 3u) result = s.color;
+      if (F.flags[2] == 4u) result = add3(scale3(s.n, R(0.5f)), V<R>(R(0.5f), R(0.5f), R(0.5f)));
+      if (F.flags[2] == 5u) {
+        const R mx = (R)gb->motion[2 * id], my = (R)gb->motion[2 * id + 1];
+        result = V<R>(clampr(R(0.5f) + mx * R(50), R(0), R(1)), clampr(R(0.5f) + my * R(50), R(0), R(1)), R(0.5f));
+      }
+    }
+    double* o = io->direct_output + 4 * id;  // Rgba32F
+    o[0] = (double)result.x;
+    o[1] = (double)result.y;
+    o[2] = (double)result.z;
+    o[3] = 1.0;
+  }
+  void resolve_indirect(int64_t x, int64_t y) {  // lighting.wgsl:123-134
+    const size_t id = idx(x, y);
+    Vec<R> result = D3<R>(io->direct_lighting + 4 * id);
+    if (valid(x, y) && F.flags[2] != 1u && F.flags[2] < 3u) {
+      const Res<R> v = from_io<R>(io->indirect_in[id]);
+      if (vpl_sample_valid(v)) {
+        const Surface<R> s = surface(x, y);
+        const irt_vpl& smp = selected_vpl(v);
+        const Vec<R> value = indirect(s, smp);
+        if (luminance(value) > R(0)) {
+          const R w = normalization(v);
+          const R vis = visibility(s.p, s.n, F3<R>(smp.position), F3<R>(smp.normal));
+          result = add3(result, scale3(scale3(value, w), vis));
+        }
+      }
+    }
+    const R c = R(65000);
+    double* o = io->output + 4 * id;  // Rgba16F
+    o[0] = (double)store16(minr(result.x, c));
+    o[1] = (double)store16(minr(result.y, c));
+    o[2] = (double)store16(minr(result.z, c));
+    o[3] = 1.0;
+  }
+
+  // ------------------------------------------------------------- SVGF
+  bool inside(int64_t x, int64_t y) const { return x >= 0 && y >= 0 && x < (int64_t)W() && y < (int64_t)H(); }
+  Vec<R> position(int64_t x, int64_t y) const {  // svgf.wgsl:19
+    return world(pixel_u(x), pixel_v(y), (R)gb->depth[idx(x, y)], F.inv_vp);
+  }
+  bool history_ok(int64_t px, int64_t py, int64_t qx, int64_t qy) const {  // svgf.wgsl:20-26
+    if (F.flags[0] == 0u || F.flags[1] != F.size[3] || !inside(qx, qy)) return false;
+    const size_t q = idx(qx, qy);
+    const R d = (R)gb->previous_depth[q];
+    if (d >= R(1)) return false;
+    const Vec<R> a = position(px, py);
+    const Vec<R> b = world(pixel_u(qx), pixel_v(qy), d, F.previous_inv_vp);
+    return length3(sub3(a, b)) < (R)F.temporal[1] * maxr(length3(sub3(a, camera())), R(1)) &&
+           dot3(D3<R>(gb->normal + 4 * idx(px, py)), D3<R>(gb->previous_normal + 4 * q)) > (R)F.temporal[2];
+  }
+  struct Guide {
+    Vec<R> position, normal, color;
+    R m0, m1, pixel_scale;
+  };
+  Guide guide(int64_t x, int64_t y) const {  // svgf.wgsl:27-31
+    const size_t i = idx(x, y);
+    Guide g;
+    g.position = position(x, y);
+    g.normal = D3<R>(gb->normal + 4 * i);
+    g.color = D3<R>(gb->color + 4 * i);
+    g.m0 = (R)gb->params[2 * i];
+    g.m1 = (R)gb->params[2 * i + 1];
+    g.pixel_scale = length3(sub3(g.position, camera())) / (R)(float)F.size[1];
+    return g;
+  }
+  R geometry_weight(const Guide& a, int64_t qx, int64_t qy, R stride) {  // svgf.wgsl:32-44
+    cnt[5]++;
+    if (!inside(qx, qy) || (R)gb->depth[idx(qx, qy)] >= R(1)) return R(0);
+    const size_t q = idx(qx, qy);
+    const Vec<R> nq = D3<R>(gb->normal + 4 * q);
+    const R wn = std::pow(maxr(dot3(a.normal, nq), R(0)), R(64));
+    const Vec<R> wq = position(qx, qy);
+    const R tolerance = maxr(R(0.0001f), a.pixel_scale * stride * R(0.5f));
+    const R wz = std::exp(-std::fabs(dot3(sub3(wq, a.position), a.normal)) / tolerance);
+    const R dm0 = a.m0 - (R)gb->params[2 * q], dm1 = a.m1 - (R)gb->params[2 * q + 1];
+    const R wc = std::exp(-length3(sub3(a.color, D3<R>(gb->color + 4 * q))) * R(8) - std::sqrt(dm0 * dm0 + dm1 * dm1) * R(8));
+    return wn * wz * wc;
+  }
+  void svgf_temporal(int64_t x, int64_t y) {  // svgf.wgsl:45-76
+    const size_t i = idx(x, y);
+    const Vec<R> c = D3<R>(io->noisy + 4 * i);
+    const R l = luminance(c);
+    Vec<R> history = V<R>(0, 0, 0);
+    R hm0 = 0, hm1 = 0, length_history = 0, total = 0;
+    if ((R)gb->depth[i] < R(1)) {
+      const R px = (pixel_u(x) - (R)gb->motion[2 * i]) * (R)W() - R(0.5f);
+      const R py = (pixel_v(y) - (R)gb->motion[2 * i + 1]) * (R)H() - R(0.5f);
+      const int32_t bx = to_i32(std::floor(px)), by = to_i32(std::floor(py));
+      const R fx = px - std::floor(px), fy = py - std::floor(py);  // fract
+      for (int yy = 0; yy < 2; ++yy)
+        for (int xx = 0; xx < 2; ++xx) {
+          const int64_t qx = (int64_t)bx + xx, qy = (int64_t)by + yy;
+          if (!history_ok(x, y, qx, qy)) continue;
+          const double* m = io->old_moments + 4 * idx(qx, qy);
+          if ((R)m[2] <= R(0)) continue;
+          const R wx = xx == 1 ? fx : R(1) - fx;
+          const R wy = yy == 1 ? fy : R(1) - fy;
+          const R w = wx * wy;
+          history = add3(history, scale3(D3<R>(io->old_radiance + 4 * idx(qx, qy)), w));
+          hm0 += (R)m[0] * w;
+          hm1 += (R)m[1] * w;
+          length_history += (R)m[2] * w;
+          total += w;
+        }
+    }
+    R age = R(1);
+    Vec<R> result = c;
+    R m0 = l, m1 = l * l;
+    if (total > R(0.001f)) {
+      history = div3(history, total);
+      hm0 /= total;
+      hm1 /= total;
+      age = minr(length_history / total + R(1), (R)F.temporal[3]);
+      R lo = R(1e20f), hi = R(0);
+      const Guide a = guide(x, y);
+      for (int yy = -1; yy <= 1; ++yy)
+        for (int xx = -1; xx <= 1; ++xx) {
+          const int64_t qx = clampr<int64_t>(x + xx, 0, (int64_t)W() - 1);
+          const int64_t qy = clampr<int64_t>(y + yy, 0, (int64_t)H() - 1);
+          if (geometry_weight(a, qx, qy, R(1)) > R(0.01f)) {
+            const R v = luminance(D3<R>(io->noisy + 4 * idx(qx, qy)));
+            lo = minr(lo, v);
+            hi = maxr(hi, v);
+          }
+        }
+      const R lh = luminance(history);
+      history = scale3(history, clampr(lh, lo * R(0.5f), hi * R(1.5f) + R(0.001f)) / maxr(lh, R(0.000001f)));
+      const R alpha = maxr(R(1) - (R)F.temporal[0], R(1) / age);
+      result = add3(scale3(history, R(1) - alpha), scale3(c, alpha));  // mix(history, c, alpha)
+      const R beta = maxr(R(0.2f), R(1) / age);
+      m0 = hm0 * (R(1) - beta) + m0 * beta;
+      m1 = hm1 * (R(1) - beta) + m1 * beta;
+    }
+    const R variance = maxr(m1 - m0 * m0, R(0));
+    double* o = io->radiance_out + 4 * i;  // Rgba16F
+    o[0] = (double)store16(result.x);
+    o[1] = (double)store16(result.y);
+    o[2] = (double)store16(result.z);
+    o[3] = (double)store16(minr(std::sqrt(variance), R(65000)));
+    double* mo = io->moments_out + 4 * i;  // Rgba32F
+    mo[0] = (double)m0;
+    mo[1] = (double)m1;
+    mo[2] = (double)age;
+    mo[3] = 0.0;
+  }
+  void svgf_variance(int64_t x, int64_t y) {  // svgf.wgsl:77-89
+    const size_t i = idx(x, y);
+    const double* c = io->input_radiance + 4 * i;
+    const R age = (R)io->moments[4 * i + 2];
+    R sigma = (R)c[3];
+    if (age < R(4) && (R)gb->depth[i] < R(1)) {
+      R s0 = 0, s1 = 0, total = 0;
+      const Guide a = guide(x, y);
+      for (int yy = -3; yy <= 3; ++yy)
+        for (int xx = -3; xx <= 3; ++xx) {
+          const int64_t qx = x + xx, qy = y + yy;
+          const R w = geometry_weight(a, qx, qy, R(1));
+          if (w <= R(0)) continue;
+          s0 += (R)io->moments[4 * idx(qx, qy)] * w;
+          s1 += (R)io->moments[4 * idx(qx, qy) + 1] * w;
+          total += w;
+        }
+      s0 /= maxr(total, R(0.00001f));
+      s1 /= maxr(total, R(0.00001f));
+      sigma = std::sqrt(maxr(s1 - s0 * s0, R(0)) * R(4) / maxr(age, R(1)));
+    }
+    double* o = io->radiance_out + 4 * i;
+    o[0] = (double)store16((R)c[0]);
+    o[1] = (double)store16((R)c[1]);
+    o[2] = (double)store16((R)c[2]);
+    o[3] = (double)store16(minr(sigma, R(65000)));
+  }
+  static R kernel(int x) {  // svgf.wgsl:90
+    if (x == 0) return R(1);
+    if (x == 1 || x == -1) return R(2.0f / 3.0f);
+    return R(1.0f / 6.0f);
+  }
+  void svgf_atrous(int64_t x, int64_t y) {  // svgf.wgsl:91-110
+    const size_t i = idx(x, y);
+    const double* cin = io->input_radiance + 4 * i;
+    double* o = io->radiance_out + 4 * i;
+    if ((R)gb->depth[i] >= R(1)) {
+      for (int k = 0; k < 4; ++k) o[k] = (double)store16((R)cin[k]);
+      return;
+    }
+    const Vec<R> c = D3<R>(cin);
+    R variance = 0, guide_weight = 0;
+    const Guide a = guide(x, y);
+    for (int yy = -1; yy <= 1; ++yy)
+      for (int xx = -1; xx <= 1; ++xx) {
+        const int64_t qx = clampr<int64_t>(x + xx, 0, (int64_t)W() - 1);
+        const int64_t qy = clampr<int64_t>(y + yy, 0, (int64_t)H() - 1);
+        const R w = (xx == 0 ? R(2) : R(1)) * (yy == 0 ? R(2) : R(1));
+        const R s = (R)io->input_radiance[4 * idx(qx, qy) + 3];
+        variance += s * s * w;
+        guide_weight += w;
+      }
+    const R phi = R(4) * std::sqrt(variance / maxr(guide_weight, R(0.00001f))) + R(0.0001f);
+    Vec<R> sum = V<R>(0, 0, 0);
+    R total = 0, v = 0;
+    const int64_t step = (int64_t)to_i32((R)(float)io->stride);
+    for (int yy = -2; yy <= 2; ++yy)
+      for (int xx = -2; xx <= 2; ++xx) {
+        const int64_t qx = x + xx * step, qy = y + yy * step;
+        const R geom = geometry_weight(a, qx, qy, (R)(float)step);
+        if (geom <= R(0)) continue;
+        const double* smp = io->input_radiance + 4 * idx(qx, qy);
+        const Vec<R> srgb = D3<R>(smp);
+        const R w = kernel(xx) * kernel(yy) * geom * std::exp(-std::fabs(luminance(srgb) - luminance(c)) / phi);
+        sum = add3(sum, scale3(srgb, w));
+        v += (R)smp[3] * (R)smp[3] * w * w;
+        total += w;
+      }
+    const R t = maxr(total, R(0.00001f));
+    o[0] = (double)store16(sum.x / t);
+    o[1] = (double)store16(sum.y / t);
+    o[2] = (double)store16(sum.z / t);
+    o[3] = (double)store16(minr(std::sqrt(v) / t, R(65000)));
+  }
+
+  void run(int pass, int64_t x, int64_t y) {
+    switch (pass) {
+      case ORACLE_CANDIDATES_DIRECT: candidates_direct(x, y); break;
+      case ORACLE_CANDIDATES_INDIRECT: candidates_indirect(x, y); break;
+      case ORACLE_TEMPORAL_DIRECT: temporal_direct(x, y); break;
+      case ORACLE_TEMPORAL_INDIRECT: temporal_indirect(x, y); break;
+      case ORACLE_SPATIAL_DIRECT: spatial_direct(x, y); break;
+      case ORACLE_SPATIAL_INDIRECT: spatial_indirect(x, y); break;
+      case ORACLE_RESOLVE_DIRECT: resolve_direct(x, y); break;
+      case ORACLE_RESOLVE_INDIRECT: resolve_indirect(x, y); break;
+      case ORACLE_SVGF_TEMPORAL: svgf_temporal(x, y); break;
+      case ORACLE_SVGF_VARIANCE: svgf_variance(x, y); break;
+      case ORACLE_SVGF_ATROUS: svgf_atrous(x, y); break;
+    }
+  }
+};
+
+template <class R>
+int run_pass(int pass, const irt_frame* frame, const oracle_scene* scene, const oracle_gbuffer* gb, oracle_io* io) {
+  const uint32_t W = frame->size[0], H = frame->size[1];
+  uint32_t y0 = io->row_begin, y1 = io->row_end;
+  if (y0 == 0 && y1 == 0) y1 = H;
+  if (y1 > H) y1 = H;
+  uint64_t total[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int threads = io->threads > 0 ? io->threads : omp_get_max_threads();
+#pragma omp parallel num_threads(threads)
+  {
+    Oracle<R> o(*frame, scene, gb, io);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t y = y0; y < (int64_t)y1; ++y)
+      for (int64_t x = 0; x < (int64_t)W; ++x) o.run(pass, x, y);
+#pragma omp critical
+    for (int k = 0; k < 7; ++k) total[k] += o.cnt[k];
+  }
+  if (io->counters)
+    for (int k = 0; k < 7; ++k) io->counters[k] += total[k];
+  return 0;
+}
+
+template <class R>
+Oracle<R> make(const irt_frame* f, const oracle_scene* s) {
+  static oracle_gbuffer g{};
+  static oracle_io i{};
+  return Oracle<R>(*f, s, &g, &i);
+}
+
+const irt_frame& dummy_frame() {
+  static irt_frame f{};
+  return f;
+}
+
+template <class R>
+int add_impl(oracle_reservoir* r, uint32_t id, uint32_t source, double target, double pdf, double weight,
+                    uint32_t count, uint32_t age, double random, uint32_t* rng) {
+  Res<R> a = from_io<R>(*r);
+  a.count += count;
+  int taken = 0;
+  if (!((R)weight <= R(0) || (R)weight > R(1e30f))) {
+    a.sum += (R)weight;
+    const R u = random >= 0 ? (R)random : (R)random_f32(rng);
+    if (u * a.sum < (R)weight) {
+      a.id = id;
+      a.source = source;
+      a.selected_target = (R)target;
+      a.pdf = (R)pdf;
+      a.age = age;
+      taken = 1;
+    }
+  }
+  *r = to_io(a);
+  return taken;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_run_pass(int bits, int pass, const irt_frame* frame, const oracle_scene* scene, const oracle_gbuffer* gb,
+                    oracle_io* io) {
+  if (!frame || !gb || !io || pass < 0 || pass > ORACLE_SVGF_ATROUS) return -1;
+  if (pass <= ORACLE_RESOLVE_INDIRECT && !scene) return -1;
+  return bits == 64 ? run_pass<double>(pass, frame, scene, gb, io) : run_pass<float>(pass, frame, scene, gb, io);
+}
+
+float oracle_random(uint32_t* state) { return random_f32(state); }
+uint32_t oracle_seed(uint32_t i, uint32_t salt, uint32_t frame_index) { return seed_of(i, salt, frame_index); }
+
+double oracle_signed_box_distance(int bits, const double p[3], const double lo[3], const double hi[3]) {
+  if (bits == 64) return Oracle<double>::signed_box_distance(D3<double>(p), D3<double>(lo), D3<double>(hi));
+  return (double)Oracle<float>::signed_box_distance(D3<float>(p), D3<float>(lo), D3<float>(hi));
+}
+
+double oracle_mesh_distance(int bits, const double p[3], const irt_mesh* mesh, const oracle_scene* scene) {
+  if (bits == 64) {
+    Oracle<double> o = make<double>(&dummy_frame(), scene);
+    return o.mesh_distance(D3<double>(p), *mesh);
+  }
+  Oracle<float> o = make<float>(&dummy_frame(), scene);
+  return (double)o.mesh_distance(D3<float>(p), *mesh);
+}
+
+double oracle_scene_distance(int bits, const irt_frame* frame, const oracle_scene* scene, const double p[3],
+                             int need_spacing, uint32_t hint, double* voxel, uint32_t* closest) {
+  uint32_t c = 0;
+  if (bits == 64) {
+    Oracle<double> o = make<double>(frame, scene);
+    double d, v;
+    o.scene_distance(D3<double>(p), need_spacing != 0, hint, &d, &v, &c);
+    if (voxel) *voxel = v;
+    if (closest) *closest = c;
+    return d;
+  }
+  Oracle<float> o = make<float>(frame, scene);
+  float d, v;
+  o.scene_distance(D3<float>(p), need_spacing != 0, hint, &d, &v, &c);
+  if (voxel) *voxel = v;
+  if (closest) *closest = c;
+  return d;
+}
+
+double oracle_visibility(int bits, const irt_frame* frame, const oracle_scene* scene, const double p[3],
+                         const double n[3], const double target[3], const double end_normal[3], uint64_t* counters) {
+  if (bits == 64) {
+    Oracle<double> o = make<double>(frame, scene);
+    double r = o.visibility(D3<double>(p), D3<double>(n), D3<double>(target), D3<double>(end_normal));
+    if (counters)
+      for (int k = 0; k < 7; ++k) counters[k] += o.cnt[k];
+    return r;
+  }
+  Oracle<float> o = make<float>(frame, scene);
+  float r = o.visibility(D3<float>(p), D3<float>(n), D3<float>(target), D3<float>(end_normal));
+  if (counters)
+    for (int k = 0; k < 7; ++k) counters[k] += o.cnt[k];
+  return r;
+}
+
+void oracle_brdf(int bits, const double s[11], const double wo[3], const double wi[3], double out[3]) {
+  if (bits == 64) {
+    Oracle<double> o = make<double>(&dummy_frame(), nullptr);
+    Surface<double> sf{D3<double>(s), D3<double>(s + 3), D3<double>(s + 6), s[9], s[10]};
+    Vec<double> r = o.brdf(sf, D3<double>(wo), D3<double>(wi));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    return;
+  }
+  Oracle<float> o = make<float>(&dummy_frame(), nullptr);
+  Surface<float> sf{D3<float>(s), D3<float>(s + 3), D3<float>(s + 6), (float)s[9], (float)s[10]};
+  Vec<float> r = o.brdf(sf, D3<float>(wo), D3<float>(wi));
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+
+int oracle_reservoir_add(int bits, oracle_reservoir* r, uint32_t id, uint32_t source, double target, double pdf,
+                         double weight, uint32_t count, uint32_t age, double random, uint32_t* rng) {
+  if (!r || (random < 0 && !rng)) return -1;
+  return bits == 64 ? add_impl<double>(r, id, source, target, pdf, weight, count, age, random, rng)
+                    : add_impl<float>(r, id, source, target, pdf, weight, count, age, random, rng);
+}
+
+double oracle_normalization(int bits, const oracle_reservoir* r) {
+  if (bits == 64) return Oracle<double>::normalization(from_io<double>(*r));
+  return (double)Oracle<float>::normalization(from_io<float>(*r));
+}
+
+double oracle_shadow_step(int bits, double result, double h, double previous, double t, double k) {
+  if (bits == 64) return Oracle<double>::shadow_step(result, h, previous, t, k);
+  return (double)Oracle<float>::shadow_step((float)result, (float)h, (float)previous, (float)t, (float)k);
+}
+
+uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }
+float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+/*
+ * oracle.h — CPU restatement of the reference's per-pixel lighting and
+ * denoising path, used ONLY as the checker in tests/, bench.py's
+ * cpu_baseline leg and __graft_entry__.smoke().  Never linked into the
+ * product library.
+ *
+ * Two instantiations share one template:
+ *   bits = 32: FP32 arithmetic in WGSL evaluation order, values written to
+ *              the reference's Rgba16F targets rounded to binary16 (the
+ *              "CPU restatement" and the CPU baseline);
+ *   bits = 64: FP64 arithmetic, no storage rounding (states the error of the
+ *              FP32 results).
+ * Both consume the same PCG stream (the FP32 random value), so decisions
+ * match except where FP rounding separates near-ties.
+ *
+ * Interfaces carry every value as double (exact for FP32 inputs).
+ */
+#ifndef IRT_ORACLE_H
+#define IRT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/irt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_reservoir {
+  uint32_t id, source, count, age;
+  double sum, selected_target, pdf;
+  uint32_t generation, pad;
+} oracle_reservoir; /* 48 B */
+
+typedef struct oracle_scene {
+  const irt_object* objects;
+  const irt_mesh* meshes;
+  const irt_node* nodes;
+  const irt_light* lights;
+  const float* atlas; /* reference atlas, x fastest */
+  uint32_t atlas_w, atlas_h, atlas_d;
+  uint32_t n_objects, n_meshes, n_nodes, n_lights;
+} oracle_scene;
+
+/* Per-pixel planes, row-major W*H, values of the reference formats. */
+typedef struct oracle_gbuffer {
+  const double* depth;           /* 1 */
+  const double* normal;          /* 4 */
+  const double* color;           /* 4 */
+  const double* params;          /* 2 */
+  const double* motion;          /* 2 */
+  const double* previous_depth;  /* 1 */
+  const double* previous_normal; /* 4 */
+} oracle_gbuffer;
+
+typedef struct oracle_io {
+  /* lighting passes (group 2 bindings, lighting.wgsl:6-21) */
+  const oracle_reservoir* direct_in;
+  const oracle_reservoir* indirect_in;
+  oracle_reservoir* direct_out;
+  oracle_reservoir* indirect_out;
+  const oracle_reservoir* direct_history;
+  const oracle_reservoir* indirect_history;
+  const irt_vpl* current_vpls;
+  const irt_vpl* previous_vpls;
+  const irt_vpl* archived_vpls;
+  double* direct_output;         /* rgba, resolve_direct */
+  const double* direct_lighting; /* rgba, resolve_indirect input */
+  double* output;                /* rgba, resolve_indirect (Rgba16F) */
+  /* SVGF passes (svgf.wgsl:3-17) */
+  const double* noisy;
+  const double* old_radiance;
+  const double* old_moments;
+  const double* input_radiance;
+  const double* moments;
+  double* radiance_out;
+  double* moments_out;
+  uint32_t stride; /* wavelet.x */
+  /* event counters (brdf, node, box object, grid object, step, tap, march); may be null */
+  uint64_t* counters;
+  /* restrict the pass to rows [row_begin, row_end) (0,0 = all rows) */
+  uint32_t row_begin, row_end;
+  int threads; /* OpenMP threads, <= 0: runtime default */
+} oracle_io;
+
+enum {
+  ORACLE_CANDIDATES_DIRECT = 0,
+  ORACLE_CANDIDATES_INDIRECT = 1,
+  ORACLE_TEMPORAL_DIRECT = 2,
+  ORACLE_TEMPORAL_INDIRECT = 3,
+  ORACLE_SPATIAL_DIRECT = 4,
+  ORACLE_SPATIAL_INDIRECT = 5,
+  ORACLE_RESOLVE_DIRECT = 6,
+  ORACLE_RESOLVE_INDIRECT = 7,
+  ORACLE_SVGF_TEMPORAL = 8,
+  ORACLE_SVGF_VARIANCE = 9,
+  ORACLE_SVGF_ATROUS = 10
+};
+
+int oracle_run_pass(int bits, int pass, const irt_frame* frame, const oracle_scene* scene,
+                    const oracle_gbuffer* gbuffer, oracle_io* io);
+
+/* Building blocks, exposed for known-answer tests. */
+float oracle_random(uint32_t* state);
+uint32_t oracle_seed(uint32_t i, uint32_t salt, uint32_t frame_index);
+double oracle_signed_box_distance(int bits, const double p[3], const double lo[3], const double hi[3]);
+double oracle_mesh_distance(int bits, const double p[3], const irt_mesh* mesh, const oracle_scene* scene);
+double oracle_scene_distance(int bits, const irt_frame* frame, const oracle_scene* scene, const double p[3],
+                             int need_spacing, uint32_t hint, double* voxel, uint32_t* closest);
+double oracle_visibility(int bits, const irt_frame* frame, const oracle_scene* scene, const double p[3],
+                         const double n[3], const double target[3], const double end_normal[3], uint64_t* counters);
+void oracle_brdf(int bits, const double surface[11], const double wo[3], const double wi[3], double out[3]);
+/* add() with the random draw supplied (random < 0: draw from *rng). */
+int oracle_reservoir_add(int bits, oracle_reservoir* r, uint32_t id, uint32_t source, double target, double pdf,
+                         double weight, uint32_t count, uint32_t age, double random, uint32_t* rng);
+double oracle_normalization(int bits, const oracle_reservoir* r);
+/* One soft-shadow update of the march (sdf.wgsl:180-181): returns the new result. */
+double oracle_shadow_step(int bits, double result, double h, double previous, double t, double k);
+uint16_t oracle_f32_to_f16(float f);
+float oracle_f16_to_f32(uint16_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""GPU parity against the CPU restatement (oracle), through the C ABI.
+
+Open loop: before every pass the GPU's own input state is read back and the
+oracle runs the same pass on exactly those inputs, so each pass is checked
+in isolation.  Bars: integer reservoir fields identical on >= 99.9 % of
+pixels; float fields of matching pixels within 1e-3 relative (observed
+~1e-6); radiance/moments within 1e-3 relative (denominator max(|ref|, 1e-4))
+on >= 99.9 % of pixels and image-level relative L2 <= 1e-3.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from instantradiositytest_amd.renderer import Renderer, half_to_float
+from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
+from parity import compare_images, compare_reservoirs, psnr, rgba16_to_f64, to_oracle_reservoirs
+
+pytestmark = pytest.mark.gpu
+
+RES_MATCH = 0.999
+IMG_WITHIN = 0.999
+REL_TOL = 1e-3
+FLOOR = 1e-4
+
+SCHED = [("candidates_direct", 2, 0, 2), ("candidates_indirect", 2, 0, 2),
+         ("temporal_direct", 0, 1, 2), ("temporal_indirect", 0, 1, 2),
+         ("spatial_direct", 1, 2, 0), ("spatial_indirect", 1, 2, 0)]
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_config(w=160, h=100, frames=6, name="plain-640x400"):
+    cfg = CONFIGS[name].scaled(w, h, frames)
+    return cfg, ConfigScene(cfg)
+
+
+def vpl_pools(r: Renderer, fi: int, V: int, slots: int):
+    ring = r.read("vpl_archive")
+    cur = ring[(fi % slots) * V:(fi % slots + 1) * V].copy()
+    if fi == 0:
+        prev = np.zeros_like(cur)
+    else:
+        ps = (fi - 1) % slots
+        prev = ring[ps * V:(ps + 1) * V].copy()
+    return cur, prev, ring
+
+
+def gbuffer_planes(r: Renderer, inputs) -> oracle.GBufferPlanes:
+    return oracle.GBufferPlanes(inputs.depth, rgba16_to_f64(inputs.normal), rgba16_to_f64(inputs.color),
+                                rgba16_to_f64(inputs.params), rgba16_to_f64(inputs.motion),
+                                r.read("previous_depth").astype(np.float64), rgba16_to_f64(r.read("previous_normal")))
+
+
+def check_stats(stats: dict):
+    bad = []
+    for key, s in stats.items():
+        if "match" in s and (s["match"] < RES_MATCH or s["max_rel"] > REL_TOL):
+            bad.append((key, s))
+        if "within" in s and (s["within"] < IMG_WITHIN or s["rel_l2"] > REL_TOL):
+            bad.append((key, s))
+    for key, s in bad:
+        log("FAIL", key, s)
+    assert not bad, f"{len(bad)} pass/frame comparisons outside the parity bar"
+
+
+def run_open_loop(precision: str, frames: int = 5, w: int = 160, h: int = 100, debug: int = 0):
+    cfg, cs = small_config(w, h, frames)
+    settings = cfg.settings()
+    r = Renderer(cs.scene, settings, precision=precision, schedule="passes")
+    r.debug_mode = debug
+    rec = oracle.SceneRecords.from_scene(cs.scene)
+    V, slots = cfg.vpl_count, settings.restir_max_history + 1
+    stats = {}
+    for fi in range(frames):
+        inputs = cs.frame_inputs(fi)
+        frame = r.begin_frame(cs.scene, inputs)
+        gb = gbuffer_planes(r, inputs)
+        cur, prev, ring = vpl_pools(r, fi, V, slots)
+        vp = dict(current_vpls=cur, previous_vpls=prev, archived_vpls=ring)
+        for name, i, o, hst in SCHED:
+            dom = "indirect" if name.endswith("indirect") else "direct"
+            din = to_oracle_reservoirs(r.read(f"direct{i}"))
+            iin = to_oracle_reservoirs(r.read(f"indirect{i}"))
+            dh = to_oracle_reservoirs(r.read(f"direct{hst}"))
+            ih = to_oracle_reservoirs(r.read(f"indirect{hst}"))
+            dout = np.zeros_like(din)
+            iout = np.zeros_like(iin)
+            r.run_pass(name)
+            oracle.run_pass(32, name, frame, rec, gb, direct_in=din, indirect_in=iin, direct_out=dout,
+                            indirect_out=iout, direct_history=dh, indirect_history=ih, **vp)
+            gpu = r.read(f"{dom}{o}")
+            stats[(fi, name)] = compare_reservoirs(gpu, dout if dom == "direct" else iout)
+        # resolves
+        d2 = to_oracle_reservoirs(r.read("direct2"))
+        i2 = to_oracle_reservoirs(r.read("indirect2"))
+        dl_ref = np.zeros((h, w, 4))
+        r.run_pass("resolve_direct")
+        oracle.run_pass(32, "resolve_direct", frame, rec, gb, direct_in=d2, indirect_in=i2, direct_output=dl_ref, **vp)
+        dl_gpu = r.read("direct_lighting").astype(np.float64)
+        stats[(fi, "resolve_direct")] = compare_images(dl_gpu, dl_ref, REL_TOL, FLOOR)
+        lt_ref = np.zeros((h, w, 4))
+        r.run_pass("resolve_indirect")
+        oracle.run_pass(32, "resolve_indirect", frame, rec, gb, direct_in=d2, indirect_in=i2,
+                        direct_lighting=np.ascontiguousarray(dl_gpu), output=lt_ref, **vp)
+        lt_gpu = rgba16_to_f64(r.read("lighting"))
+        stats[(fi, "resolve_indirect")] = compare_images(lt_gpu, lt_ref, REL_TOL, FLOOR)
+        # SVGF
+        noisy = lt_gpu
+        fb_old = rgba16_to_f64(r.read("feedback_old"))
+        mo_old = r.read("moments_old").astype(np.float64)
+        t_ref = np.zeros((h, w, 4))
+        m_ref = np.zeros((h, w, 4))
+        r.run_pass("svgf_temporal")
+        oracle.run_pass(32, "svgf_temporal", frame, None, gb, noisy=noisy, old_radiance=fb_old, old_moments=mo_old,
+                        input_radiance=fb_old, moments=mo_old, radiance_out=t_ref, moments_out=m_ref)
+        t_gpu = rgba16_to_f64(r.read("temporal"))
+        m_gpu = r.read("moments").astype(np.float64)
+        stats[(fi, "svgf_temporal")] = compare_images(t_gpu, t_ref, REL_TOL, FLOOR)
+        stats[(fi, "svgf_moments")] = compare_images(m_gpu, m_ref, REL_TOL, FLOOR)
+        s0_ref = np.zeros((h, w, 4))
+        r.run_pass("svgf_variance")
+        oracle.run_pass(32, "svgf_variance", frame, None, gb, noisy=noisy, input_radiance=t_gpu,
+                        moments=np.ascontiguousarray(m_gpu), radiance_out=s0_ref)
+        s0_gpu = rgba16_to_f64(r.read("scratch0"))
+        stats[(fi, "svgf_variance")] = compare_images(s0_gpu[..., [3, 3, 3]], s0_ref[..., [3, 3, 3]], REL_TOL, FLOOR)
+        for it in range(settings.svgf_iterations):
+            src = rgba16_to_f64(r.read(f"scratch{it % 2}"))
+            out_ref = np.zeros((h, w, 4))
+            r.run_pass("svgf_atrous", it)
+            oracle.run_pass(32, "svgf_atrous", frame, None, gb, noisy=noisy, input_radiance=src,
+                            moments=np.ascontiguousarray(m_gpu), radiance_out=out_ref, stride=1 << it)
+            out_gpu = rgba16_to_f64(r.read(f"scratch{1 - it % 2}"))
+            stats[(fi, f"svgf_atrous{it}")] = compare_images(out_gpu, out_ref, REL_TOL, FLOOR)
+            stats[(fi, f"svgf_atrous{it}_sigma")] = compare_images(out_gpu[..., [3, 3, 3]], out_ref[..., [3, 3, 3]],
+                                                                   REL_TOL, FLOOR)
+        r.end_frame()
+        r.finish_frame()
+    return stats
+
+
+@pytest.mark.parametrize("precision", ["fp32"])
+def test_open_loop_every_pass_matches_oracle(require_gpu, precision):
+    log(f"[parity] open loop precision={precision} 160x100 x5 frames")
+    stats = run_open_loop(precision)
+    worst = min((s.get("match", s.get("within")) for s in stats.values()))
+    log(f"[parity] worst match/within fraction {worst:.6f}")
+    for key, s in sorted(stats.items(), key=lambda kv: str(kv[0])):
+        if key[0] == 4:
+            log(key, s)
+    check_stats(stats)
+
+
+def run_frames(r: Renderer, cs: ConfigScene, frames: int):
+    for fi in range(frames):
+        r.encode(cs.scene, cs.frame_inputs(fi))
+    r.sync()
+
+
+def test_fused_schedule_matches_pass_schedule(require_gpu):
+    log("[parity] fused vs passes 160x100 x6 frames")
+    cfg, cs = small_config(160, 100, 6)
+    a = Renderer(cs.scene, cfg.settings(), schedule="passes")
+    b = Renderer(cs.scene, cfg.settings(), schedule="fused")
+    run_frames(a, cs, 6)
+    run_frames(b, cs, 6)
+    for name in ("direct2", "indirect2"):
+        s = compare_reservoirs(b.read(name), to_oracle_reservoirs(a.read(name)))
+        log(name, s)
+        assert s["match"] >= RES_MATCH
+    for name in ("lighting", "history", "feedback"):
+        s = compare_images(rgba16_to_f64(b.read(name)), rgba16_to_f64(a.read(name)), REL_TOL, FLOOR)
+        log(name, s)
+        assert s["within"] >= 0.99 and s["rel_l2"] <= 1e-2
+    s = compare_images(b.read("moments").astype(np.float64), a.read("moments").astype(np.float64), REL_TOL, FLOOR)
+    log("moments", s)
+    assert s["within"] >= 0.99
+
+
+def test_closed_loop_frames_track_oracle(require_gpu):
+    """Eight frames end to end on both sides (each side feeds itself)."""
+    log("[parity] closed loop 160x100 x8 frames")
+    cfg, cs = small_config(160, 100, 8)
+    settings = cfg.settings()
+    r = Renderer(cs.scene, settings, schedule="fused")
+    rec = oracle.SceneRecords.from_scene(cs.scene)
+    o32 = oracle.OracleRenderer(32, cfg.width, cfg.height, cfg.vpl_count, settings.restir_max_history)
+    o64 = oracle.OracleRenderer(64, cfg.width, cfg.height, cfg.vpl_count, settings.restir_max_history)
+    for fi in range(8):
+        inputs = cs.frame_inputs(fi)
+        frame = r.encode(cs.scene, inputs)
+        o32.encode(frame, rec, inputs, settings.svgf_iterations, 0)
+        o64.encode(frame, rec, inputs, settings.svgf_iterations, 0)
+    r.sync()
+    lt = rgba16_to_f64(r.read("lighting"))
+    hist = rgba16_to_f64(r.read("history"))
+    s_lt = compare_images(lt, o32.lighting, REL_TOL, FLOOR)
+    s_h = compare_images(hist, o32.histories[7 % 2], REL_TOL, FLOOR)
+    p32 = psnr(hist, o32.histories[1])
+    p64 = psnr(hist, o64.histories[1])
+    p_o = psnr(o32.histories[1], o64.histories[1])
+    log("closed loop lighting", s_lt)
+    log("closed loop history", s_h)
+    log(f"PSNR gpu-vs-fp32 {p32:.1f} dB, gpu-vs-fp64 {p64:.1f} dB, fp32-oracle-vs-fp64 {p_o:.1f} dB")
+    assert p64 >= 45.0
+    assert s_lt["rel_l2"] <= 1e-2

@@ -1,0 +1,252 @@
+"""C ABI library and host precompute (CPU only, no device calls).
+
+Known answers come from the reference's own unit tests:
+settings.rs:112-131, renderer.rs:2170-2261, scene.rs:430-444.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from instantradiositytest_amd import _lib
+from instantradiositytest_amd.scenes import look_at_camera, quat_from_matrix
+from instantradiositytest_amd.types import (CAMERA_DT, FRAME_DT, LIGHT_DT, MESH_DT, NODE_DT, OBJECT_DT,
+                                            RESERVOIR_DT, VPL_DT, Settings, make_settings, ptr)
+from instantradiositytest_amd.renderer import pack_frame
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions() -> list[str]:
+    text = (ROOT / "include" / "irt.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(irt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.irt()
+    names = declared_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_abi_version():
+    assert _lib.irt().irt_abi_version() == 1
+
+
+def test_record_sizes_match_reference_layouts():
+    # renderer.rs:2254-2261 (Frame 384, GpuObject 208, GpuMesh 128, GpuLight 48) and common.wgsl:14-16.
+    assert FRAME_DT.itemsize == 384
+    assert OBJECT_DT.itemsize == 208
+    assert MESH_DT.itemsize == 128
+    assert LIGHT_DT.itemsize == 48
+    assert NODE_DT.itemsize == 48
+    assert VPL_DT.itemsize == 48
+    assert RESERVOIR_DT.itemsize == 32
+    assert ctypes.sizeof(Settings) == 80
+    # field offsets of the Frame uniform (common.wgsl:3-8)
+    off = {n: FRAME_DT.fields[n][1] for n in FRAME_DT.names}
+    assert off == {"vp": 0, "inv_vp": 64, "previous_vp": 128, "previous_inv_vp": 192, "camera": 256, "size": 272,
+                   "counts": 288, "restir": 304, "temporal": 320, "sdf": 336, "rr": 352, "flags": 368}
+
+
+def validate(s: Settings) -> int:
+    return _lib.irt().irt_settings_validate(ctypes.byref(s))
+
+
+def test_default_settings_validate():
+    assert validate(make_settings()) == 0
+
+
+@pytest.mark.parametrize("edit", [
+    dict(render_width=0), dict(vpl_count=0), dict(temporal_max_history=0), dict(restir_max_history=0),
+    dict(restir_max_history=33), dict(svgf_iterations=0), dict(svgf_iterations=6),
+    dict(sdf_hit_epsilon=float("inf")), dict(vpl_rr_max_survival=1.0), dict(depth_threshold=float("nan")),
+])
+def test_invalid_settings_are_rejected(edit):
+    # settings.rs:112-131
+    assert validate(make_settings(**edit)) == -2
+
+
+def test_settings_messages_follow_reference():
+    L = _lib.irt()
+    assert validate(make_settings(render_width=9000)) == -2
+    assert L.irt_global_error() == b"invalid render size"
+    assert validate(make_settings(sdf_max_steps=0)) == -2
+    assert L.irt_global_error() == b"invalid SDF settings"
+    with pytest.raises(KeyError):
+        make_settings(unknown_field=1)
+
+
+def box_vertices():
+    v = np.array([[(-1 if i & 1 == 0 else 1), (-1 if i & 2 == 0 else 1), (-1 if i & 4 == 0 else 1)]
+                  for i in range(8)], np.float32)
+    idx = np.array([0, 2, 6, 0, 6, 4, 1, 5, 7, 1, 7, 3, 0, 4, 5, 0, 5, 1, 2, 3, 7, 2, 7, 6, 0, 1, 3, 0, 3, 2,
+                    4, 6, 7, 4, 7, 5], np.uint32)
+    return v, idx
+
+
+def is_box(v, idx, lo=(-1, -1, -1), hi=(1, 1, 1)):
+    lo = np.array(lo, np.float32)
+    hi = np.array(hi, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    return _lib.irt().irt_host_is_box_geometry(ptr(v), len(v), 3, ptr(idx), len(idx), ptr(lo), ptr(hi))
+
+
+def test_box_certification_known_answers():
+    # renderer.rs:2197-2230
+    v, idx = box_vertices()
+    assert is_box(v, idx) == 1
+    assert is_box(v, idx[:33]) == 0
+    overlap = idx.copy()
+    overlap[3:6] = idx[:3]
+    assert is_box(v, overlap) == 0
+    hole = idx.copy()
+    hole[3:6] = [0, 2, 4]
+    assert is_box(v, hole) == 0
+    nonface = idx.copy()
+    nonface[:3] = [0, 3, 7]
+    assert is_box(v, nonface) == 0
+    moved = v.copy()
+    moved[0, 0] += 0.001
+    assert is_box(moved, idx) == 0
+
+
+def lipschitz(samples, res, spacing):
+    s = np.ascontiguousarray(samples, np.float32)
+    sp = np.ascontiguousarray(spacing, np.float32)
+    return _lib.irt().irt_host_field_lipschitz_bound(ptr(s), res, ptr(sp))
+
+
+def test_lipschitz_bound_known_answers():
+    # renderer.rs:2170-2195
+    spacing = np.array([0.25, 0.5, 1.0], np.float32)
+    samples = np.array([10 + np.dot(np.array([(i & 1), (i >> 1) & 1, i >> 2], np.float32) * spacing, [1, 2, 3])
+                        for i in range(8)], np.float32)
+    b = lipschitz(samples, 2, spacing)
+    assert b >= math.sqrt(14) * (1 - 1e-6)
+    assert b < math.sqrt(14) * 1.001
+    assert lipschitz(np.zeros(8), 2, spacing) >= 1.0
+    assert lipschitz(np.full(8, -100.0), 2, spacing) >= 1.0
+    n = 16
+    i = np.arange(n ** 3)
+    p = np.stack([i % n, (i // n) % n, i // (n * n)], axis=1).astype(np.float32) * 0.2 - 1.5
+    sphere = np.linalg.norm(p, axis=1) - 1.0
+    sb = lipschitz(sphere, n, np.full(3, 0.2, np.float32))
+    assert 1.0 <= sb < 1.2
+
+
+def hierarchy(lo, hi):
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    n = len(lo)
+    nodes = np.zeros(2 * n - 1, NODE_DT)
+    assert _lib.irt().irt_host_hierarchy(ptr(lo), ptr(hi), n, ptr(nodes)) == 2 * n - 1
+    return nodes
+
+
+def test_hierarchy_preorder_escape_links():
+    # three unit boxes along x: root splits on x, median n/2 = 1 -> left = {0}, right = {1, 2}
+    lo = np.array([[0, 0, 0], [2, 0, 0], [4, 0, 0]], np.float32)
+    hi = lo + 1
+    nodes = hierarchy(lo, hi)
+    info = nodes["info"]
+    # preorder: 0 root, 1 leaf(obj 0), 2 inner, 3 leaf(obj 1), 4 leaf(obj 2)
+    assert list(info[0]) == [1, 2, 0, 5]
+    assert list(info[1]) == [0, 0, 1, 2]
+    assert list(info[2]) == [3, 4, 0, 5]
+    assert list(info[3]) == [1, 0, 1, 4]
+    assert list(info[4]) == [2, 0, 1, 5]
+    assert np.allclose(nodes["lo"][0][:3], [0, 0, 0]) and np.allclose(nodes["hi"][0][:3], [5, 1, 1])
+
+
+def test_hierarchy_invariants_random():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 27, 48):
+        lo = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+        hi = lo + rng.uniform(0.05, 1.0, (n, 3)).astype(np.float32)
+        nodes = hierarchy(lo, hi)
+        leaves = sorted(int(nd["info"][0]) for nd in nodes if nd["info"][2] == 1)
+        assert leaves == list(range(n))
+        for i, nd in enumerate(nodes):
+            esc = int(nd["info"][3])
+            assert i < esc <= len(nodes)
+            if nd["info"][2] != 1:
+                left, right = int(nd["info"][0]), int(nd["info"][1])
+                assert left == i + 1 and right == int(nodes[left]["info"][3])
+                assert int(nodes[right]["info"][3]) == esc
+                for c in (left, right):
+                    assert np.all(nodes[c]["lo"][:3] >= nd["lo"][:3]) and np.all(nodes[c]["hi"][:3] <= nd["hi"][:3])
+
+
+def test_atlas_grid_matches_reference_rule():
+    # renderer.rs:778-791
+    L = _lib.irt()
+    for m in (1, 2, 6, 7, 8, 9, 24, 27, 28, 48):
+        g = np.zeros(3, np.uint32)
+        L.irt_host_atlas_grid(m, 64, ptr(g))
+        gx, gy, gz = (int(x) for x in g)
+        assert gx ** 3 >= m and (gx - 1) ** 3 < m
+        assert gx * gy * gy >= m and gx * (gy - 1) ** 2 < m
+        assert gz == -(-m // (gx * gy))
+
+
+def test_pack_frame_fields():
+    s = make_settings(render_width=320, render_height=200)
+    vp = np.eye(4, dtype=np.float32).reshape(-1) * 2
+    vp[15] = 1
+    f = pack_frame(s, vp, vp, np.array([1, 2, 3], np.float32), 7, 3, 2, True, 1, 4, 10)[0]
+    assert list(f["size"]) == [320, 200, 7, 3]
+    assert list(f["counts"]) == [4, 100, 3, 10]
+    assert list(f["restir"]) == [8, 8, 4, 12]
+    assert np.allclose(f["temporal"], [0.9, 0.03, 0.85, 32])
+    assert np.allclose(f["sdf"], [96, 0.001, 16, 0])
+    assert np.allclose(f["rr"], [0.1, 0.95, 4, 0])
+    assert list(f["flags"]) == [1, 2, 1, 0]
+    inv = f["inv_vp"].reshape(4, 4)
+    assert np.allclose(inv[:3, :3], np.eye(3) * 0.5) and inv[3, 3] == 1
+
+
+def test_depth_reconstruction_roundtrip():
+    # scene.rs:430-444: camera at (4,-3,2) rotated 0.3 rad about z, fov 1, near .1, far 100, aspect 1.6
+    sg = _lib.scenegen()
+    c, s = math.cos(0.3), math.sin(0.3)
+    rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    cam = np.zeros((), CAMERA_DT)
+    cam["position"] = (4, -3, 2)
+    cam["rotation"] = quat_from_matrix(rot)
+    cam["fov_y"], cam["near_plane"], cam["far_plane"] = 1.0, 0.1, 100.0
+    vp = np.zeros(16, np.float32)
+    sg.sg_camera_matrix(ptr(np.ascontiguousarray(cam)), ctypes.c_float(1.6), ptr(vp))
+    inv = np.zeros(16, np.float32)
+    assert _lib.irt().irt_host_inverse(ptr(vp), ptr(inv)) == 1
+    M = vp.reshape(4, 4).T.astype(np.float64)   # column-major storage
+    Mi = inv.reshape(4, 4).T.astype(np.float64)
+    p = np.array([4, -3, 2]) + rot @ np.array([0.2, 0.1, -4.0])
+    h = M @ np.append(p, 1)
+    ndc = h / h[3]
+    assert 0 < ndc[2] < 1
+    w = Mi @ ndc
+    assert np.linalg.norm(w[:3] / w[3] - p) < 1e-4
+
+
+def test_binary16_conversion_round_to_nearest_even():
+    from instantradiositytest_amd.renderer import float_to_half
+    import oracle
+    L = oracle.lib()
+    rng = np.random.default_rng(1)
+    vals = np.concatenate([rng.normal(0, 1, 3000), rng.normal(0, 3e4, 500), rng.uniform(-1e-4, 1e-4, 500),
+                           [0.0, -0.0, 65504.0, 65519.0, 65520.0, 1e-8, 6e-8, 2.0 ** -24, 1.5 * 2 ** -24]])
+    vals = vals.astype(np.float32)
+    ref = float_to_half(vals)
+    got = np.array([L.oracle_f32_to_f16(float(v)) for v in vals], np.uint16)
+    assert np.array_equal(ref, got)
+    back = np.array([L.oracle_f16_to_f32(int(b)) for b in ref], np.float32)
+    assert np.array_equal(back, ref.view(np.float16).astype(np.float32))
