@@ -28,6 +28,11 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # One IEEE rounding per source operation, as in the FP32 restatement:
+    # no multiply-add contraction.  Hot loops write fmaf() explicitly where
+    # they deviate on purpose.
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-munsafe-fp-atomics",
     "-Wall",
     "-Wno-unused-function",

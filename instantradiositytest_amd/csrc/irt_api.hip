@@ -101,6 +101,8 @@ struct irt_ctx {
   void* feedback[2] = {nullptr, nullptr};
   void* history = nullptr;
   float4* moments[2] = {nullptr, nullptr};
+  GuidePx* guide = nullptr;  // per-pixel guide records of the current G-buffer (fused schedule)
+  bool guide_dirty = true;
   unsigned long long* counters = nullptr;
   bool counting = false;
   bool timing = false;
@@ -337,6 +339,19 @@ int check_ready(irt_ctx* c) {
   return IRT_OK;
 }
 
+// Rebuilds the guide records when the G-buffer or the frame changed.
+template <class P>
+int ensure_guide(irt_ctx* c) {
+  if (!c->guide_dirty) return IRT_OK;
+  const uint32_t n = (uint32_t)c->npx;
+  hipLaunchKernelGGL((k_guide<P>), dim3((n + 255) / 256), dim3(256), 0, c->stream, c->frame, c->depth,
+                     (const typename P::N4*)c->normal, (const typename P::N4*)c->color,
+                     (const typename P::P2*)c->params, c->guide);
+  HIP_CHECK(c, hipGetLastError());
+  c->guide_dirty = false;
+  return IRT_OK;
+}
+
 // One reservoir/resolve pass with the reference's bindings (pass_group, renderer.rs:1617-1674).
 template <class P>
 int lighting_pass(irt_ctx* c, uint32_t pass) {
@@ -360,10 +375,16 @@ int run_lighting(irt_ctx* c) {
   if (rc) return rc;
   if (c->schedule == IRT_SCHEDULE_FUSED) {
     timer_begin(c, "reservoirs_a");
-    LAUNCH(c, k_reservoirs_a, P, light_args<P>(c, 2, 1, 2), c->direct[0], c->indirect[0]);
+    rc = ensure_guide<P>(c);
+    if (rc) return rc;
+    LightArgs<P> a = light_args<P>(c, 2, 1, 2);
+    a.guide = c->guide;
+    LAUNCH(c, k_reservoirs_a, P, a, c->direct[0], c->indirect[0]);
     timer_end(c);
     timer_begin(c, "reservoirs_b_resolve");
-    LAUNCH(c, k_reservoirs_b, P, light_args<P>(c, 1, 2, 0));
+    LightArgs<P> b = light_args<P>(c, 1, 2, 0);
+    b.guide = c->guide;
+    LAUNCH(c, k_reservoirs_b, P, b);
     timer_end(c);
     return IRT_OK;
   }
@@ -380,9 +401,10 @@ int run_lighting(irt_ctx* c) {
 }
 
 template <class P>
-int svgf_temporal_pass(irt_ctx* c) {
+int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u, old = 1u - cur;
   SvgfArgs<P> a = svgf_args<P>(c);
+  a.guide = guide;
   a.old_radiance = (const typename P::R4*)c->feedback[old];
   a.old_moments = c->moments[old];
   a.input = (const typename P::R4*)c->feedback[old];
@@ -393,9 +415,10 @@ int svgf_temporal_pass(irt_ctx* c) {
   return IRT_OK;
 }
 template <class P>
-int svgf_variance_pass(irt_ctx* c) {
+int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u;
   SvgfArgs<P> a = svgf_args<P>(c);
+  a.guide = guide;
   a.input = (const typename P::R4*)c->temporal;
   a.moments = c->moments[cur];
   a.out = (typename P::R4*)c->scratch[0];
@@ -403,9 +426,10 @@ int svgf_variance_pass(irt_ctx* c) {
   return IRT_OK;
 }
 template <class P>
-int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2) {
+int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u;
   SvgfArgs<P> a = svgf_args<P>(c);
+  a.guide = guide;
   a.input = (const typename P::R4*)in;
   a.moments = c->moments[cur];
   a.out = (typename P::R4*)out;
@@ -430,10 +454,18 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
     timer_end(c);
     return IRT_OK;
   }
-  int rc = svgf_temporal_pass<P>(c);
-  if (!rc) rc = svgf_variance_pass<P>(c);
+  const bool fused = c->schedule == IRT_SCHEDULE_FUSED;
+  int rc = IRT_OK;
+  const GuidePx* guide = nullptr;
+  if (fused) {
+    rc = ensure_guide<P>(c);
+    if (rc) return rc;
+    guide = c->guide;
+  }
+  rc = svgf_temporal_pass<P>(c, guide);
+  if (!rc) rc = svgf_variance_pass<P>(c, guide);
   if (rc) return rc;
-  if (c->schedule == IRT_SCHEDULE_FUSED) {
+  if (fused) {
     // Feedback and final copies folded into the a-trous stores.
     void* in = c->scratch[0];
     for (uint32_t i = 0; i < iterations; ++i) {
@@ -448,7 +480,7 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
       } else {
         out = in == c->scratch[1] ? c->scratch[0] : c->scratch[1];
       }
-      rc = svgf_atrous_pass<P>(c, i, in, out, out2);
+      rc = svgf_atrous_pass<P>(c, i, in, out, out2, guide);
       if (rc) return rc;
       in = out;
     }
@@ -590,6 +622,7 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   A((void**)&c->moments[0], n * 16);
   A((void**)&c->moments[1], n * 16);
   A((void**)&c->counters, 8 * sizeof(unsigned long long));
+  A((void**)&c->guide, n * sizeof(GuidePx));
   // Depth targets start cleared to 1 (far plane) as the reference's depth pass clears them.
   if (!rc) {
     std::vector<float> ones(n, 1.0f);
@@ -723,6 +756,7 @@ int irt_set_frame(irt_ctx* c, const irt_frame* f) {
   if (f->counts[3] == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame must have at least one object");
   c->frame = *f;
   c->have_frame = true;
+  c->guide_dirty = true;
   return IRT_OK;
 }
 
@@ -731,6 +765,7 @@ int irt_set_gbuffer(irt_ctx* c, const float* depth, const uint16_t* normal, cons
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
   const size_t n = c->npx;
+  c->guide_dirty = true;
   HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyHostToDevice, c->stream));
   if (mixed(c)) {
     HIP_CHECK(c, hipMemcpyAsync(c->normal, normal, n * 8, hipMemcpyHostToDevice, c->stream));
@@ -754,6 +789,7 @@ int irt_set_gbuffer_device(irt_ctx* c, const float* depth, const uint16_t* norma
                            const uint16_t* params, const uint16_t* motion) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
+  c->guide_dirty = true;
   if (mixed(c)) {
     const size_t n = c->npx;
     HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyDeviceToDevice, c->stream));
@@ -838,6 +874,7 @@ int irt_write(irt_ctx* c, uint32_t b, const void* src, size_t bytes) {
   if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "write size mismatch");
   std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
   to_internal(d.fmt, (const uint8_t*)src, tmp.data(), d.count);
+  if (b == IRT_BUF_DEPTH || b == IRT_BUF_NORMAL || b == IRT_BUF_COLOR || b == IRT_BUF_PARAMS) c->guide_dirty = true;
   HIP_CHECK(c, hipMemcpyAsync(d.ptr, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return IRT_OK;

@@ -33,9 +33,10 @@ __device__ __forceinline__ v3 operator*(float s, v3 a) { return v3{s * a.x, s * 
 __device__ __forceinline__ v3 operator/(v3 a, float s) { return v3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float length(v3 a) { return sqrtf(dot(a, a)); }
+// WGSL normalize(e) = e / length(e), divided per component.
 __device__ __forceinline__ v3 normalize(v3 a) {
-  float s = 1.0f / sqrtf(dot(a, a));
-  return a * s;
+  float l = sqrtf(dot(a, a));
+  return v3{a.x / l, a.y / l, a.z / l};
 }
 __device__ __forceinline__ v3 vmax(v3 a, v3 b) { return v3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
 __device__ __forceinline__ v3 vmin(v3 a, v3 b) { return v3{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
@@ -99,6 +100,17 @@ __device__ __forceinline__ void st_r16(h4* p, uint32_t i, float4 v) {
 __device__ __forceinline__ void st4(float4* p, uint32_t i, float4 v) { p[i] = v; }
 __device__ __forceinline__ void st4(h4* p, uint32_t i, float4 v) { st_r16(p, i, v); }
 
+// Per-pixel guide record, built once per frame by k_guide: the world
+// position of the depth sample (FP32, computed exactly as world() does) and
+// the binary16 G-buffer channels that neighbourhood taps read.  The G-buffer
+// holds binary16 values in both profiles, so the record is lossless.  32 B.
+struct __align__(16) GuidePx {
+  float px, py, pz;
+  uint32_t valid;  // depth < 1
+  _Float16 nx, ny, nz, cr, cg, cb, m0, m1;
+};
+static_assert(sizeof(GuidePx) == 32, "guide record is 32 bytes");
+
 // Storage profiles.
 struct ProfileF32 {
   typedef float4 N4;  // normal, colour (xyz/rgb, w)
@@ -159,7 +171,8 @@ struct Surface {
   float rough, metal;
 };
 
-// x^n for the two fixed exponents used by the path.
+// pow(x, 5) and pow(x, 64) for x >= 0 as fixed multiply chains: (x^2)^2 * x
+// and six squarings.  The FP32 restatement evaluates the same chains.
 __device__ __forceinline__ float pow5(float x) {
   float x2 = x * x;
   return x2 * x2 * x;
@@ -205,7 +218,7 @@ __device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, v3 ca
   v3 lp = v3{l.position_strength[0], l.position_strength[1], l.position_strength[2]};
   v3 delta = lp - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0001f);
-  v3 wi = delta * rsqrtf(r2);
+  v3 wi = delta * (1.0f / sqrtf(r2));  // inverseSqrt(r2)
   if (l.color_type[3] > 0.5f) {
     v3 axis = normalize(v3{l.direction_angle[0], l.direction_angle[1], l.direction_angle[2]});
     if (dot(-wi, axis) < cosf(l.direction_angle[3] * 0.5f)) return v3{0.f, 0.f, 0.f};
@@ -223,7 +236,7 @@ template <class C>
 __device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, v3 camera, C& cnt) {
   v3 delta = ld3(v.position) - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0025f);
-  v3 wi = delta * rsqrtf(r2);
+  v3 wi = delta * (1.0f / sqrtf(r2));  // inverseSqrt(r2)
   cnt.add(C_BRDF, 1);
   v3 b = brdf(s, normalize(camera - s.p), wi);
   float k1 = fmaxf(dot(s.n, wi), 0.0f);
@@ -275,6 +288,24 @@ __device__ __forceinline__ irt_reservoir cap(irt_reservoir a, uint32_t limit) {
 }
 
 // --------------------------------------------------------------- SDF
+// Candidate lists of a uniform grid over the scene.  Each cell lists every
+// object that can hold the minimum distance anywhere in the cell (objects
+// whose lower bound over the cell does not exceed the smallest upper bound
+// over the cell), nearest lower bound first.  count == 255: list overflow,
+// queries in that cell walk the hierarchy instead.
+struct CellList {
+  uint8_t count;
+  uint8_t ids[15];
+};
+static_assert(sizeof(CellList) == 16, "cell list is 16 bytes");
+constexpr uint8_t CELL_OVERFLOW = 255;
+struct GridView {
+  const CellList* cells;  // null: no grid, hierarchy walk only
+  const float4* boxes;    // per object: world AABB lo (xyz), hi (xyz) as two float4
+  float x0, y0, z0, inv_cell;
+  uint32_t nx, ny, nz;
+};
+
 // Scene records as the kernels see them.
 struct SceneView {
   const irt_object* objects;
@@ -283,6 +314,7 @@ struct SceneView {
   const irt_light* lights;
   const float* atlas;  // reference atlas layout: x + w*(y + h*z)
   uint32_t atlas_w, atlas_h;
+  GridView grid;
 };
 
 // signed_box_distance — sdf.wgsl:1-4.
@@ -303,15 +335,16 @@ __device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_
   float t000 = base[0], t100 = base[1], t010 = base[sy], t110 = base[sy + 1];
   float t001 = base[sz], t101 = base[sz + 1], t011 = base[sz + sy], t111 = base[sz + sy + 1];
   float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+  // value += texel * w.x * w.y * w.z, corners in z, y, x order (common.wgsl:33-36).
   float value = 0.0f;
-  value += t000 * (gx * gy * gz);
-  value += t100 * (fx * gy * gz);
-  value += t010 * (gx * fy * gz);
-  value += t110 * (fx * fy * gz);
-  value += t001 * (gx * gy * fz);
-  value += t101 * (fx * gy * fz);
-  value += t011 * (gx * fy * fz);
-  value += t111 * (fx * fy * fz);
+  value += t000 * gx * gy * gz;
+  value += t100 * fx * gy * gz;
+  value += t010 * gx * fy * gz;
+  value += t110 * fx * fy * gz;
+  value += t001 * gx * gy * fz;
+  value += t101 * fx * gy * fz;
+  value += t011 * gx * fy * fz;
+  value += t111 * fx * fy * fz;
   return value;
 }
 
@@ -360,10 +393,50 @@ struct Distance {
   uint32_t object_id;
 };
 
-// scene_distance — sdf.wgsl:42-60.  Stackless preorder walk with escape links.
+// scene_distance — sdf.wgsl:42-60.  The reference walks a stackless preorder
+// hierarchy; its result is the minimum object distance (pruning only drops
+// objects whose conservative lower bound exceeds the running minimum).  When
+// the candidate grid is present, the same minimum is taken over the cell's
+// candidate list with the same per-object bound test.
+template <class C>
+__device__ __forceinline__ bool scene_distance_grid(const SceneView& sc, float hit_eps, v3 p, bool need_spacing,
+                                                    uint32_t hint, C& cnt, Distance* out) {
+  const GridView& g = sc.grid;
+  const float fx = (p.x - g.x0) * g.inv_cell, fy = (p.y - g.y0) * g.inv_cell, fz = (p.z - g.z0) * g.inv_cell;
+  if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)g.nx && fy < (float)g.ny && fz < (float)g.nz))
+    return false;
+  const uint32_t cx = min((uint32_t)fx, g.nx - 1u), cy = min((uint32_t)fy, g.ny - 1u), cz = min((uint32_t)fz, g.nz - 1u);
+  const CellList L = g.cells[((size_t)cz * g.ny + cy) * g.nx + cx];
+  if (L.count == CELL_OVERFLOW) return false;
+  float best = 1e20f, voxel = hit_eps;
+  uint32_t closest = hint;
+  for (uint32_t k = 0; k < L.count; ++k) {
+    const uint32_t j = L.ids[k];
+    cnt.add(C_NODE, 1);
+    const float4 lo = g.boxes[2 * j], hi = g.boxes[2 * j + 1];
+    v3 delta = vmaxs(vmax(v3{lo.x - p.x, lo.y - p.y, lo.z - p.z}, v3{p.x - hi.x, p.y - hi.y, p.z - hi.z}), 0.0f);
+    v3 lb = delta * 0.9998f;
+    float bp = fmaxf(best, 0.0f);
+    if (dot(lb, lb) > bp * bp) continue;
+    float vx;
+    float d = object_distance(sc, p, j, need_spacing, hit_eps, &vx, cnt);
+    if (d < best) {
+      best = d;
+      voxel = vx;
+      closest = j;
+    }
+  }
+  *out = Distance{best, voxel, closest};
+  return true;
+}
+
 template <class C>
 __device__ __forceinline__ Distance scene_distance(const SceneView& sc, uint32_t n_objects, float hit_eps, v3 p,
                                                    bool need_spacing, uint32_t hint, C& cnt) {
+  if (sc.grid.cells) {
+    Distance d;
+    if (scene_distance_grid(sc, hit_eps, p, need_spacing, hint, cnt, &d)) return d;
+  }
   float best = 1e20f, voxel = hit_eps;
   uint32_t closest = hint;
   if (hint < n_objects) best = object_distance(sc, p, hint, need_spacing, hit_eps, &voxel, cnt);

@@ -31,7 +31,7 @@ __device__ __forceinline__ Px pixel(uint32_t W, uint32_t H) {
 
 template <class P>
 __device__ __forceinline__ bool valid_surface(const LightArgs<P>& a, const Px& p, Surface* s) {
-  bool valid = p.active && a.depth[p.id] < 1.0f;
+  bool valid = p.active && (a.guide ? a.guide[p.id].valid != 0u : a.depth[p.id] < 1.0f);
   if (valid) {
     *s = load_surface(a, p.x, p.y);
   } else {
@@ -214,6 +214,36 @@ __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
     if (a.out2) st_r16(a.out2, p.id, r);
   }
   cnt.flush(a.counters);
+}
+
+// Guide records of one frame: world position (world(), common.wgsl:46) and
+// the binary16 channels of the G-buffer.
+template <class P>
+__global__ __launch_bounds__(256) void k_guide(irt_frame F, const float* depth, const typename P::N4* normal,
+                                               const typename P::N4* color, const typename P::P2* params,
+                                               GuidePx* out) {
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  const int32_t x = (int32_t)(i % W), y = (int32_t)(i / W);
+  const float d = depth[i];
+  const v3 p = world(pixel_u(x, W), pixel_u(y, H), d, F.inv_vp);
+  const float4 n = ld4(normal, i), c = ld4(color, i);
+  const float2 m = ld2(params, i);
+  GuidePx g;
+  g.px = p.x;
+  g.py = p.y;
+  g.pz = p.z;
+  g.valid = d < 1.0f ? 1u : 0u;
+  g.nx = (_Float16)n.x;
+  g.ny = (_Float16)n.y;
+  g.nz = (_Float16)n.z;
+  g.cr = (_Float16)c.x;
+  g.cg = (_Float16)c.y;
+  g.cb = (_Float16)c.z;
+  g.m0 = (_Float16)m.x;
+  g.m1 = (_Float16)m.y;
+  out[i] = g;
 }
 
 // ------------------------------------------------------------- format I/O

@@ -30,12 +30,23 @@ struct LightArgs {
   typename P::L4* direct_lighting;
   typename P::R4* lighting;
   unsigned long long* counters;
+  const GuidePx* guide;  // when set, surface data comes from the per-pixel guide records
 };
 
 template <class P>
 __device__ __forceinline__ Surface load_surface(const LightArgs<P>& a, int32_t x, int32_t y) {
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  if (a.guide) {
+    const GuidePx g = a.guide[id];
+    Surface s;
+    s.p = v3{g.px, g.py, g.pz};
+    s.n = normalize(v3{(float)g.nx, (float)g.ny, (float)g.nz});
+    s.color = v3{(float)g.cr, (float)g.cg, (float)g.cb};
+    s.rough = (float)g.m0;
+    s.metal = (float)g.m1;
+    return s;
+  }
   float2 rm = ld2(a.params, id);
   Surface s;
   s.p = world(pixel_u(x, W), pixel_u(y, H), a.depth[id], a.F.inv_vp);
@@ -65,10 +76,18 @@ template <class P>
 __device__ __forceinline__ bool compatible(const LightArgs<P>& a, const Surface& s, int32_t qx, int32_t qy) {
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
-  float d = a.depth[q];
-  if (d >= 1.0f) return false;
-  v3 p = world(pixel_u(qx, W), pixel_u(qy, H), d, a.F.inv_vp);
-  v3 n = normalize(xyz(ld4(a.normal, q)));
+  v3 p, n;
+  if (a.guide) {
+    const GuidePx g = a.guide[q];
+    if (!g.valid) return false;
+    p = v3{g.px, g.py, g.pz};
+    n = normalize(v3{(float)g.nx, (float)g.ny, (float)g.nz});
+  } else {
+    float d = a.depth[q];
+    if (d >= 1.0f) return false;
+    p = world(pixel_u(qx, W), pixel_u(qy, H), d, a.F.inv_vp);
+    n = normalize(xyz(ld4(a.normal, q)));
+  }
   v3 cam = ld3(a.F.camera);
   return dot(s.n, n) > a.F.temporal[2] && length(s.p - p) < a.F.temporal[1] * fmaxf(length(s.p - cam), 1.0f);
 }
@@ -307,6 +326,7 @@ struct SvgfArgs {
   float4* moments_out;
   uint32_t stride;
   unsigned long long* counters;
+  const GuidePx* guide;  // when set, positions and G-buffer channels come from the guide records
 };
 
 // inside, position, history_ok, guide, geometry_weight — svgf.wgsl:18-44.
@@ -322,7 +342,24 @@ __device__ __forceinline__ bool inside(const SvgfArgs<P>& a, int32_t x, int32_t 
 template <class P>
 __device__ __forceinline__ v3 position(const SvgfArgs<P>& a, int32_t x, int32_t y) {
   const uint32_t W = a.F.size[0], H = a.F.size[1];
-  return world(pixel_u(x, W), pixel_u(y, H), a.depth[(uint32_t)y * W + (uint32_t)x], a.F.inv_vp);
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  if (a.guide) {
+    const GuidePx g = a.guide[id];
+    return v3{g.px, g.py, g.pz};
+  }
+  return world(pixel_u(x, W), pixel_u(y, H), a.depth[id], a.F.inv_vp);
+}
+template <class P>
+__device__ __forceinline__ bool valid_px(const SvgfArgs<P>& a, uint32_t id) {
+  return a.guide ? a.guide[id].valid != 0u : a.depth[id] < 1.0f;
+}
+template <class P>
+__device__ __forceinline__ v3 normal_px(const SvgfArgs<P>& a, uint32_t id) {
+  if (a.guide) {
+    const GuidePx g = a.guide[id];
+    return v3{(float)g.nx, (float)g.ny, (float)g.nz};
+  }
+  return xyz(ld4(a.normal, id));
 }
 template <class P>
 __device__ __forceinline__ bool history_ok(const SvgfArgs<P>& a, int32_t px, int32_t py, int32_t qx, int32_t qy) {
@@ -336,19 +373,28 @@ __device__ __forceinline__ bool history_ok(const SvgfArgs<P>& a, int32_t px, int
   v3 pb = world(pixel_u(qx, W), pixel_u(qy, H), d, F.previous_inv_vp);
   v3 cam = ld3(F.camera);
   return length(pa - pb) < F.temporal[1] * fmaxf(length(pa - cam), 1.0f) &&
-         dot(xyz(ld4(a.normal, (uint32_t)py * W + (uint32_t)px)), xyz(ld4(a.old_normal, q))) > F.temporal[2];
+         dot(normal_px(a, (uint32_t)py * W + (uint32_t)px), xyz(ld4(a.old_normal, q))) > F.temporal[2];
 }
 template <class P>
 __device__ __forceinline__ Guide guide(const SvgfArgs<P>& a, int32_t x, int32_t y) {
   const uint32_t W = a.F.size[0];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   Guide g;
-  g.position = position(a, x, y);
-  g.normal = xyz(ld4(a.normal, id));
-  g.color = xyz(ld4(a.color, id));
-  float2 rm = ld2(a.params, id);
-  g.m0 = rm.x;
-  g.m1 = rm.y;
+  if (a.guide) {
+    const GuidePx r = a.guide[id];
+    g.position = v3{r.px, r.py, r.pz};
+    g.normal = v3{(float)r.nx, (float)r.ny, (float)r.nz};
+    g.color = v3{(float)r.cr, (float)r.cg, (float)r.cb};
+    g.m0 = (float)r.m0;
+    g.m1 = (float)r.m1;
+  } else {
+    g.position = position(a, x, y);
+    g.normal = xyz(ld4(a.normal, id));
+    g.color = xyz(ld4(a.color, id));
+    float2 rm = ld2(a.params, id);
+    g.m0 = rm.x;
+    g.m1 = rm.y;
+  }
   g.pixel_scale = length(g.position - ld3(a.F.camera)) / (float)a.F.size[1];
   return g;
 }
@@ -359,15 +405,26 @@ __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Gui
   if (!inside(a, qx, qy)) return 0.0f;
   const uint32_t W = a.F.size[0];
   const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
-  float d = a.depth[q];
-  if (d >= 1.0f) return 0.0f;
-  v3 nq = xyz(ld4(a.normal, q));
+  v3 nq, wq, cq;
+  float2 rm;
+  if (a.guide) {
+    const GuidePx r = a.guide[q];
+    if (!r.valid) return 0.0f;
+    nq = v3{(float)r.nx, (float)r.ny, (float)r.nz};
+    wq = v3{r.px, r.py, r.pz};
+    cq = v3{(float)r.cr, (float)r.cg, (float)r.cb};
+    rm = make_float2((float)r.m0, (float)r.m1);
+  } else {
+    float d = a.depth[q];
+    if (d >= 1.0f) return 0.0f;
+    nq = xyz(ld4(a.normal, q));
+    wq = world(pixel_u(qx, W), pixel_u(qy, a.F.size[1]), d, a.F.inv_vp);
+    cq = xyz(ld4(a.color, q));
+    rm = ld2(a.params, q);
+  }
   float wn = pow64(fmaxf(dot(g.normal, nq), 0.0f));
-  v3 wq = world(pixel_u(qx, W), pixel_u(qy, a.F.size[1]), d, a.F.inv_vp);
   float tolerance = fmaxf(0.0001f, g.pixel_scale * stride * 0.5f);
   float wz = __expf(-fabsf(dot(wq - g.position, g.normal)) / tolerance);
-  v3 cq = xyz(ld4(a.color, q));
-  float2 rm = ld2(a.params, q);
   float dm0 = g.m0 - rm.x, dm1 = g.m1 - rm.y;
   float wc = __expf(-length(g.color - cq) * 8.0f - sqrtf(dm0 * dm0 + dm1 * dm1) * 8.0f);
   return wn * wz * wc;
@@ -384,7 +441,7 @@ __device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, i
   float l = luminance(c);
   v3 history = v3{0.f, 0.f, 0.f};
   float hm0 = 0.f, hm1 = 0.f, length_history = 0.f, total = 0.f;
-  if (a.depth[id] < 1.0f) {
+  if (valid_px(a, id)) {
     float2 m = ld2(a.motion, id);
     float px = (pixel_u(x, W) - m.x) * (float)W - 0.5f;
     float py = (pixel_u(y, H) - m.y) * (float)H - 0.5f;
@@ -451,7 +508,7 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
   float4 c = ld4(a.input, id);
   float age = a.moments[id].z;
   float sigma = c.w;
-  if (age < 4.0f && a.depth[id] < 1.0f) {
+  if (age < 4.0f && valid_px(a, id)) {
     float s0 = 0.f, s1 = 0.f, total = 0.f;
     Guide g = guide(a, x, y);
     for (int32_t j = -3; j <= 3; ++j) {
@@ -481,7 +538,7 @@ __device__ __forceinline__ float4 svgf_atrous(const SvgfArgs<P>& a, int32_t x, i
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   float4 c = ld4(a.input, id);
-  if (a.depth[id] >= 1.0f) return c;
+  if (!valid_px(a, id)) return c;
   float variance = 0.f, guide_weight = 0.f;
   Guide g = guide(a, x, y);
   for (int32_t j = -1; j <= 1; ++j) {

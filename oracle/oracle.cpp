@@ -39,6 +39,22 @@ template <class R>
 Vec<R> normalize3(Vec<R> a) { return div3(a, length3(a)); }  // WGSL: e / length(e)
 template <class R>
 R maxr(R a, R b) { return a > b ? a : b; }  // operands never NaN where used
+// pow(x, 5) / pow(x, 64) for x >= 0: the FP32 instance evaluates the fixed
+// multiply chains (x^2)^2*x and six squarings; the FP64 instance uses pow().
+inline float pow5(float x) {
+  const float x2 = x * x;
+  return x2 * x2 * x;
+}
+inline double pow5(double x) { return std::pow(x, 5.0); }
+inline float pow64(float x) {
+  float y = x * x;
+  y = y * y;
+  y = y * y;
+  y = y * y;
+  y = y * y;
+  return y * y;
+}
+inline double pow64(double x) { return std::pow(x, 64.0); }
 template <class R>
 R minr(R a, R b) { return a < b ? a : b; }
 template <class R>
@@ -191,7 +207,7 @@ struct Oracle {
     const R c04 = R(0.04f);
     const Vec<R> f0 = V<R>(c04 * (R(1) - s.metal) + s.color.x * s.metal, c04 * (R(1) - s.metal) + s.color.y * s.metal,
                            c04 * (R(1) - s.metal) + s.color.z * s.metal);
-    const R p5 = std::pow(R(1) - vh, R(5));
+    const R p5 = pow5(R(1) - vh);
     const Vec<R> f = V<R>(f0.x + (R(1) - f0.x) * p5, f0.y + (R(1) - f0.y) * p5, f0.z + (R(1) - f0.z) * p5);
     const R den = maxr(R(4) * nv * nl, R(0.00001f));
     Vec<R> out;
@@ -646,7 +662,7 @@ struct Oracle {
     if (!inside(qx, qy) || (R)gb->depth[idx(qx, qy)] >= R(1)) return R(0);
     const size_t q = idx(qx, qy);
     const Vec<R> nq = D3<R>(gb->normal + 4 * q);
-    const R wn = std::pow(maxr(dot3(a.normal, nq), R(0)), R(64));
+    const R wn = pow64(maxr(dot3(a.normal, nq), R(0)));
     const Vec<R> wq = position(qx, qy);
     const R tolerance = maxr(R(0.0001f), a.pixel_scale * stride * R(0.5f));
     const R wz = std::exp(-std::fabs(dot3(sub3(wq, a.position), a.normal)) / tolerance);
