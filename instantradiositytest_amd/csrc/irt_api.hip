@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -103,6 +104,16 @@ struct irt_ctx {
   float4* moments[2] = {nullptr, nullptr};
   GuidePx* guide = nullptr;  // per-pixel guide records of the current G-buffer (fused schedule)
   bool guide_dirty = true;
+  // candidate grid over the scene (fused schedule)
+  CellList* cells = nullptr;
+  size_t cells_cap = 0;
+  float4* obj_boxes = nullptr;
+  uint32_t obj_boxes_cap = 0;
+  unsigned int* mesh_k2 = nullptr;
+  uint32_t mesh_k2_cap = 0;
+  GridView grid{};
+  bool grid_dirty = true;
+  bool mesh_k_dirty = true;
   unsigned long long* counters = nullptr;
   bool counting = false;
   bool timing = false;
@@ -274,7 +285,8 @@ LightArgs<P> light_args(irt_ctx* c, int in, int out, int hist) {
   LightArgs<P> a;
   std::memset(&a, 0, sizeof(a));
   a.F = c->frame;
-  a.sc = SceneView{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1]};
+  a.sc = SceneView{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1],
+                   GridView{}};
   a.depth = c->depth;
   a.normal = (const typename P::N4*)c->normal;
   a.color = (const typename P::N4*)c->color;
@@ -352,6 +364,48 @@ int ensure_guide(irt_ctx* c) {
   return IRT_OK;
 }
 
+// Rebuilds the candidate grid when objects, meshes or the atlas changed.
+// Leaves c->grid.cells null (hierarchy walk) for scenes above the grid's
+// object limit.
+int ensure_grid(irt_ctx* c) {
+  if (!c->grid_dirty) return IRT_OK;
+  c->grid.cells = nullptr;
+  if (c->n_objects == 0 || c->n_objects > GRID_MAX_OBJECTS || !c->atlas || !c->meshes) {
+    c->grid_dirty = false;
+    return IRT_OK;
+  }
+  SceneView sc{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1], GridView{}};
+  if (c->mesh_k_dirty) {
+    int rc = ensure_array(c, &c->mesh_k2, &c->mesh_k2_cap, c->n_meshes);
+    if (rc) return rc;
+    HIP_CHECK(c, hipMemsetAsync(c->mesh_k2, 0, sizeof(unsigned int) * c->n_meshes, c->stream));
+    hipLaunchKernelGGL(k_mesh_gradient, dim3(64, c->n_meshes), dim3(256), 0, c->stream, sc, c->n_meshes, c->mesh_k2);
+    HIP_CHECK(c, hipGetLastError());
+    c->mesh_k_dirty = false;
+  }
+  const size_t total = (size_t)c->grid.nx * c->grid.ny * c->grid.nz;
+  if (c->cells_cap < total) {
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (auto& a : c->allocs)
+      if (a == c->cells) a = nullptr;
+    if (c->cells) HIP_CHECK(c, hipFree(c->cells));
+    c->cells = nullptr;
+    int rc = dev_alloc(c, (void**)&c->cells, total * sizeof(CellList));
+    if (rc) return rc;
+    c->cells_cap = total;
+  }
+  GridView g = c->grid;
+  g.cells = c->cells;
+  g.boxes = c->obj_boxes;
+  sc.grid = g;
+  hipLaunchKernelGGL(k_build_cells, dim3((uint32_t)((total + 63) / 64)), dim3(64), 0, c->stream, sc, c->n_objects,
+                     c->s.sdf_hit_epsilon, (const unsigned int*)c->mesh_k2, c->cells);
+  HIP_CHECK(c, hipGetLastError());
+  c->grid = g;
+  c->grid_dirty = false;
+  return IRT_OK;
+}
+
 // One reservoir/resolve pass with the reference's bindings (pass_group, renderer.rs:1617-1674).
 template <class P>
 int lighting_pass(irt_ctx* c, uint32_t pass) {
@@ -376,14 +430,17 @@ int run_lighting(irt_ctx* c) {
   if (c->schedule == IRT_SCHEDULE_FUSED) {
     timer_begin(c, "reservoirs_a");
     rc = ensure_guide<P>(c);
+    if (!rc) rc = ensure_grid(c);
     if (rc) return rc;
     LightArgs<P> a = light_args<P>(c, 2, 1, 2);
     a.guide = c->guide;
+    a.sc.grid = c->grid;
     LAUNCH(c, k_reservoirs_a, P, a, c->direct[0], c->indirect[0]);
     timer_end(c);
     timer_begin(c, "reservoirs_b_resolve");
     LightArgs<P> b = light_args<P>(c, 1, 2, 0);
     b.guide = c->guide;
+    b.sc.grid = c->grid;
     LAUNCH(c, k_reservoirs_b, P, b);
     timer_end(c);
     return IRT_OK;
@@ -680,6 +737,8 @@ int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, u
   c->atlas_dims[0] = w;
   c->atlas_dims[1] = h;
   c->atlas_dims[2] = d;
+  c->mesh_k_dirty = true;
+  c->grid_dirty = true;
   return IRT_OK;
 }
 
@@ -700,6 +759,8 @@ int irt_set_meshes(irt_ctx* c, const irt_mesh* meshes, uint32_t count) {
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
   c->n_meshes = count;
   c->host_meshes.assign(meshes, meshes + count);
+  c->mesh_k_dirty = true;
+  c->grid_dirty = true;
   return IRT_OK;
 }
 
@@ -730,6 +791,37 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   c->n_objects = count;
   c->n_nodes = node_count;
   c->max_mesh_ref = max_ref;
+  // Object world boxes (the hierarchy's leaves) and the grid over the root box.
+  std::vector<float4> boxes(2 * (size_t)count, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (uint32_t i = 0; i < node_count; ++i)
+    if (nodes[i].info[2] == 1u) {
+      const uint32_t j = nodes[i].info[0];
+      boxes[2 * j] = make_float4(nodes[i].lo[0], nodes[i].lo[1], nodes[i].lo[2], 0.f);
+      boxes[2 * j + 1] = make_float4(nodes[i].hi[0], nodes[i].hi[1], nodes[i].hi[2], 0.f);
+    }
+  rc = ensure_array(c, &c->obj_boxes, &c->obj_boxes_cap, 2 * count);
+  if (rc) return rc;
+  HIP_CHECK(c, hipMemcpyAsync(c->obj_boxes, boxes.data(), sizeof(float4) * boxes.size(), hipMemcpyHostToDevice,
+                              c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  {
+    const float margin = 1e-3f;
+    float lo[3], ext = 0.f, e[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = nodes[0].lo[a] - margin;
+      e[a] = nodes[0].hi[a] + margin - lo[a];
+      ext = std::max(ext, e[a]);
+    }
+    const float cell = std::max(ext / 48.0f, 1e-4f);
+    c->grid.x0 = lo[0];
+    c->grid.y0 = lo[1];
+    c->grid.z0 = lo[2];
+    c->grid.inv_cell = 1.0f / cell;
+    c->grid.nx = std::max(1u, (uint32_t)std::ceil(e[0] / cell));
+    c->grid.ny = std::max(1u, (uint32_t)std::ceil(e[1] / cell));
+    c->grid.nz = std::max(1u, (uint32_t)std::ceil(e[2] / cell));
+  }
+  c->grid_dirty = true;
   return IRT_OK;
 }
 

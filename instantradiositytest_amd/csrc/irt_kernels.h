@@ -246,6 +246,104 @@ __global__ __launch_bounds__(256) void k_guide(irt_frame F, const float* depth, 
   out[i] = g;
 }
 
+// --------------------------------------------------------- candidate grid
+// Largest corner-gradient norm of a mesh's trilinear field over all of its
+// cells (the trilinear gradient is a convex combination of the eight corner
+// gradients).  One y-z slab of cells per thread; result in k2[m] as the bits
+// of the squared norm (non-negative floats order like unsigned integers).
+__global__ __launch_bounds__(256) void k_mesh_gradient(SceneView sc, uint32_t n_meshes, unsigned int* k2) {
+  const uint32_t m = blockIdx.y;
+  if (m >= n_meshes) return;
+  const irt_mesh mesh = sc.meshes[m];
+  const uint32_t res = mesh.field[3];
+  const uint32_t ncell = res - 1u;
+  float best = 1.0f;
+  if (mesh.geometry_lo[3] <= 0.5f) {
+    const size_t sy = sc.atlas_w, sz = (size_t)sc.atlas_w * sc.atlas_h;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell * ncell * ncell; c += gridDim.x * blockDim.x) {
+      const uint32_t x = c % ncell, y = (c / ncell) % ncell, z = c / (ncell * ncell);
+      const float* b = sc.atlas + ((size_t)(mesh.field[2] + z) * sc.atlas_h + (mesh.field[1] + y)) * sc.atlas_w +
+                       (mesh.field[0] + x);
+      float v[8];
+      for (int i = 0; i < 8; ++i) v[i] = b[(i & 1) + ((i >> 1) & 1) * sy + (i >> 2) * sz];
+      for (int k = 0; k < 8; ++k) {
+        float dx = (v[(k & 6) | 1] - v[k & 6]) * mesh.grid[0];
+        float dy = (v[(k & 5) | 2] - v[k & 5]) * mesh.grid[1];
+        float dz = (v[(k & 3) | 4] - v[k & 3]) * mesh.grid[2];
+        best = fmaxf(best, dx * dx + dy * dy + dz * dz);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off, 64));
+  if (threadIdx.x % 64 == 0) atomicMax(k2 + m, __float_as_uint(best));
+}
+
+// Candidate lists, one thread per cell.  Lower bound of object j over the
+// cell: 0.9998 x the gap between the cell and j's world box (the hierarchy's
+// own pruning bound), or -inf when they overlap.  Upper bound: the object
+// distance at the cell centre plus K_j x the half diagonal, K_j a Lipschitz
+// constant of object_distance (1 for certified boxes, sqrt(K_mesh^2 + 1)
+// for grid meshes: the padded-box extension combines the field gradient with
+// a unit outside term), with a relative FP margin.
+constexpr uint32_t GRID_MAX_OBJECTS = 64;
+__global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_objects, float hit_eps,
+                                                    const unsigned int* k2_mesh, CellList* cells) {
+  const GridView& g = sc.grid;
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t total = g.nx * g.ny * g.nz;
+  if (cell >= total) return;
+  const uint32_t cx = cell % g.nx, cy = (cell / g.nx) % g.ny, cz = cell / (g.nx * g.ny);
+  const float cs = 1.0f / g.inv_cell;
+  const v3 lo = v3{g.x0 + cx * cs, g.y0 + cy * cs, g.z0 + cz * cs};
+  const v3 hi = v3{lo.x + cs, lo.y + cs, lo.z + cs};
+  const v3 center = v3{(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+  const float half_diag = 0.5f * cs * 1.7320508f * 1.0001f;
+  float lower[GRID_MAX_OBJECTS];
+  float upper_min = INFINITY;
+  Count<false> cnt;
+  for (uint32_t j = 0; j < n_objects; ++j) {
+    const float4 blo = g.boxes[2 * j], bhi = g.boxes[2 * j + 1];
+    const float gx = fmaxf(fmaxf(blo.x - hi.x, lo.x - bhi.x), 0.0f);
+    const float gy = fmaxf(fmaxf(blo.y - hi.y, lo.y - bhi.y), 0.0f);
+    const float gz = fmaxf(fmaxf(blo.z - hi.z, lo.z - bhi.z), 0.0f);
+    const float gap = sqrtf(gx * gx + gy * gy + gz * gz);
+    lower[j] = gap > 0.0f ? 0.9998f * gap * (1.0f - 1e-6f) : -INFINITY;
+    const irt_object& obj = sc.objects[j];
+    const irt_mesh& mesh = sc.meshes[obj.info[0]];
+    float k = 1.0f;
+    if (mesh.geometry_lo[3] <= 0.5f) {
+      const float k2 = __uint_as_float(k2_mesh[obj.info[0]]);
+      k = sqrtf(k2 + 1.0f);
+    }
+    float vx;
+    const float dc = object_distance(sc, center, j, false, hit_eps, &vx, cnt);
+    float up = dc + k * 1.001f * half_diag;
+    up += fabsf(up) * 1e-5f + 1e-6f;
+    upper_min = fminf(upper_min, up);
+  }
+  // Candidates: lower bound <= smallest upper bound, sorted by lower bound.
+  CellList out;
+  out.count = 0;
+  uint32_t n = 0;
+  uint8_t ids[GRID_MAX_OBJECTS];
+  for (uint32_t j = 0; j < n_objects; ++j) {
+    if (!(lower[j] <= upper_min)) continue;
+    uint32_t k = n++;
+    while (k > 0 && lower[ids[k - 1]] > lower[j]) {
+      ids[k] = ids[k - 1];
+      --k;
+    }
+    ids[k] = (uint8_t)j;
+  }
+  if (n > 15) {
+    out.count = CELL_OVERFLOW;
+  } else {
+    out.count = (uint8_t)n;
+    for (uint32_t k = 0; k < 15; ++k) out.ids[k] = k < n ? ids[k] : 0;
+  }
+  cells[cell] = out;
+}
+
 // ------------------------------------------------------------- format I/O
 // Reference formats (Depth32Float + binary16 channels) -> internal G-buffer.
 template <class P>

@@ -163,24 +163,29 @@ def run_frames(r: Renderer, cs: ConfigScene, frames: int):
     r.sync()
 
 
-def test_fused_schedule_matches_pass_schedule(require_gpu):
-    log("[parity] fused vs passes 160x100 x6 frames")
-    cfg, cs = small_config(160, 100, 6)
+@pytest.mark.parametrize("name,w,h", [("plain-640x400", 160, 100), ("plain-1920x1200", 320, 200)])
+def test_fused_schedule_matches_pass_schedule(require_gpu, name, w, h):
+    """The fused kernels (guide records, candidate-grid scene queries) against
+    the per-pass kernels (G-buffer planes, hierarchy walk): same arithmetic,
+    so the outputs agree except where an exact distance tie picks another
+    closest object."""
+    log(f"[parity] fused vs passes {name} {w}x{h} x5 frames")
+    cfg, cs = small_config(w, h, 5, name)
     a = Renderer(cs.scene, cfg.settings(), schedule="passes")
     b = Renderer(cs.scene, cfg.settings(), schedule="fused")
-    run_frames(a, cs, 6)
-    run_frames(b, cs, 6)
-    for name in ("direct2", "indirect2"):
-        s = compare_reservoirs(b.read(name), to_oracle_reservoirs(a.read(name)))
-        log(name, s)
-        assert s["match"] >= RES_MATCH
-    for name in ("lighting", "history", "feedback"):
-        s = compare_images(rgba16_to_f64(b.read(name)), rgba16_to_f64(a.read(name)), REL_TOL, FLOOR)
-        log(name, s)
-        assert s["within"] >= 0.99 and s["rel_l2"] <= 1e-2
+    run_frames(a, cs, 5)
+    run_frames(b, cs, 5)
+    for buf in ("direct2", "indirect2"):
+        s = compare_reservoirs(b.read(buf), to_oracle_reservoirs(a.read(buf)))
+        log(buf, s)
+        assert s["match"] >= RES_MATCH and s["max_rel"] <= REL_TOL
+    for buf in ("lighting", "history", "feedback"):
+        s = compare_images(rgba16_to_f64(b.read(buf)), rgba16_to_f64(a.read(buf)), REL_TOL, FLOOR)
+        log(buf, s)
+        assert s["within"] >= IMG_WITHIN and s["rel_l2"] <= REL_TOL
     s = compare_images(b.read("moments").astype(np.float64), a.read("moments").astype(np.float64), REL_TOL, FLOOR)
     log("moments", s)
-    assert s["within"] >= 0.99
+    assert s["within"] >= IMG_WITHIN
 
 
 def test_closed_loop_frames_track_oracle(require_gpu):
