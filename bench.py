@@ -191,7 +191,14 @@ def main() -> int:
         "roofline": roofline,
     }
     if counts_by_pass:
+        executed = counts_by_pass.pop("_executed", None)
         line["events_per_frame"] = {k: v for k, v in counts_by_pass.items()}
+        if executed:
+            line["executed_events_per_frame"] = executed
+            if roofline is not None and dominant in executed:
+                ex = flops_of(executed[dominant]) / (pass_ms[dominant] * 1e-3) / 1e12
+                roofline["executed_tflops"] = round(ex, 3)
+                roofline["executed_frac"] = round(ex / FP32_VALU_PEAK_TFLOPS, 5)
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         line["cpu_baseline"] = cpu_baseline(cs, settings, args.cpu_frames, threads)
@@ -238,6 +245,7 @@ def instrumented_counts(r, cs, settings, staged, host_frames, last: int) -> dict
         r.enable_counters(True)
         check(L.irt_pass(ctx, i, 0), ctx)
         out[PASS_TIMER_NAME.get(name, name)] = r.counters()
+    # leave the passes' results in place for the executed-work count below
     r.enable_counters(True)
     check(L.irt_svgf(ctx, settings.svgf_iterations, 0), ctx)
     out["svgf"] = r.counters()
@@ -248,6 +256,24 @@ def instrumented_counts(r, cs, settings, staged, host_frames, last: int) -> dict
             for k, v in out[PASS_TIMER_NAME.get(m, m)].items():
                 agg[k] = agg.get(k, 0) + v
         out[fused] = agg
+    if r.schedule_name == "fused":
+        # Work the fused kernels execute (candidate-grid queries count their
+        # candidate tests as node tests), on the same frame's inputs.
+        executed = {}
+        r.enable_counters(True)
+        r.schedule("fused")
+        check(L.irt_lighting(ctx), ctx)
+        executed["reservoirs"] = r.counters()
+        r.enable_counters(True)
+        check(L.irt_svgf(ctx, settings.svgf_iterations, 0), ctx)
+        executed["svgf"] = r.counters()
+        r.enable_counters(False)
+        # split the lighting counts between the two fused kernels: march events belong to reservoirs_b
+        lit = executed.pop("reservoirs")
+        a_brdf = out["reservoirs_a"]["brdf"]
+        executed["reservoirs_a"] = {k: (a_brdf if k == "brdf" else 0) for k in lit}
+        executed["reservoirs_b_resolve"] = {k: (v - a_brdf if k == "brdf" else v) for k, v in lit.items()}
+        out["_executed"] = executed
     return out
 
 

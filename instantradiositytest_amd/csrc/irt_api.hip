@@ -114,6 +114,12 @@ struct irt_ctx {
   GridView grid{};
   bool grid_dirty = true;
   bool mesh_k_dirty = true;
+  // decoupled resolve (fused schedule)
+  ResolveRec* resolve_rec = nullptr;
+  RayReq* rays = nullptr;
+  float* vis = nullptr;
+  uint32_t* queue = nullptr;
+  uint32_t march_blocks = 0;
   unsigned long long* counters = nullptr;
   bool counting = false;
   bool timing = false;
@@ -437,11 +443,33 @@ int run_lighting(irt_ctx* c) {
     a.sc.grid = c->grid;
     LAUNCH(c, k_reservoirs_a, P, a, c->direct[0], c->indirect[0]);
     timer_end(c);
-    timer_begin(c, "reservoirs_b_resolve");
     LightArgs<P> b = light_args<P>(c, 1, 2, 0);
     b.guide = c->guide;
     b.sc.grid = c->grid;
-    LAUNCH(c, k_reservoirs_b, P, b);
+    if (!c->grid.cells || c->n_objects > LDS_MAX_OBJECTS || c->n_meshes > LDS_MAX_OBJECTS) {
+      // scene beyond the LDS staging limits: one kernel per pixel
+      timer_begin(c, "reservoirs_b_resolve");
+      LAUNCH(c, k_reservoirs_b, P, b);
+      timer_end(c);
+      return IRT_OK;
+    }
+    timer_begin(c, "spatial_setup");
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->queue, 16u);
+    LAUNCH(c, k_spatial_setup, P, b, c->resolve_rec, c->rays);
+    timer_end(c);
+    timer_begin(c, "march");
+    if (c->counting)
+      hipLaunchKernelGGL(k_march<true>, dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc, c->frame, c->n_meshes,
+                         c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
+    else
+      hipLaunchKernelGGL(k_march<false>, dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc, c->frame,
+                         c->n_meshes, c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
+    HIP_CHECK(c, hipGetLastError());
+    timer_end(c);
+    timer_begin(c, "resolve_combine");
+    hipLaunchKernelGGL((k_resolve_combine<P>), grid_of(c), dim3(TILE, TILE, 1), 0, c->stream, b, c->resolve_rec,
+                       c->vis);
+    HIP_CHECK(c, hipGetLastError());
     timer_end(c);
     return IRT_OK;
   }
@@ -680,6 +708,17 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   A((void**)&c->moments[1], n * 16);
   A((void**)&c->counters, 8 * sizeof(unsigned long long));
   A((void**)&c->guide, n * sizeof(GuidePx));
+  A((void**)&c->resolve_rec, n * sizeof(ResolveRec));
+  A((void**)&c->rays, 2 * n * sizeof(RayReq));
+  A((void**)&c->vis, 2 * n * sizeof(float));
+  A((void**)&c->queue, 64);
+  if (!rc) {
+    hipDeviceProp_t prop;
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+    c->march_blocks = (uint32_t)cus * 8u;
+  }
   // Depth targets start cleared to 1 (far plane) as the reference's depth pass clears them.
   if (!rc) {
     std::vector<float> ones(n, 1.0f);

@@ -406,12 +406,17 @@ __device__ __forceinline__ bool scene_distance_grid(const SceneView& sc, float h
   if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)g.nx && fy < (float)g.ny && fz < (float)g.nz))
     return false;
   const uint32_t cx = min((uint32_t)fx, g.nx - 1u), cy = min((uint32_t)fy, g.ny - 1u), cz = min((uint32_t)fz, g.nz - 1u);
-  const CellList L = g.cells[((size_t)cz * g.ny + cy) * g.nx + cx];
-  if (L.count == CELL_OVERFLOW) return false;
+  // The 16-byte list as four dwords; byte 0 is the count, byte k+1 the k-th id
+  // (register selects rather than a dynamically indexed array).
+  const uint4 L = reinterpret_cast<const uint4*>(g.cells)[((size_t)cz * g.ny + cy) * g.nx + cx];
+  const uint32_t count = L.x & 0xffu;
+  if (count == CELL_OVERFLOW) return false;
   float best = 1e20f, voxel = hit_eps;
   uint32_t closest = hint;
-  for (uint32_t k = 0; k < L.count; ++k) {
-    const uint32_t j = L.ids[k];
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t b = k + 1u, word = b >> 2;
+    const uint32_t w = word == 0u ? L.x : (word == 1u ? L.y : (word == 2u ? L.z : L.w));
+    const uint32_t j = (w >> ((b & 3u) * 8u)) & 0xffu;
     cnt.add(C_NODE, 1);
     const float4 lo = g.boxes[2 * j], hi = g.boxes[2 * j + 1];
     v3 delta = vmaxs(vmax(v3{lo.x - p.x, lo.y - p.y, lo.z - p.z}, v3{p.x - hi.x, p.y - hi.y, p.z - hi.z}), 0.0f);
@@ -504,6 +509,83 @@ __device__ __forceinline__ float visibility(const SceneView& sc, const irt_frame
     t += h;
   }
   return 0.0f;
+}
+
+// ------------------------------------------------------------ ray march
+// visibility() split into a set-up and a single step so that marches can be
+// scheduled independently of pixels.  Same operations in the same order.
+struct MarchRay {
+  float ox, oy, oz, limit;
+  float dx, dy, dz;
+  uint32_t closest;
+};
+static_assert(sizeof(MarchRay) == 32, "ray record is 32 bytes");
+
+// The clearance queries and ray set-up of sdf.wgsl:62-74.  Returns false when
+// the visibility is already decided (limit <= 0 -> 1).
+template <class C>
+__device__ __forceinline__ bool march_setup(const SceneView& sc, const irt_frame& F, v3 p, v3 n, v3 target,
+                                            v3 end_normal, MarchRay* ray, C& cnt) {
+  const uint32_t n_objects = F.counts[3];
+  const float eps0 = F.sdf[1];
+  cnt.add(C_MARCH, 1);
+  Distance clearance = scene_distance(sc, n_objects, eps0, p, true, 0xffffffffu, cnt);
+  float epsilon = fmaxf(eps0, clearance.voxel * 0.65f);
+  v3 origin = p + n * epsilon;
+  v3 endpoint = target;
+  if (dot(end_normal, end_normal) > 0.5f) {
+    Distance e = scene_distance(sc, n_objects, eps0, target, true, 0xffffffffu, cnt);
+    endpoint = endpoint + end_normal * fmaxf(eps0, e.voxel * 0.65f);
+  }
+  v3 delta = endpoint - origin;
+  float limit = length(delta) - eps0;
+  if (limit <= 0.0f) return false;
+  v3 direction = normalize(delta);
+  *ray = MarchRay{origin.x, origin.y, origin.z, limit, direction.x, direction.y, direction.z, clearance.object_id};
+  return true;
+}
+
+struct MarchState {
+  float t, result, previous;
+  uint32_t closest, i;
+};
+__device__ __forceinline__ MarchState march_begin(const irt_frame& F, const MarchRay& r) {
+  return MarchState{F.sdf[1], 1.0f, 1e20f, r.closest, 0u};
+}
+// One iteration of the loop of sdf.wgsl:76-86.  Returns true when the march
+// has finished, with the visibility in *vis.
+template <class C>
+__device__ __forceinline__ bool march_step(const SceneView& sc, const irt_frame& F, const MarchRay& r, MarchState& s,
+                                           float* vis, C& cnt) {
+  const float eps0 = F.sdf[1];
+  if (s.i >= f2u_sat(F.sdf[0])) {
+    *vis = 0.0f;
+    return true;
+  }
+  if (s.t >= r.limit) {
+    *vis = clampf(s.result, 0.0f, 1.0f);
+    return true;
+  }
+  cnt.add(C_STEP, 1);
+  const v3 q = v3{r.ox, r.oy, r.oz} + v3{r.dx, r.dy, r.dz} * s.t;
+  Distance qd = scene_distance(sc, F.counts[3], eps0, q, false, s.closest, cnt);
+  const float h = qd.distance;
+  s.closest = qd.object_id;
+  if (h < eps0) {
+    *vis = 0.0f;
+    return true;
+  }
+  const float y = h * h / (2.0f * s.previous);
+  const float d = sqrtf(fmaxf(h * h - y * y, 0.0f));
+  if (s.t > y) s.result = fminf(s.result, F.sdf[2] * d / (s.t - y));
+  if (s.result < 0.001f) {
+    *vis = 0.0f;
+    return true;
+  }
+  s.previous = h;
+  s.t += h;
+  s.i += 1u;
+  return false;
 }
 
 }  // namespace irt

@@ -74,6 +74,8 @@ class Renderer:
         self.L = L
         self.settings = settings
         self.precision = precision
+        self.schedule_name = schedule
+        self.timing = False
         ctx = ctypes.c_void_p()
         check(L.irt_create(ctypes.byref(settings), PRECISIONS[precision], ctypes.byref(ctx)))
         self.ctx = ctx
@@ -101,6 +103,7 @@ class Renderer:
 
     def schedule(self, name: str):
         check(self.L.irt_set_schedule(self.ctx, SCHEDULES[name]), self.ctx)
+        self.schedule_name = name
 
     # -- frame ------------------------------------------------------------
     def begin_frame(self, scene: Scene, inputs: FrameInputs) -> np.ndarray:
@@ -175,6 +178,7 @@ class Renderer:
     # -- timing / counters ---------------------------------------------------
     def enable_timing(self, on: bool = True):
         check(self.L.irt_enable_timing(self.ctx, int(on)), self.ctx)
+        self.timing = on
 
     def pass_times(self) -> dict:
         names = (ctypes.c_char * 32 * 32)()
