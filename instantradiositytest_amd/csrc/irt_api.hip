@@ -77,7 +77,8 @@ struct irt_ctx {
   uint32_t n_meshes = 0, cap_meshes = 0;
   irt_light* lights = nullptr;
   uint32_t n_lights = 0, cap_lights = 0;
-  float* atlas = nullptr;
+  float* atlas = nullptr;  // FP32 profile
+  _Float16* atlas16 = nullptr;  // mixed profile
   uint32_t atlas_dims[3] = {0, 0, 0};
   irt_frame frame{};
   bool have_frame = false;
@@ -292,7 +293,7 @@ LightArgs<P> light_args(irt_ctx* c, int in, int out, int hist) {
   std::memset(&a, 0, sizeof(a));
   a.F = c->frame;
   a.sc = SceneView{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1],
-                   GridView{}};
+                   GridView{}, c->atlas16};
   a.depth = c->depth;
   a.normal = (const typename P::N4*)c->normal;
   a.color = (const typename P::N4*)c->color;
@@ -331,6 +332,17 @@ SvgfArgs<P> svgf_args(irt_ctx* c) {
   return a;
 }
 
+#define LAUNCH_SVGF(c, kern, P, fast, ...)                                                               \
+  do {                                                                                                   \
+    if ((c)->counting)                                                                                   \
+      hipLaunchKernelGGL((kern<P, true, false>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+    else if (fast)                                                                                       \
+      hipLaunchKernelGGL((kern<P, false, true>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+    else                                                                                                 \
+      hipLaunchKernelGGL((kern<P, false, false>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+    HIP_CHECK(c, hipGetLastError());                                                                     \
+  } while (0)
+
 #define LAUNCH(c, kern, P, ...)                                                                          \
   do {                                                                                                   \
     if ((c)->counting)                                                                                   \
@@ -344,7 +356,7 @@ int check_ready(irt_ctx* c) {
   if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
   if (!c->objects || c->n_objects == 0) return c->fail(IRT_ERR_STATE, "no objects set (irt_set_objects)");
   if (!c->meshes || c->n_meshes == 0) return c->fail(IRT_ERR_STATE, "no meshes set (irt_set_meshes)");
-  if (!c->atlas) return c->fail(IRT_ERR_STATE, "no SDF atlas set (irt_set_sdf_atlas)");
+  if (!c->atlas && !c->atlas16) return c->fail(IRT_ERR_STATE, "no SDF atlas set (irt_set_sdf_atlas)");
   if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "an object references a missing mesh");
   for (const irt_mesh& m : c->host_meshes) {
     const uint32_t res = m.field[3];
@@ -376,11 +388,12 @@ int ensure_guide(irt_ctx* c) {
 int ensure_grid(irt_ctx* c) {
   if (!c->grid_dirty) return IRT_OK;
   c->grid.cells = nullptr;
-  if (c->n_objects == 0 || c->n_objects > GRID_MAX_OBJECTS || !c->atlas || !c->meshes) {
+  if (c->n_objects == 0 || c->n_objects > GRID_MAX_OBJECTS || (!c->atlas && !c->atlas16) || !c->meshes) {
     c->grid_dirty = false;
     return IRT_OK;
   }
-  SceneView sc{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1], GridView{}};
+  SceneView sc{c->objects, c->meshes, c->nodes, c->lights, c->atlas, c->atlas_dims[0], c->atlas_dims[1], GridView{},
+               c->atlas16};
   if (c->mesh_k_dirty) {
     int rc = ensure_array(c, &c->mesh_k2, &c->mesh_k2_cap, c->n_meshes);
     if (rc) return rc;
@@ -496,7 +509,7 @@ int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   a.moments = c->moments[old];
   a.out = (typename P::R4*)c->temporal;
   a.moments_out = c->moments[cur];
-  LAUNCH(c, k_svgf_temporal, P, a);
+  LAUNCH_SVGF(c, k_svgf_temporal, P, guide != nullptr, a);
   return IRT_OK;
 }
 template <class P>
@@ -507,7 +520,7 @@ int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   a.input = (const typename P::R4*)c->temporal;
   a.moments = c->moments[cur];
   a.out = (typename P::R4*)c->scratch[0];
-  LAUNCH(c, k_svgf_variance, P, a);
+  LAUNCH_SVGF(c, k_svgf_variance, P, guide != nullptr, a);
   return IRT_OK;
 }
 template <class P>
@@ -520,7 +533,7 @@ int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, co
   a.out = (typename P::R4*)out;
   a.out2 = (typename P::R4*)out2;
   a.stride = 1u << i;
-  LAUNCH(c, k_svgf_atrous, P, a);
+  LAUNCH_SVGF(c, k_svgf_atrous, P, guide != nullptr, a);
   return IRT_OK;
 }
 
@@ -761,18 +774,31 @@ int irt_set_schedule(irt_ctx* c, uint32_t schedule) {
 int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, uint32_t d) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!texels || w < 2 || h < 2 || d < 2) return c->fail(IRT_ERR_INVALID_ARGUMENT, "invalid SDF atlas");
-  const size_t bytes = (size_t)w * h * d * 4;
-  if (c->atlas) {
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    for (auto& a : c->allocs)
-      if (a == c->atlas) a = nullptr;
-    HIP_CHECK(c, hipFree(c->atlas));
-    c->atlas = nullptr;
-  }
-  int rc = dev_alloc(c, (void**)&c->atlas, bytes);
-  if (rc) return rc;
-  HIP_CHECK(c, hipMemcpyAsync(c->atlas, texels, bytes, hipMemcpyHostToDevice, c->stream));
+  const size_t count = (size_t)w * h * d;
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  for (void** slot : {(void**)&c->atlas, (void**)&c->atlas16}) {
+    if (*slot) {
+      for (auto& a : c->allocs)
+        if (a == *slot) a = nullptr;
+      HIP_CHECK(c, hipFree(*slot));
+      *slot = nullptr;
+    }
+  }
+  if (mixed(c)) {
+    // 16-bit samples rounded toward -inf: every stored sample is <= the FP32
+    // sample, so the conservative bounds built on it stay lower bounds.
+    std::vector<uint16_t> h16(count);
+    for (size_t i = 0; i < count; ++i) h16[i] = irt_host::f32_to_f16_down(texels[i]);
+    int rc = dev_alloc(c, (void**)&c->atlas16, count * 2);
+    if (rc) return rc;
+    HIP_CHECK(c, hipMemcpyAsync(c->atlas16, h16.data(), count * 2, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  } else {
+    int rc = dev_alloc(c, (void**)&c->atlas, count * 4);
+    if (rc) return rc;
+    HIP_CHECK(c, hipMemcpyAsync(c->atlas, texels, count * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
   c->atlas_dims[0] = w;
   c->atlas_dims[1] = h;
   c->atlas_dims[2] = d;

@@ -76,10 +76,11 @@ __device__ __forceinline__ float4 round_f16(float4 v) {
 
 // -------------------------------------------------------- storage formats
 // 16-bit channel storage (binary16 bit patterns) for the mixed profile.
-struct h4 {
+// Aligned to their size so that loads and stores are single 64-/32-bit accesses.
+struct __align__(8) h4 {
   _Float16 x, y, z, w;
 };
-struct h2 {
+struct __align__(4) h2 {
   _Float16 x, y;
 };
 __device__ __forceinline__ float4 ld4(const float4* p, uint32_t i) { return p[i]; }
@@ -110,6 +111,28 @@ struct __align__(16) GuidePx {
   _Float16 nx, ny, nz, cr, cg, cb, m0, m1;
 };
 static_assert(sizeof(GuidePx) == 32, "guide record is 32 bytes");
+// Loads one guide record as two 16-byte vector loads and unpacks the halves
+// in registers (field-wise access would issue one 16-bit load per channel).
+__device__ __forceinline__ _Float16 lo16(uint32_t w) { return __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
+__device__ __forceinline__ _Float16 hi16(uint32_t w) { return __builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+__device__ __forceinline__ GuidePx ld_guide(const GuidePx* g, uint32_t i) {
+  const uint4* p = reinterpret_cast<const uint4*>(g + i);
+  const uint4 a = p[0], b = p[1];
+  GuidePx r;
+  r.px = __uint_as_float(a.x);
+  r.py = __uint_as_float(a.y);
+  r.pz = __uint_as_float(a.z);
+  r.valid = a.w;
+  r.nx = lo16(b.x);
+  r.ny = hi16(b.x);
+  r.nz = lo16(b.y);
+  r.cr = hi16(b.y);
+  r.cg = lo16(b.z);
+  r.cb = hi16(b.z);
+  r.m0 = lo16(b.w);
+  r.m1 = hi16(b.w);
+  return r;
+}
 
 // Storage profiles.
 struct ProfileF32 {
@@ -315,6 +338,7 @@ struct SceneView {
   const float* atlas;  // reference atlas layout: x + w*(y + h*z)
   uint32_t atlas_w, atlas_h;
   GridView grid;
+  const _Float16* atlas16;  // mixed profile: the same layout, samples rounded toward -inf (else null)
 };
 
 // signed_box_distance — sdf.wgsl:1-4.
@@ -330,10 +354,18 @@ __device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_
   uint32_t cz = min(f2u_sat(floorf(q.z)), res - 2u);
   float fx = q.x - (float)cx, fy = q.y - (float)cy, fz = q.z - (float)cz;
   const uint32_t w = sc.atlas_w, h = sc.atlas_h;
-  const float* base = sc.atlas + ((size_t)(origin[2] + cz) * h + (origin[1] + cy)) * w + (origin[0] + cx);
+  const size_t off = ((size_t)(origin[2] + cz) * h + (origin[1] + cy)) * w + (origin[0] + cx);
   const size_t sy = w, sz = (size_t)w * h;
-  float t000 = base[0], t100 = base[1], t010 = base[sy], t110 = base[sy + 1];
-  float t001 = base[sz], t101 = base[sz + 1], t011 = base[sz + sy], t111 = base[sz + sy + 1];
+  float t000, t100, t010, t110, t001, t101, t011, t111;
+  if (sc.atlas16) {
+    const _Float16* b = sc.atlas16 + off;
+    t000 = (float)b[0], t100 = (float)b[1], t010 = (float)b[sy], t110 = (float)b[sy + 1];
+    t001 = (float)b[sz], t101 = (float)b[sz + 1], t011 = (float)b[sz + sy], t111 = (float)b[sz + sy + 1];
+  } else {
+    const float* b = sc.atlas + off;
+    t000 = b[0], t100 = b[1], t010 = b[sy], t110 = b[sy + 1];
+    t001 = b[sz], t101 = b[sz + 1], t011 = b[sz + sy], t111 = b[sz + sy + 1];
+  }
   float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
   // value += texel * w.x * w.y * w.z, corners in z, y, x order (common.wgsl:33-36).
   float value = 0.0f;

@@ -310,6 +310,17 @@ uint16_t f32_to_f16(float f) {
   if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
   return (uint16_t)(sign | q);
 }
+// binary16 rounded toward -infinity: the largest binary16 value <= f.
+uint16_t f32_to_f16_down(float f) {
+  if (!(f == f)) return 0x7e00u;
+  uint16_t h = f32_to_f16(f);
+  if (f16_to_f32(h) > f) {
+    if (h == 0x0000u) return 0x8001u;       // +0 -> -min subnormal
+    if (h & 0x8000u) return (uint16_t)(h + 1u);  // negative: larger magnitude
+    return (uint16_t)(h - 1u);              // positive: smaller magnitude
+  }
+  return h;
+}
 float f16_to_f32(uint16_t h) {
   uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
   uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;

@@ -24,5 +24,6 @@ int pack_frame(const irt_settings& s, const float* vp, const float* pvp, const f
                uint32_t lights, uint32_t objects, irt_frame* f);
 uint16_t f32_to_f16(float f);
 float f16_to_f32(uint16_t h);
+uint16_t f32_to_f16_down(float f);
 
 }  // namespace irt_host

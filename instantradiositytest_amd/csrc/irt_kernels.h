@@ -481,31 +481,31 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 }
 
 // ------------------------------------------------------------------- SVGF
-template <class P, bool CNT>
+template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_temporal(SvgfArgs<P> a) {
   Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
   if (p.active) {
     float4 rad, mom;
-    svgf_temporal(a, p.x, p.y, &rad, &mom, cnt);
+    svgf_temporal<P, Count<CNT>, FAST>(a, p.x, p.y, &rad, &mom, cnt);
     st_r16(a.out, p.id, rad);
     a.moments_out[p.id] = mom;
   }
   cnt.flush(a.counters);
 }
-template <class P, bool CNT>
+template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_variance(SvgfArgs<P> a) {
   Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
-  if (p.active) st_r16(a.out, p.id, svgf_variance(a, p.x, p.y, cnt));
+  if (p.active) st_r16(a.out, p.id, svgf_variance<P, Count<CNT>, FAST>(a, p.x, p.y, cnt));
   cnt.flush(a.counters);
 }
-template <class P, bool CNT>
+template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
   Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
   if (p.active) {
-    float4 r = svgf_atrous(a, p.x, p.y, cnt);
+    float4 r = svgf_atrous<P, Count<CNT>, FAST>(a, p.x, p.y, cnt);
     st_r16(a.out, p.id, r);
     if (a.out2) st_r16(a.out2, p.id, r);
   }
@@ -558,10 +558,13 @@ __global__ __launch_bounds__(256) void k_mesh_gradient(SceneView sc, uint32_t n_
     const size_t sy = sc.atlas_w, sz = (size_t)sc.atlas_w * sc.atlas_h;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell * ncell * ncell; c += gridDim.x * blockDim.x) {
       const uint32_t x = c % ncell, y = (c / ncell) % ncell, z = c / (ncell * ncell);
-      const float* b = sc.atlas + ((size_t)(mesh.field[2] + z) * sc.atlas_h + (mesh.field[1] + y)) * sc.atlas_w +
-                       (mesh.field[0] + x);
+      const size_t off = ((size_t)(mesh.field[2] + z) * sc.atlas_h + (mesh.field[1] + y)) * sc.atlas_w +
+                         (mesh.field[0] + x);
       float v[8];
-      for (int i = 0; i < 8; ++i) v[i] = b[(i & 1) + ((i >> 1) & 1) * sy + (i >> 2) * sz];
+      for (int i = 0; i < 8; ++i) {
+        const size_t o = off + (i & 1) + ((i >> 1) & 1) * sy + (i >> 2) * sz;
+        v[i] = sc.atlas16 ? (float)sc.atlas16[o] : sc.atlas[o];
+      }
       for (int k = 0; k < 8; ++k) {
         float dx = (v[(k & 6) | 1] - v[k & 6]) * mesh.grid[0];
         float dy = (v[(k & 5) | 2] - v[k & 5]) * mesh.grid[1];

@@ -38,7 +38,7 @@ __device__ __forceinline__ Surface load_surface(const LightArgs<P>& a, int32_t x
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   if (a.guide) {
-    const GuidePx g = a.guide[id];
+    const GuidePx g = ld_guide(a.guide, id);
     Surface s;
     s.p = v3{g.px, g.py, g.pz};
     s.n = normalize(v3{(float)g.nx, (float)g.ny, (float)g.nz});
@@ -78,7 +78,7 @@ __device__ __forceinline__ bool compatible(const LightArgs<P>& a, const Surface&
   const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
   v3 p, n;
   if (a.guide) {
-    const GuidePx g = a.guide[q];
+    const GuidePx g = ld_guide(a.guide, q);
     if (!g.valid) return false;
     p = v3{g.px, g.py, g.pz};
     n = normalize(v3{(float)g.nx, (float)g.ny, (float)g.nz});
@@ -344,7 +344,7 @@ __device__ __forceinline__ v3 position(const SvgfArgs<P>& a, int32_t x, int32_t 
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   if (a.guide) {
-    const GuidePx g = a.guide[id];
+    const GuidePx g = ld_guide(a.guide, id);
     return v3{g.px, g.py, g.pz};
   }
   return world(pixel_u(x, W), pixel_u(y, H), a.depth[id], a.F.inv_vp);
@@ -356,7 +356,7 @@ __device__ __forceinline__ bool valid_px(const SvgfArgs<P>& a, uint32_t id) {
 template <class P>
 __device__ __forceinline__ v3 normal_px(const SvgfArgs<P>& a, uint32_t id) {
   if (a.guide) {
-    const GuidePx g = a.guide[id];
+    const GuidePx g = ld_guide(a.guide, id);
     return v3{(float)g.nx, (float)g.ny, (float)g.nz};
   }
   return xyz(ld4(a.normal, id));
@@ -381,7 +381,7 @@ __device__ __forceinline__ Guide guide(const SvgfArgs<P>& a, int32_t x, int32_t 
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   Guide g;
   if (a.guide) {
-    const GuidePx r = a.guide[id];
+    const GuidePx r = ld_guide(a.guide, id);
     g.position = v3{r.px, r.py, r.pz};
     g.normal = v3{(float)r.nx, (float)r.ny, (float)r.nz};
     g.color = v3{(float)r.cr, (float)r.cg, (float)r.cb};
@@ -398,9 +398,21 @@ __device__ __forceinline__ Guide guide(const SvgfArgs<P>& a, int32_t x, int32_t 
   g.pixel_scale = length(g.position - ld3(a.F.camera)) / (float)a.F.size[1];
   return g;
 }
-template <class P, class C>
+// FAST (fused schedule): per-pixel reciprocals of the tolerance and of phi
+// replace the per-tap divisions and the hardware square root replaces the
+// correctly rounded one (<= 1 ulp in FP32 before the binary16 store).
+template <bool FAST>
+__device__ __forceinline__ float wsqrt(float x) {
+  if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+  return sqrtf(x);
+}
+__device__ __forceinline__ float tap_tolerance(const Guide& g, float stride) {
+  return fmaxf(0.0001f, g.pixel_scale * stride * 0.5f);
+}
+
+template <class P, class C, bool FAST = false>
 __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Guide& g, int32_t qx, int32_t qy,
-                                                 float stride, C& cnt) {
+                                                 float stride, float inv_tol, C& cnt) {
   cnt.add(C_TAP, 1);
   if (!inside(a, qx, qy)) return 0.0f;
   const uint32_t W = a.F.size[0];
@@ -408,7 +420,7 @@ __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Gui
   v3 nq, wq, cq;
   float2 rm;
   if (a.guide) {
-    const GuidePx r = a.guide[q];
+    const GuidePx r = ld_guide(a.guide, q);
     if (!r.valid) return 0.0f;
     nq = v3{(float)r.nx, (float)r.ny, (float)r.nz};
     wq = v3{r.px, r.py, r.pz};
@@ -423,15 +435,21 @@ __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Gui
     rm = ld2(a.params, q);
   }
   float wn = pow64(fmaxf(dot(g.normal, nq), 0.0f));
-  float tolerance = fmaxf(0.0001f, g.pixel_scale * stride * 0.5f);
-  float wz = __expf(-fabsf(dot(wq - g.position, g.normal)) / tolerance);
+  float plane = fabsf(dot(wq - g.position, g.normal));
+  float wz;
+  if constexpr (FAST) {
+    wz = __expf(-plane * inv_tol);
+  } else {
+    wz = __expf(-plane / tap_tolerance(g, stride));
+  }
   float dm0 = g.m0 - rm.x, dm1 = g.m1 - rm.y;
-  float wc = __expf(-length(g.color - cq) * 8.0f - sqrtf(dm0 * dm0 + dm1 * dm1) * 8.0f);
+  v3 dc = g.color - cq;
+  float wc = __expf(-wsqrt<FAST>(dot(dc, dc)) * 8.0f - wsqrt<FAST>(dm0 * dm0 + dm1 * dm1) * 8.0f);
   return wn * wz * wc;
 }
 
 // svgf_temporal — svgf.wgsl:45-76.
-template <class P, class C>
+template <class P, class C, bool FAST = false>
 __device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, int32_t y, float4* rad_out,
                                               float4* mom_out, C& cnt) {
   const irt_frame& F = a.F;
@@ -476,11 +494,12 @@ __device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, i
     age = fminf(length_history / total + 1.0f, F.temporal[3]);
     float lo = 1e20f, hi = 0.0f;
     Guide g = guide(a, x, y);
+    const float inv_tol = FAST ? 1.0f / tap_tolerance(g, 1.0f) : 0.0f;
     for (int32_t j = -1; j <= 1; ++j) {
       for (int32_t i = -1; i <= 1; ++i) {
         int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
         int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
-        if (geometry_weight(a, g, qx, qy, 1.0f, cnt) > 0.01f) {
+        if (geometry_weight<P, C, FAST>(a, g, qx, qy, 1.0f, inv_tol, cnt) > 0.01f) {
           float v = luminance(xyz(ld4(a.noisy, (uint32_t)qy * W + (uint32_t)qx)));
           lo = fminf(lo, v);
           hi = fmaxf(hi, v);
@@ -501,7 +520,7 @@ __device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, i
 }
 
 // svgf_variance — svgf.wgsl:77-89.
-template <class P, class C>
+template <class P, class C, bool FAST = false>
 __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x, int32_t y, C& cnt) {
   const uint32_t W = a.F.size[0];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
@@ -511,10 +530,11 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
   if (age < 4.0f && valid_px(a, id)) {
     float s0 = 0.f, s1 = 0.f, total = 0.f;
     Guide g = guide(a, x, y);
+    const float inv_tol = FAST ? 1.0f / tap_tolerance(g, 1.0f) : 0.0f;
     for (int32_t j = -3; j <= 3; ++j) {
       for (int32_t i = -3; i <= 3; ++i) {
         int32_t qx = x + i, qy = y + j;
-        float w = geometry_weight(a, g, qx, qy, 1.0f, cnt);
+        float w = geometry_weight<P, C, FAST>(a, g, qx, qy, 1.0f, inv_tol, cnt);
         if (w <= 0.0f) continue;
         float4 mo = a.moments[(uint32_t)qy * W + (uint32_t)qx];
         s0 += mo.x * w;
@@ -533,7 +553,7 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
 // kernel, svgf_atrous — svgf.wgsl:90-110.
 __device__ __forceinline__ float bspline(int32_t x) { return x == 0 ? 1.0f : ((x == 1 || x == -1) ? 2.0f / 3.0f : 1.0f / 6.0f); }
 
-template <class P, class C>
+template <class P, class C, bool FAST = false>
 __device__ __forceinline__ float4 svgf_atrous(const SvgfArgs<P>& a, int32_t x, int32_t y, C& cnt) {
   const uint32_t W = a.F.size[0], H = a.F.size[1];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
@@ -556,13 +576,16 @@ __device__ __forceinline__ float4 svgf_atrous(const SvgfArgs<P>& a, int32_t x, i
   v3 sum = v3{0.f, 0.f, 0.f};
   float total = 0.f, v = 0.f;
   const int32_t step = (int32_t)a.stride;
+  const float inv_tol = FAST ? 1.0f / tap_tolerance(g, (float)step) : 0.0f;
+  const float inv_phi = FAST ? 1.0f / phi : 0.0f;
   for (int32_t j = -2; j <= 2; ++j) {
     for (int32_t i = -2; i <= 2; ++i) {
       int32_t qx = x + i * step, qy = y + j * step;
-      float geom = geometry_weight(a, g, qx, qy, (float)step, cnt);
+      float geom = geometry_weight<P, C, FAST>(a, g, qx, qy, (float)step, inv_tol, cnt);
       if (geom <= 0.0f) continue;
       float4 smp = ld4(a.input, (uint32_t)qy * W + (uint32_t)qx);
-      float w = bspline(i) * bspline(j) * geom * __expf(-fabsf(luminance(xyz(smp)) - lc) / phi);
+      float dl = fabsf(luminance(xyz(smp)) - lc);
+      float w = bspline(i) * bspline(j) * geom * __expf(FAST ? -dl * inv_phi : -dl / phi);
       sum = sum + xyz(smp) * w;
       v += smp.w * smp.w * w * w;
       total += w;
