@@ -304,6 +304,39 @@ int irt_pass_times(irt_ctx* ctx, char names[][32], float* ms, int capacity);
 int irt_enable_counters(irt_ctx* ctx, int enable);
 int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
 
+/* ------------------------------------------------ batched views
+ * One launch per pass over the tiles of many views (contexts).  The
+ * reference renders one view per Renderer::encode (renderer.rs:1675-1984);
+ * a batch runs that frame for every view at once with the fused schedule's
+ * arithmetic (results equal each view's own irt_encode under
+ * IRT_SCHEDULE_FUSED).  Per frame: set each view's frame, G-buffer and VPLs
+ * as usual, then call irt_batch_encode instead of irt_encode per view.
+ * Views must share the precision profile and the SVGF iteration count, hold
+ * at most 64 objects and 64 meshes, and render debug modes 0..2.
+ * The batch's stream waits for the views' streams and the views' streams
+ * wait for the batch. */
+typedef struct irt_batch irt_batch;
+int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out);
+void irt_batch_destroy(irt_batch* batch);
+const char* irt_batch_last_error(const irt_batch* batch);
+int irt_batch_encode(irt_batch* batch);
+/* Pass kinds: 0 guide, 1 reservoirs A, 2 spatial set-up, 3 resolve combine,
+ * 4 SVGF temporal, 5 SVGF variance, 6 a-trous, 7 history copy.
+ * Tile shapes: 0 = 8x8, 1 = 16x8, 2 = 16x16 pixels. */
+int irt_batch_set_tile(irt_batch* batch, uint32_t kind, uint32_t shape);
+/* Times one frame per tile shape (`repeats` times each) and keeps the
+ * fastest shape per pass kind; renders frames from the views' current
+ * inputs. */
+int irt_batch_autotune(irt_batch* batch, uint32_t repeats, uint32_t shapes_out[8]);
+/* Capture the launch sequence in a HIP graph on first encode and replay it
+ * (per-frame arguments live in device memory uploaded before the replay). */
+int irt_batch_use_graph(irt_batch* batch, int enable);
+int irt_batch_sync(irt_batch* batch);
+void* irt_batch_stream(irt_batch* batch);
+/* One encode with per-pass GPU times (ms[0..7] by pass kind, ms[8] march). */
+int irt_batch_encode_timed(irt_batch* batch, float ms[9]);
+int irt_batch_pass_times(irt_batch* batch, float ms[9]);
+
 /* ------------------------------------------------ host precompute (a23)
  * The scene-side helpers renderer.rs runs on the CPU before upload. */
 

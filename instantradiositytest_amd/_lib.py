@@ -71,6 +71,17 @@ def irt() -> ctypes.CDLL:
         "irt_pass_times": (c_int, [P, P, POINTER(c_float), c_int]),
         "irt_enable_counters": (c_int, [P, c_int]),
         "irt_read_counters": (c_int, [P, POINTER(c_uint64)]),
+        "irt_batch_create": (c_int, [P, u32, POINTER(c_void_p)]),
+        "irt_batch_destroy": (None, [P]),
+        "irt_batch_last_error": (c_char_p, [P]),
+        "irt_batch_encode": (c_int, [P]),
+        "irt_batch_set_tile": (c_int, [P, u32, u32]),
+        "irt_batch_autotune": (c_int, [P, u32, POINTER(c_uint32)]),
+        "irt_batch_use_graph": (c_int, [P, c_int]),
+        "irt_batch_sync": (c_int, [P]),
+        "irt_batch_stream": (c_void_p, [P]),
+        "irt_batch_encode_timed": (c_int, [P, POINTER(c_float)]),
+        "irt_batch_pass_times": (c_int, [P, POINTER(c_float)]),
         "irt_host_is_box_geometry": (c_int, [P, u32, u32, P, u32, P, P]),
         "irt_host_field_lipschitz_bound": (c_float, [P, u32, P]),
         "irt_host_hierarchy": (c_int, [P, P, u32, P]),

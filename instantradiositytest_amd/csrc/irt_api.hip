@@ -1085,3 +1085,5 @@ int irt_read_counters(irt_ctx* c, uint64_t counts[IRT_COUNTER_COUNT]) {
 }
 
 }  // extern "C"
+
+#include "irt_batch_impl.h"

@@ -1,0 +1,547 @@
+// Batched frames: one launch per pass covers the tiles of many views.
+//
+// Included at the end of irt_api.hip (it uses that translation unit's
+// context helpers).  Views keep their own device state; a batch holds, per
+// view, the pass arguments the fused schedule would use (uploaded each frame
+// from double-buffered pinned memory) and tile tables (view, tile x, tile y)
+// for three tile shapes.  Pixel passes are flat launches over the tile table
+// (the dispatcher balances tiles across CUs); the march is one persistent
+// kernel that walks the views in order, stages each view's scene in LDS and
+// drains that view's request slots with the per-wave refill of k_march.
+// The launch sequence can be captured once in a HIP graph and replayed: the
+// kernels read everything that changes per frame from device memory.
+
+namespace {
+
+enum BatchKind {
+  BK_GUIDE = 0,
+  BK_RES_A = 1,
+  BK_SETUP = 2,
+  BK_COMBINE = 3,
+  BK_SVGF_T = 4,
+  BK_SVGF_V = 5,
+  BK_SVGF_A = 6,
+  BK_END = 7,
+  BK_COUNT = 8
+};
+static const char* kBatchNames[BK_COUNT] = {"guide", "reservoirs_a", "spatial_setup", "resolve_combine",
+                                           "svgf_temporal", "svgf_variance", "svgf_atrous", "history"};
+constexpr int kShapes[3][2] = {{8, 8}, {16, 8}, {16, 16}};
+
+template <class P>
+struct BatchView {
+  LightArgs<P> la;  // reservoirs A: in 2, out 1, history 2
+  LightArgs<P> lb;  // spatial set-up / combine: in 1, out 2
+  SvgfArgs<P> st, sv, sa[5];
+  irt_reservoir* direct0;
+  irt_reservoir* indirect0;
+  ResolveRec* rec;
+  RayReq* reqs;
+  float* vis;
+  GuidePx* guide;
+  float* prev_depth;
+  typename P::N4* prev_normal;
+  uint32_t W, H, iterations, pad;
+};
+
+struct MarchView {
+  SceneView sc;
+  irt_frame F;
+  const RayReq* reqs;
+  uint32_t* queue;
+  float* vis;
+  uint32_t n_meshes, n_rays;
+};
+
+template <int K, class P, int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict__ views,
+                                                  const uint32_t* __restrict__ tiles, uint32_t iter) {
+  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+  const uint32_t v = e >> 24, tx = (e >> 12) & 0xfffu, ty = e & 0xfffu;
+  const BatchView<P>& bv = views[v];
+  Px p;
+  p.x = (int32_t)(tx * TW + threadIdx.x % TW);
+  p.y = (int32_t)(ty * TH + threadIdx.x / TW);
+  p.active = (uint32_t)p.x < bv.W && (uint32_t)p.y < bv.H;
+  p.id = p.active ? (uint32_t)p.y * bv.W + (uint32_t)p.x : 0u;
+  Count<false> cnt;
+  if constexpr (K == BK_GUIDE) {
+    if (p.active)
+      body_guide<P>(bv.la.F, bv.la.depth, bv.la.normal, bv.la.color, bv.la.params, bv.guide, p.x, p.y, p.id);
+  } else if constexpr (K == BK_RES_A) {
+    body_reservoirs_a(bv.la, p, bv.direct0, bv.indirect0, cnt);
+  } else if constexpr (K == BK_SETUP) {
+    body_spatial_setup(bv.lb, p, bv.rec, bv.reqs, cnt);
+  } else if constexpr (K == BK_COMBINE) {
+    body_resolve_combine(bv.lb, p, bv.rec, bv.vis);
+  } else if constexpr (K == BK_SVGF_T) {
+    body_svgf_temporal<P, Count<false>, true>(bv.st, p, cnt);
+  } else if constexpr (K == BK_SVGF_V) {
+    body_svgf_variance<P, Count<false>, true>(bv.sv, p, cnt);
+  } else if constexpr (K == BK_SVGF_A) {
+    body_svgf_atrous<P, Count<false>, true>(bv.sa[iter], p, cnt);
+  } else if constexpr (K == BK_END) {
+    if (p.active) {
+      bv.prev_depth[p.id] = bv.la.depth[p.id];
+      bv.prev_normal[p.id] = bv.la.normal[p.id];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_zero_queues(const MarchView* __restrict__ views, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n * 16u) views[i / 16u].queue[i % 16u] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_march_views(const MarchView* __restrict__ views, uint32_t n_views) {
+  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
+  __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
+  __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
+  Count<false> cnt;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const MarchView& mv = views[v];
+    __syncthreads();  // the previous view's readers are done with the LDS records
+    const uint32_t n_obj = mv.F.counts[3];
+    const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
+    for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
+      reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(mv.sc.objects)[i];
+    const uint32_t words_mesh = mv.n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
+    for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
+      reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(mv.sc.meshes)[i];
+    for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = mv.sc.grid.boxes[i];
+    __syncthreads();
+    SceneView lsc = mv.sc;
+    lsc.objects = s_obj;
+    lsc.meshes = s_mesh;
+    lsc.grid.boxes = s_box;
+    march_queue(lsc, mv.F, mv.reqs, mv.n_rays, mv.queue, mv.vis, cnt);
+  }
+}
+
+}  // namespace
+
+struct irt_batch {
+  std::vector<irt_ctx*> views;
+  uint32_t precision = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t view_bytes = 0;
+  void* d_views = nullptr;
+  void* h_views[2] = {nullptr, nullptr};
+  MarchView* d_march = nullptr;
+  MarchView* h_march[2] = {nullptr, nullptr};
+  hipEvent_t staged[2] = {nullptr, nullptr};
+  bool staged_used[2] = {false, false};
+  int flip = 0;
+  uint32_t* d_tiles[3] = {nullptr, nullptr, nullptr};
+  uint32_t n_tiles[3] = {0, 0, 0};
+  uint32_t shape[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t march_blocks = 0;
+  bool use_graph = false;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_iterations = 0;
+  hipEvent_t done = nullptr;
+  hipEvent_t view_ready = nullptr;
+  bool timing = false;
+  hipEvent_t tev[BK_COUNT + 2][2];
+  float last_ms[BK_COUNT + 1];
+  std::string err;
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+namespace {
+
+#define BHIP(b, expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return (b)->fail(IRT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class P>
+void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
+  std::memset(&bv, 0, sizeof(bv));
+  const uint32_t cur = c->frame.size[2] % 2u, old = 1u - cur;
+  bv.la = light_args<P>(c, 2, 1, 2);
+  bv.la.guide = c->guide;
+  bv.la.sc.grid = c->grid;
+  bv.lb = light_args<P>(c, 1, 2, 0);
+  bv.lb.guide = c->guide;
+  bv.lb.sc.grid = c->grid;
+  bv.direct0 = c->direct[0];
+  bv.indirect0 = c->indirect[0];
+  bv.rec = c->resolve_rec;
+  bv.reqs = c->rays;
+  bv.vis = c->vis;
+  bv.guide = c->guide;
+  bv.prev_depth = c->prev_depth;
+  bv.prev_normal = (typename P::N4*)c->prev_normal;
+  bv.W = c->W;
+  bv.H = c->H;
+  bv.iterations = c->s.svgf_iterations;
+  SvgfArgs<P> base = svgf_args<P>(c);
+  base.guide = c->guide;
+  bv.st = base;
+  bv.st.old_radiance = (const typename P::R4*)c->feedback[old];
+  bv.st.old_moments = c->moments[old];
+  bv.st.input = (const typename P::R4*)c->feedback[old];
+  bv.st.moments = c->moments[old];
+  bv.st.out = (typename P::R4*)c->temporal;
+  bv.st.moments_out = c->moments[cur];
+  bv.sv = base;
+  bv.sv.input = (const typename P::R4*)c->temporal;
+  bv.sv.moments = c->moments[cur];
+  bv.sv.out = (typename P::R4*)c->scratch[0];
+  // a-trous with the feedback / final copies folded into the stores (run_svgf, fused)
+  void* in = c->scratch[0];
+  const uint32_t it = c->s.svgf_iterations;
+  for (uint32_t i = 0; i < it; ++i) {
+    const bool last = i + 1 == it;
+    void* out;
+    void* out2 = nullptr;
+    if (i == 0) {
+      out = c->feedback[cur];
+      if (last) out2 = c->history;
+    } else if (last) {
+      out = c->history;
+    } else {
+      out = in == c->scratch[1] ? c->scratch[0] : c->scratch[1];
+    }
+    bv.sa[i] = base;
+    bv.sa[i].input = (const typename P::R4*)in;
+    bv.sa[i].moments = c->moments[cur];
+    bv.sa[i].out = (typename P::R4*)out;
+    bv.sa[i].out2 = (typename P::R4*)out2;
+    bv.sa[i].stride = 1u << i;
+    in = out;
+  }
+  mv.sc = bv.lb.sc;
+  mv.F = c->frame;
+  mv.reqs = c->rays;
+  mv.queue = c->queue;
+  mv.vis = c->vis;
+  mv.n_meshes = c->n_meshes;
+  mv.n_rays = (uint32_t)(2 * c->npx);
+}
+
+template <int K, class P>
+int launch_kind(irt_batch* b, uint32_t iter) {
+  const uint32_t sh = b->shape[K];
+  const BatchView<P>* views = (const BatchView<P>*)b->d_views;
+  const uint32_t n = b->n_tiles[sh];
+  if (n == 0) return IRT_OK;
+  if (sh == 0)
+    hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, b->stream, views, b->d_tiles[0], iter);
+  else if (sh == 1)
+    hipLaunchKernelGGL((k_batch<K, P, 16, 8>), dim3(n), dim3(128), 0, b->stream, views, b->d_tiles[1], iter);
+  else
+    hipLaunchKernelGGL((k_batch<K, P, 16, 16>), dim3(n), dim3(256), 0, b->stream, views, b->d_tiles[2], iter);
+  BHIP(b, hipGetLastError());
+  return IRT_OK;
+}
+
+void bmark(irt_batch* b, int slot, int end) {
+  if (b->timing) (void)hipEventRecord(b->tev[slot][end], b->stream);
+}
+
+// The launch sequence of one batched frame (also what the graph captures).
+template <class P>
+int batch_sequence(irt_batch* b) {
+  const uint32_t nv = (uint32_t)b->views.size();
+  int rc = IRT_OK;
+  bmark(b, BK_GUIDE, 0);
+  rc = launch_kind<BK_GUIDE, P>(b, 0);
+  bmark(b, BK_GUIDE, 1);
+  if (!rc) {
+    bmark(b, BK_RES_A, 0);
+    rc = launch_kind<BK_RES_A, P>(b, 0);
+    bmark(b, BK_RES_A, 1);
+  }
+  if (!rc) {
+    bmark(b, BK_SETUP, 0);
+    hipLaunchKernelGGL(k_zero_queues, dim3((nv * 16 + 63) / 64), dim3(64), 0, b->stream, b->d_march, nv);
+    rc = launch_kind<BK_SETUP, P>(b, 0);
+    bmark(b, BK_SETUP, 1);
+  }
+  if (!rc) {
+    bmark(b, BK_COUNT, 0);  // march
+    hipLaunchKernelGGL(k_march_views, dim3(b->march_blocks), dim3(256), 0, b->stream, b->d_march, nv);
+    BHIP(b, hipGetLastError());
+    bmark(b, BK_COUNT, 1);
+    bmark(b, BK_COMBINE, 0);
+    rc = launch_kind<BK_COMBINE, P>(b, 0);
+    bmark(b, BK_COMBINE, 1);
+  }
+  if (!rc) {
+    bmark(b, BK_SVGF_T, 0);
+    rc = launch_kind<BK_SVGF_T, P>(b, 0);
+    bmark(b, BK_SVGF_T, 1);
+  }
+  if (!rc) {
+    bmark(b, BK_SVGF_V, 0);
+    rc = launch_kind<BK_SVGF_V, P>(b, 0);
+    bmark(b, BK_SVGF_V, 1);
+  }
+  bmark(b, BK_SVGF_A, 0);
+  for (uint32_t i = 0; !rc && i < b->graph_iterations; ++i) rc = launch_kind<BK_SVGF_A, P>(b, i);
+  bmark(b, BK_SVGF_A, 1);
+  if (!rc) {
+    bmark(b, BK_END, 0);
+    rc = launch_kind<BK_END, P>(b, 0);
+    bmark(b, BK_END, 1);
+  }
+  return rc;
+}
+
+template <class P>
+int batch_encode(irt_batch* b) {
+  const uint32_t nv = (uint32_t)b->views.size();
+  // per-view preconditions and lazy structures on the views' own streams
+  for (irt_ctx* c : b->views) {
+    int rc = check_ready(c);
+    if (!rc) rc = ensure_grid(c);
+    if (rc) return b->fail(rc, std::string("view: ") + c->err);
+    if (!c->grid.cells || c->n_objects > LDS_MAX_OBJECTS || c->n_meshes > LDS_MAX_OBJECTS)
+      return b->fail(IRT_ERR_LIMIT, "batched views need scenes of at most 64 objects and meshes");
+    if (c->frame.flags[2] >= 3u) return b->fail(IRT_ERR_STATE, "batched views render debug modes < 3");
+    if (c->s.svgf_iterations != b->graph_iterations)
+      return b->fail(IRT_ERR_INVALID_ARGUMENT, "batched views share one SVGF iteration count");
+    BHIP(b, hipEventRecord(b->view_ready, c->stream));
+    BHIP(b, hipStreamWaitEvent(b->stream, b->view_ready, 0));
+  }
+  // per-frame arguments into the free pinned buffer, then to the device
+  const int f = b->flip;
+  if (b->staged_used[f]) BHIP(b, hipEventSynchronize(b->staged[f]));
+  BatchView<P>* hv = (BatchView<P>*)b->h_views[f];
+  for (uint32_t i = 0; i < nv; ++i) fill_view<P>(b->views[i], hv[i], b->h_march[f][i]);
+  BHIP(b, hipMemcpyAsync(b->d_views, hv, b->view_bytes, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->d_march, b->h_march[f], sizeof(MarchView) * nv, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipEventRecord(b->staged[f], b->stream));
+  b->staged_used[f] = true;
+  b->flip = 1 - f;
+  int rc = IRT_OK;
+  if (b->use_graph && !b->timing) {
+    if (!b->exec) {
+      hipGraph_t graph = nullptr;
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = batch_sequence<P>(b);
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  } else {
+    rc = batch_sequence<P>(b);
+    if (rc) return rc;
+  }
+  BHIP(b, hipEventRecord(b->done, b->stream));
+  for (irt_ctx* c : b->views) {
+    BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
+    c->guide_dirty = false;  // the batch built this frame's guide records
+  }
+  return IRT_OK;
+}
+
+void drop_graph(irt_batch* b) {
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out) {
+  if (!views || !out || count == 0 || count > 255) {
+    irt_host::set_global_error("batch needs 1..255 views");
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  irt_batch* b = new irt_batch();
+  b->views.assign(views, views + count);
+  b->precision = views[0]->precision;
+  b->device = views[0]->device;
+  b->graph_iterations = views[0]->s.svgf_iterations;
+  auto bail = [&](const std::string& m) {
+    irt_host::set_global_error(m);
+    irt_batch_destroy(b);
+    return IRT_ERR_DEVICE;
+  };
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!views[i]) return bail("null view");
+    if (views[i]->precision != b->precision) {
+      irt_host::set_global_error("batched views must share a precision profile");
+      irt_batch_destroy(b);
+      return IRT_ERR_INVALID_ARGUMENT;
+    }
+    if (views[i]->W > 4095 * 8 || views[i]->H > 4095 * 8) return bail("view too large for the tile table");
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed");
+  b->view_bytes = (b->precision == IRT_PRECISION_MIXED ? sizeof(BatchView<ProfileMixed>)
+                                                        : sizeof(BatchView<ProfileF32>)) * count;
+  if (hipMalloc(&b->d_views, b->view_bytes) != hipSuccess ||
+      hipMalloc((void**)&b->d_march, sizeof(MarchView) * count) != hipSuccess)
+    return bail("hipMalloc failed");
+  for (int f = 0; f < 2; ++f) {
+    if (hipHostMalloc(&b->h_views[f], b->view_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_march[f], sizeof(MarchView) * count, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&b->staged[f], hipEventDisableTiming) != hipSuccess)
+      return bail("pinned staging allocation failed");
+  }
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->view_ready, hipEventDisableTiming) != hipSuccess)
+    return bail("hipEventCreate failed");
+  for (auto& pair : b->tev)
+    for (auto& e : pair)
+      if (hipEventCreate(&e) != hipSuccess) return bail("hipEventCreate failed");
+  for (int s = 0; s < 3; ++s) {
+    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
+    std::vector<uint32_t> tiles;
+    for (uint32_t v = 0; v < count; ++v) {
+      const uint32_t nx = (views[v]->W + tw - 1) / tw, ny = (views[v]->H + th - 1) / th;
+      for (uint32_t ty = 0; ty < ny; ++ty)
+        for (uint32_t tx = 0; tx < nx; ++tx) tiles.push_back((v << 24) | (tx << 12) | ty);
+    }
+    b->n_tiles[s] = (uint32_t)tiles.size();
+    if (hipMalloc((void**)&b->d_tiles[s], tiles.size() * 4) != hipSuccess ||
+        hipMemcpy(b->d_tiles[s], tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail("tile table upload failed");
+  }
+  b->march_blocks = views[0]->march_blocks;
+  *out = b;
+  return IRT_OK;
+}
+
+void irt_batch_destroy(irt_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->d_views) (void)hipFree(b->d_views);
+  if (b->d_march) (void)hipFree(b->d_march);
+  for (int f = 0; f < 2; ++f) {
+    if (b->h_views[f]) (void)hipHostFree(b->h_views[f]);
+    if (b->h_march[f]) (void)hipHostFree(b->h_march[f]);
+    if (b->staged[f]) (void)hipEventDestroy(b->staged[f]);
+  }
+  for (auto& t : b->d_tiles)
+    if (t) (void)hipFree(t);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->view_ready) (void)hipEventDestroy(b->view_ready);
+  for (auto& pair : b->tev)
+    for (auto& e : pair)
+      if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* irt_batch_last_error(const irt_batch* b) { return b ? b->err.c_str() : irt_host::global_error(); }
+
+int irt_batch_encode(irt_batch* b) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  return b->precision == IRT_PRECISION_MIXED ? batch_encode<ProfileMixed>(b) : batch_encode<ProfileF32>(b);
+}
+
+int irt_batch_set_tile(irt_batch* b, uint32_t kind, uint32_t shape) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  if (kind >= BK_COUNT || shape > 2) return b->fail(IRT_ERR_INVALID_ARGUMENT, "bad pass kind or tile shape");
+  if (b->shape[kind] != shape) drop_graph(b);
+  b->shape[kind] = shape;
+  return IRT_OK;
+}
+
+int irt_batch_use_graph(irt_batch* b, int enable) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  b->use_graph = enable != 0;
+  if (!b->use_graph) drop_graph(b);
+  return IRT_OK;
+}
+
+int irt_batch_sync(irt_batch* b) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return IRT_OK;
+}
+
+void* irt_batch_stream(irt_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+int irt_batch_pass_times(irt_batch* b, float ms[9]) {
+  if (!b || !ms) return IRT_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k <= BK_COUNT; ++k) ms[k] = b->last_ms[k];
+  return BK_COUNT + 1;
+}
+
+// Times one batched frame per tile shape (all pixel passes at that shape)
+// and keeps, per pass, the fastest shape.  Each trial renders a frame of
+// every view from its current inputs: run it during warm-up.
+int irt_batch_autotune(irt_batch* b, uint32_t repeats, uint32_t shapes_out[8]) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  if (repeats == 0) repeats = 1;
+  float best[BK_COUNT];
+  uint32_t pick[BK_COUNT];
+  for (int k = 0; k < BK_COUNT; ++k) {
+    best[k] = INFINITY;
+    pick[k] = 0;
+  }
+  drop_graph(b);
+  const bool graph = b->use_graph;
+  b->use_graph = false;
+  b->timing = true;
+  for (uint32_t s = 0; s < 3; ++s) {
+    for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = s;
+    float acc[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t r = 0; r < repeats; ++r) {
+      int rc = irt_batch_encode(b);
+      if (rc) {
+        b->timing = false;
+        b->use_graph = graph;
+        return rc;
+      }
+      BHIP(b, hipStreamSynchronize(b->stream));
+      for (int k = 0; k < BK_COUNT; ++k) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, b->tev[k][0], b->tev[k][1]) == hipSuccess) acc[k] += t;
+      }
+    }
+    for (int k = 0; k < BK_COUNT; ++k)
+      if (acc[k] < best[k]) {
+        best[k] = acc[k];
+        pick[k] = s;
+      }
+  }
+  for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = pick[k];
+  b->timing = false;
+  b->use_graph = graph;
+  if (shapes_out)
+    for (int k = 0; k < BK_COUNT; ++k) shapes_out[k] = pick[k];
+  return IRT_OK;
+}
+
+// Per-pass GPU times of one batched frame (no graph), for profiling.
+int irt_batch_encode_timed(irt_batch* b, float ms[9]) {
+  if (!b || !ms) return IRT_ERR_INVALID_ARGUMENT;
+  b->timing = true;
+  int rc = irt_batch_encode(b);
+  b->timing = false;
+  if (rc) return rc;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  for (int k = 0; k <= BK_COUNT; ++k) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, b->tev[k][0], b->tev[k][1]);
+    ms[k] = t;
+    b->last_ms[k] = t;
+  }
+  return IRT_OK;
+}
+
+}  // extern "C"

@@ -142,23 +142,27 @@ __global__ __launch_bounds__(64) void k_resolve_indirect(LightArgs<P> a) {
 // Reservoirs A: candidates (both domains) then temporal reuse (both domains)
 // on the in-register candidate reservoirs.  direct0/indirect0 keep the
 // candidate reservoirs observable when non-null.
+template <class P, class C>
+__device__ __forceinline__ void body_reservoirs_a(const LightArgs<P>& a, const Px& p, irt_reservoir* direct0,
+                                                  irt_reservoir* indirect0, C& cnt) {
+  Surface s;
+  bool valid = valid_surface(a, p, &s);
+  if (!p.active) return;
+  irt_reservoir rd = candidates_direct(a, p.id, valid, s, cnt);
+  irt_reservoir ri = candidates_indirect(a, p.id, valid, s, cnt);
+  if (direct0) direct0[p.id] = rd;
+  if (indirect0) indirect0[p.id] = ri;
+  uint32_t q;
+  bool hv = temporal_gate(a, p, valid, s, &q);
+  a.direct_out[p.id] = temporal_direct(a, rd, p.id, hv, q, s, cnt);
+  a.indirect_out[p.id] = temporal_indirect(a, ri, p.id, hv, q, s, cnt);
+}
 template <class P, bool CNT>
 __global__ __launch_bounds__(64) void k_reservoirs_a(LightArgs<P> a, irt_reservoir* direct0,
                                                      irt_reservoir* indirect0) {
   Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
-  Surface s;
-  bool valid = valid_surface(a, p, &s);
-  if (p.active) {
-    irt_reservoir rd = candidates_direct(a, p.id, valid, s, cnt);
-    irt_reservoir ri = candidates_indirect(a, p.id, valid, s, cnt);
-    if (direct0) direct0[p.id] = rd;
-    if (indirect0) indirect0[p.id] = ri;
-    uint32_t q;
-    bool hv = temporal_gate(a, p, valid, s, &q);
-    a.direct_out[p.id] = temporal_direct(a, rd, p.id, hv, q, s, cnt);
-    a.indirect_out[p.id] = temporal_indirect(a, ri, p.id, hv, q, s, cnt);
-  }
+  body_reservoirs_a(a, p, direct0, indirect0, cnt);
   cnt.flush(a.counters);
 }
 
@@ -220,10 +224,9 @@ __device__ __forceinline__ RayReq make_req(v3 p, v3 n, v3 t, v3 e, uint32_t slot
   return RayReq{p.x, p.y, p.z, n.x, n.y, n.z, t.x, t.y, t.z, e.x, e.y, e.z, slot, 1u, 0u, 0u};
 }
 
-template <class P, bool CNT>
-__global__ __launch_bounds__(64) void k_spatial_setup(LightArgs<P> a, ResolveRec* rec, RayReq* reqs) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
-  Count<CNT> cnt;
+template <class P, class C>
+__device__ __forceinline__ void body_spatial_setup(const LightArgs<P>& a, const Px& p, ResolveRec* rec, RayReq* reqs,
+                                                   C& cnt) {
   Surface s;
   const irt_frame& F = a.F;
   bool valid = valid_surface(a, p, &s);
@@ -277,44 +280,26 @@ __global__ __launch_bounds__(64) void k_spatial_setup(LightArgs<P> a, ResolveRec
       reqs[2 * p.id + 1].valid = 0u;
     }
   }
+}
+template <class P, bool CNT>
+__global__ __launch_bounds__(64) void k_spatial_setup(LightArgs<P> a, ResolveRec* rec, RayReq* reqs) {
+  Px p = pixel(a.F.size[0], a.F.size[1]);
+  Count<CNT> cnt;
+  body_spatial_setup(a, p, rec, reqs, cnt);
   cnt.flush(a.counters);
 }
 
-// Persistent march over the request slots (queue[1] = slots taken).
-// Scene records are staged in LDS once per workgroup.  Every wave leaves the
-// loop once the queue is exhausted and its last march (<= sdf.x steps) ends.
-constexpr uint32_t LDS_MAX_OBJECTS = 64;
-constexpr uint32_t MARCH_CHUNK = 256;
-constexpr uint32_t MARCH_REFILL = 8;
-template <bool CNT>
-__global__ __launch_bounds__(256) void k_march(SceneView sc, irt_frame F, uint32_t n_meshes, const RayReq* reqs,
-                                               uint32_t n_rays, uint32_t* queue, float* vis,
-                                               unsigned long long* counters) {
-  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
-  __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
-  __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
-  // The host launches this kernel only when the scene fits the LDS arrays
-  // (objects and meshes <= LDS_MAX_OBJECTS) and the candidate grid exists.
-  const uint32_t n_obj = F.counts[3];
-  const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
-  for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
-    reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(sc.objects)[i];
-  const uint32_t words_mesh = n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
-  for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
-    reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(sc.meshes)[i];
-  for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = sc.grid.boxes[i];
-  __syncthreads();
-  Count<CNT> cnt;
-  SceneView lsc = sc;
-  lsc.objects = s_obj;
-  lsc.meshes = s_mesh;
-  lsc.grid.boxes = s_box;
-  // Each wave claims chunks of MARCH_CHUNK rays with one atomic and hands
-  // them to its idle lanes; lanes refill as soon as MARCH_REFILL of them are
-  // idle, so a wave is not held by its longest march.
-  // Every loop iteration runs one scene query per busy lane: phase 0 the
-  // clearance query at the surface point, phase 1 the clearance query at a
-  // VPL endpoint, phase 2 the march steps (sdf.wgsl:61-81 in that order).
+constexpr uint32_t MARCH_CHUNK = 256;  // request slots claimed per atomic
+constexpr uint32_t MARCH_REFILL = 8;   // idle lanes that trigger a refill
+
+// The persistent march loop over one view's request slots (queue[1] = slots
+// taken).  Every loop iteration runs one scene query per busy lane.
+template <class C>
+__device__ __forceinline__ void march_queue(const SceneView& lsc, const irt_frame& F, const RayReq* reqs,
+                                            uint32_t n_rays, uint32_t* queue, float* vis, C& cnt) {
+  // Phase 0: the clearance query at the surface point; phase 1: the
+  // clearance query at a VPL endpoint; phase 2: the march steps
+  // (sdf.wgsl:61-81 in that order).
   const uint32_t lane = __lane_id();
   const float eps0 = F.sdf[1];
   const uint32_t max_steps = f2u_sat(F.sdf[0]);
@@ -445,12 +430,45 @@ __global__ __launch_bounds__(256) void k_march(SceneView sc, irt_frame F, uint32
       }
     }
   }
+}
+
+// Persistent march over the request slots (queue[1] = slots taken).
+// Scene records are staged in LDS once per workgroup.  Every wave leaves the
+// loop once the queue is exhausted and its last march (<= sdf.x steps) ends.
+constexpr uint32_t LDS_MAX_OBJECTS = 64;
+template <bool CNT>
+__global__ __launch_bounds__(256) void k_march(SceneView sc, irt_frame F, uint32_t n_meshes, const RayReq* reqs,
+                                               uint32_t n_rays, uint32_t* queue, float* vis,
+                                               unsigned long long* counters) {
+  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
+  __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
+  __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
+  // The host launches this kernel only when the scene fits the LDS arrays
+  // (objects and meshes <= LDS_MAX_OBJECTS) and the candidate grid exists.
+  const uint32_t n_obj = F.counts[3];
+  const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
+  for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(sc.objects)[i];
+  const uint32_t words_mesh = n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
+  for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(sc.meshes)[i];
+  for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = sc.grid.boxes[i];
+  __syncthreads();
+  Count<CNT> cnt;
+  SceneView lsc = sc;
+  lsc.objects = s_obj;
+  lsc.meshes = s_mesh;
+  lsc.grid.boxes = s_box;
+  // Each wave claims chunks of MARCH_CHUNK rays with one atomic and hands
+  // them to its idle lanes; lanes refill as soon as MARCH_REFILL of them are
+  // idle, so a wave is not held by its longest march.
+  march_queue(lsc, F, reqs, n_rays, queue, vis, cnt);
   cnt.flush(counters);
 }
 
 template <class P>
-__global__ __launch_bounds__(64) void k_resolve_combine(LightArgs<P> a, const ResolveRec* rec, const float* vis) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
+__device__ __forceinline__ void body_resolve_combine(const LightArgs<P>& a, const Px& p, const ResolveRec* rec,
+                                                     const float* vis) {
   if (!p.active) return;
   Surface s;
   bool valid = valid_surface(a, p, &s);
@@ -475,53 +493,62 @@ __global__ __launch_bounds__(64) void k_resolve_combine(LightArgs<P> a, const Re
   st_r16(a.lighting, p.id, make_float4(result.x, result.y, result.z, 1.0f));
 }
 
+template <class P>
+__global__ __launch_bounds__(64) void k_resolve_combine(LightArgs<P> a, const ResolveRec* rec, const float* vis) {
+  body_resolve_combine(a, pixel(a.F.size[0], a.F.size[1]), rec, vis);
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
 // ------------------------------------------------------------------- SVGF
+template <class P, class C, bool FAST>
+__device__ __forceinline__ void body_svgf_temporal(const SvgfArgs<P>& a, const Px& p, C& cnt) {
+  if (!p.active) return;
+  float4 rad, mom;
+  svgf_temporal<P, C, FAST>(a, p.x, p.y, &rad, &mom, cnt);
+  st_r16(a.out, p.id, rad);
+  a.moments_out[p.id] = mom;
+}
+template <class P, class C, bool FAST>
+__device__ __forceinline__ void body_svgf_variance(const SvgfArgs<P>& a, const Px& p, C& cnt) {
+  if (p.active) st_r16(a.out, p.id, svgf_variance<P, C, FAST>(a, p.x, p.y, cnt));
+}
+template <class P, class C, bool FAST>
+__device__ __forceinline__ void body_svgf_atrous(const SvgfArgs<P>& a, const Px& p, C& cnt) {
+  if (!p.active) return;
+  float4 r = svgf_atrous<P, C, FAST>(a, p.x, p.y, cnt);
+  st_r16(a.out, p.id, r);
+  if (a.out2) st_r16(a.out2, p.id, r);
+}
 template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_temporal(SvgfArgs<P> a) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
-  if (p.active) {
-    float4 rad, mom;
-    svgf_temporal<P, Count<CNT>, FAST>(a, p.x, p.y, &rad, &mom, cnt);
-    st_r16(a.out, p.id, rad);
-    a.moments_out[p.id] = mom;
-  }
+  body_svgf_temporal<P, Count<CNT>, FAST>(a, pixel(a.F.size[0], a.F.size[1]), cnt);
   cnt.flush(a.counters);
 }
 template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_variance(SvgfArgs<P> a) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
-  if (p.active) st_r16(a.out, p.id, svgf_variance<P, Count<CNT>, FAST>(a, p.x, p.y, cnt));
+  body_svgf_variance<P, Count<CNT>, FAST>(a, pixel(a.F.size[0], a.F.size[1]), cnt);
   cnt.flush(a.counters);
 }
 template <class P, bool CNT, bool FAST = false>
 __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
-  if (p.active) {
-    float4 r = svgf_atrous<P, Count<CNT>, FAST>(a, p.x, p.y, cnt);
-    st_r16(a.out, p.id, r);
-    if (a.out2) st_r16(a.out2, p.id, r);
-  }
+  body_svgf_atrous<P, Count<CNT>, FAST>(a, pixel(a.F.size[0], a.F.size[1]), cnt);
   cnt.flush(a.counters);
 }
 
 // Guide records of one frame: world position (world(), common.wgsl:46) and
 // the binary16 channels of the G-buffer.
 template <class P>
-__global__ __launch_bounds__(256) void k_guide(irt_frame F, const float* depth, const typename P::N4* normal,
-                                               const typename P::N4* color, const typename P::P2* params,
-                                               GuidePx* out) {
+__device__ __forceinline__ void body_guide(const irt_frame& F, const float* depth, const typename P::N4* normal,
+                                           const typename P::N4* color, const typename P::P2* params, GuidePx* out,
+                                           int32_t x, int32_t y, uint32_t i) {
   const uint32_t W = F.size[0], H = F.size[1];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W * H) return;
-  const int32_t x = (int32_t)(i % W), y = (int32_t)(i / W);
   const float d = depth[i];
   const v3 p = world(pixel_u(x, W), pixel_u(y, H), d, F.inv_vp);
   const float4 n = ld4(normal, i), c = ld4(color, i);
@@ -540,6 +567,15 @@ __global__ __launch_bounds__(256) void k_guide(irt_frame F, const float* depth, 
   g.m0 = (_Float16)m.x;
   g.m1 = (_Float16)m.y;
   out[i] = g;
+}
+template <class P>
+__global__ __launch_bounds__(256) void k_guide(irt_frame F, const float* depth, const typename P::N4* normal,
+                                               const typename P::N4* color, const typename P::P2* params,
+                                               GuidePx* out) {
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  body_guide<P>(F, depth, normal, color, params, out, (int32_t)(i % W), (int32_t)(i / W), i);
 }
 
 // --------------------------------------------------------- candidate grid

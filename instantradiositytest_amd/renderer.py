@@ -231,6 +231,89 @@ class Renderer:
             pass
 
 
+BATCH_KINDS = ("guide", "reservoirs_a", "spatial_setup", "resolve_combine", "svgf_temporal", "svgf_variance",
+               "svgf_atrous", "history")
+TILE_SHAPES = ("8x8", "16x8", "16x16")
+
+
+class Batch:
+    """Many views rendered by one launch per pass (irt_batch_*).
+
+    Each view is a ``Renderer`` (its own device state, fused arithmetic);
+    ``encode`` performs ``begin_frame`` on every view, one batched frame and
+    the per-view bookkeeping, so each view's buffers afterwards hold what its
+    own ``encode`` would have produced.
+    """
+
+    def __init__(self, renderers: list[Renderer], graph: bool = False):
+        L = _lib.irt()
+        self.L = L
+        self.views = list(renderers)
+        arr = (ctypes.c_void_p * len(self.views))(*[r.ctx.value for r in self.views])
+        b = ctypes.c_void_p()
+        check(L.irt_batch_create(arr, len(self.views), ctypes.byref(b)))
+        self.b = b
+        self.use_graph(graph)
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            msg = self.L.irt_batch_last_error(self.b) or b""
+            raise _lib.IrtError(rc, msg.decode())
+        return rc
+
+    def use_graph(self, on: bool = True):
+        self._check(self.L.irt_batch_use_graph(self.b, int(on)))
+
+    def set_tile(self, kind: str | int, shape: str | int):
+        k = BATCH_KINDS.index(kind) if isinstance(kind, str) else kind
+        s = TILE_SHAPES.index(shape) if isinstance(shape, str) else shape
+        self._check(self.L.irt_batch_set_tile(self.b, k, s))
+
+    def begin(self, scenes, inputs):
+        for r, sc, inp in zip(self.views, scenes, inputs):
+            r.begin_frame(sc, inp)
+
+    def finish(self):
+        for r in self.views:
+            r.finish_frame()
+
+    def encode(self, scenes, inputs):
+        """One frame of every view; ``scenes``/``inputs`` are per view."""
+        self.begin(scenes, inputs)
+        self._check(self.L.irt_batch_encode(self.b))
+        self.finish()
+
+    def encode_timed(self, scenes, inputs) -> dict:
+        self.begin(scenes, inputs)
+        ms = (ctypes.c_float * 9)()
+        self._check(self.L.irt_batch_encode_timed(self.b, ms))
+        self.finish()
+        return {**{k: float(ms[i]) for i, k in enumerate(BATCH_KINDS)}, "march": float(ms[8])}
+
+    def autotune(self, scenes, inputs, repeats: int = 2) -> dict:
+        """Picks a tile shape per pass kind; renders 3 * repeats frames of
+        every view from the given inputs (frame bookkeeping advances once)."""
+        self.begin(scenes, inputs)
+        out = (ctypes.c_uint32 * 8)()
+        self._check(self.L.irt_batch_autotune(self.b, repeats, out))
+        self.finish()
+        return {k: TILE_SHAPES[out[i]] for i, k in enumerate(BATCH_KINDS)}
+
+    def sync(self):
+        self._check(self.L.irt_batch_sync(self.b))
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.irt_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def half_to_float(a: np.ndarray) -> np.ndarray:
     """binary16 bit patterns -> float32."""
     return np.ascontiguousarray(a, dtype=np.uint16).view(np.float16).astype(np.float32)
