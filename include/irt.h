@@ -304,6 +304,21 @@ int irt_pass_times(irt_ctx* ctx, char names[][32], float* ms, int capacity);
 int irt_enable_counters(irt_ctx* ctx, int enable);
 int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
 
+/* Device-to-device copy of src's scene (SDF atlas, meshes, objects,
+ * hierarchy, lights, candidate grid) into dst; both contexts must use the
+ * same precision profile.  Synchronises both contexts' streams.  The
+ * reference uploads one scene per Renderer (renderer.rs:713-817); this lets
+ * a server give each session its own copy of a prepared scene without a
+ * host round trip. */
+int irt_copy_scene(irt_ctx* dst, irt_ctx* src);
+
+/* Device bytes held by the context (per-view state + scene). */
+size_t irt_device_bytes(const irt_ctx* ctx);
+/* The scene part of irt_device_bytes (atlas, records, candidate grid). */
+size_t irt_scene_bytes(const irt_ctx* ctx);
+/* Per-view state bytes irt_create would allocate for these settings. */
+size_t irt_estimate_bytes(const irt_settings* settings, uint32_t precision);
+
 /* ------------------------------------------------ batched views
  * One launch per pass over the tiles of many views (contexts).  The
  * reference renders one view per Renderer::encode (renderer.rs:1675-1984);
@@ -311,14 +326,17 @@ int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
  * arithmetic (results equal each view's own irt_encode under
  * IRT_SCHEDULE_FUSED).  Per frame: set each view's frame, G-buffer and VPLs
  * as usual, then call irt_batch_encode instead of irt_encode per view.
- * Views must share the precision profile and the SVGF iteration count, hold
- * at most 64 objects and 64 meshes, and render debug modes 0..2.
- * The batch's stream waits for the views' streams and the views' streams
- * wait for the batch. */
+ * Views must share the precision profile and the device, hold at most 64
+ * objects and 64 meshes, and render debug modes 0..2; sizes and SVGF
+ * iteration counts may differ.  The batch's stream waits for the views'
+ * streams and the views' streams wait for the batch. */
 typedef struct irt_batch irt_batch;
 int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out);
 void irt_batch_destroy(irt_batch* batch);
 const char* irt_batch_last_error(const irt_batch* batch);
+/* Replace the batch's views (1..255 distinct contexts); a no-op when the
+ * list is unchanged.  Drops a captured graph. */
+int irt_batch_set_views(irt_batch* batch, irt_ctx* const* views, uint32_t count);
 int irt_batch_encode(irt_batch* batch);
 /* Pass kinds: 0 guide, 1 reservoirs A, 2 spatial set-up, 3 resolve combine,
  * 4 SVGF temporal, 5 SVGF variance, 6 a-trous, 7 history copy.
@@ -336,6 +354,88 @@ void* irt_batch_stream(irt_batch* batch);
 /* One encode with per-pass GPU times (ms[0..7] by pass kind, ms[8] march). */
 int irt_batch_encode_timed(irt_batch* batch, float ms[9]);
 int irt_batch_pass_times(irt_batch* batch, float ms[9]);
+
+/* ------------------------------------------------ multi-session frame server
+ * Sessions keep their reservoirs, SVGF history, previous G-buffer and VPL
+ * ring resident on the device, each with its own copy of a prepared scene.
+ * Admission: a free slot and room in the HBM budget (per-view state +
+ * scene copy).  Clients submit frame requests whose G-buffer and VPL pool
+ * are device pointers (valid until the frame completes); irt_server_step
+ * renders the oldest pending request of every session that has one in one
+ * batched frame, so requests arriving while earlier steps run join the next
+ * step.  A session's frames run in submission order and its results are the
+ * same whichever sessions share its steps.  The server keeps the frame
+ * bookkeeping of Renderer::encode (renderer.rs:1687-1735, 1973-1982) per
+ * session.  Not thread-safe: one host thread drives a server. */
+typedef struct irt_server irt_server;
+typedef struct irt_server_config {
+  uint32_t max_sessions;     /* 1..255 */
+  uint32_t precision;        /* irt_precision of every session */
+  uint64_t hbm_budget_bytes; /* device bytes for sessions: state + scene copies */
+  uint32_t max_in_flight;    /* batched steps queued on the device, 1..8 */
+  uint32_t use_graph;        /* replay steps of unchanged membership from a HIP graph */
+  uint32_t tile_shapes[8];   /* per batch pass kind (irt_batch_set_tile) */
+} irt_server_config;
+typedef struct irt_frame_request {
+  float vp[16];              /* view-projection, column-major */
+  float camera[3];
+  uint32_t generation;       /* scene generation (renderer.rs:1706) */
+  uint32_t light_count;
+  uint32_t debug_mode;       /* 0..2 */
+  uint32_t vpl_count;        /* must equal the session's settings.vpl_count */
+  uint32_t reserved;
+  const float* depth;        /* device pointers, reference formats (irt_set_gbuffer) */
+  const uint16_t* normal;
+  const uint16_t* color;
+  const uint16_t* params;
+  const uint16_t* motion;
+  const irt_vpl* vpls;
+  uint64_t tag;              /* returned in the completion */
+} irt_frame_request;
+typedef struct irt_completion {
+  uint64_t tag;
+  uint32_t session;
+  uint32_t frame_index;
+  uint64_t pixels;
+  double submit_s;           /* irt_server_now() clock */
+  double launch_s;
+  double done_s;             /* when the server saw the step finish */
+  float gpu_ms;              /* GPU time of the step that rendered the frame */
+  uint32_t batch_views;      /* sessions in that step */
+} irt_completion;
+typedef struct irt_server_status {
+  uint32_t open_sessions;
+  uint32_t idle_contexts;
+  uint32_t steps_in_flight;
+  uint32_t pending_requests;
+  uint64_t device_bytes;
+  uint64_t budget_bytes;
+} irt_server_status;
+
+double irt_server_now(void);
+int irt_server_create(const irt_server_config* config, irt_server** out);
+void irt_server_destroy(irt_server* server);
+const char* irt_server_last_error(const irt_server* server);
+/* Admit a session rendering `scene`'s scene with `settings`.  IRT_ERR_LIMIT
+ * when no slot is free or the HBM budget would be exceeded. */
+int irt_server_open(irt_server* server, const irt_settings* settings, irt_ctx* scene, uint32_t* session);
+/* The session's context (read-backs); NULL for a closed session. */
+irt_ctx* irt_server_context(irt_server* server, uint32_t session);
+/* Ends a session; its pending requests are dropped, frames in flight finish. */
+int irt_server_close(irt_server* server, uint32_t session);
+int irt_server_submit(irt_server* server, uint32_t session, const irt_frame_request* request);
+/* Launches one batched step; returns the number of frames in it (0: nothing
+ * pending, or max_in_flight steps already queued). */
+int irt_server_step(irt_server* server);
+/* Moves up to `capacity` finished frames into `out`; with `wait`, blocks on
+ * the oldest step when nothing has finished yet.  Returns the count. */
+int irt_server_poll(irt_server* server, irt_completion* out, uint32_t capacity, int wait);
+int irt_server_sync(irt_server* server);
+int irt_server_stats(const irt_server* server, irt_server_status* out);
+
+/* Clears every per-view buffer and the frame state, as after irt_create
+ * (the scene is kept). */
+int irt_reset(irt_ctx* ctx);
 
 /* ------------------------------------------------ host precompute (a23)
  * The scene-side helpers renderer.rs runs on the CPU before upload. */

@@ -8,10 +8,10 @@ no fallback path.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-from .types import Settings
+from .types import Completion, FrameRequest, ServerConfig, ServerStatus, Settings
 
 LIB_DIR = Path(__file__).resolve().parent / "_lib"
 
@@ -71,9 +71,27 @@ def irt() -> ctypes.CDLL:
         "irt_pass_times": (c_int, [P, P, POINTER(c_float), c_int]),
         "irt_enable_counters": (c_int, [P, c_int]),
         "irt_read_counters": (c_int, [P, POINTER(c_uint64)]),
+        "irt_copy_scene": (c_int, [P, P]),
+        "irt_device_bytes": (c_size_t, [P]),
+        "irt_scene_bytes": (c_size_t, [P]),
+        "irt_estimate_bytes": (c_size_t, [POINTER(Settings), u32]),
+        "irt_reset": (c_int, [P]),
+        "irt_server_now": (c_double, []),
+        "irt_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
+        "irt_server_destroy": (None, [P]),
+        "irt_server_last_error": (c_char_p, [P]),
+        "irt_server_open": (c_int, [P, POINTER(Settings), P, POINTER(c_uint32)]),
+        "irt_server_context": (c_void_p, [P, u32]),
+        "irt_server_close": (c_int, [P, u32]),
+        "irt_server_submit": (c_int, [P, u32, POINTER(FrameRequest)]),
+        "irt_server_step": (c_int, [P]),
+        "irt_server_poll": (c_int, [P, POINTER(Completion), u32, c_int]),
+        "irt_server_sync": (c_int, [P]),
+        "irt_server_stats": (c_int, [P, POINTER(ServerStatus)]),
         "irt_batch_create": (c_int, [P, u32, POINTER(c_void_p)]),
         "irt_batch_destroy": (None, [P]),
         "irt_batch_last_error": (c_char_p, [P]),
+        "irt_batch_set_views": (c_int, [P, P, u32]),
         "irt_batch_encode": (c_int, [P]),
         "irt_batch_set_tile": (c_int, [P, u32, u32]),
         "irt_batch_autotune": (c_int, [P, u32, POINTER(c_uint32)]),

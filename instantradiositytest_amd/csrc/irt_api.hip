@@ -126,7 +126,8 @@ struct irt_ctx {
   bool timing = false;
   std::vector<Timer> timers;
   size_t n_timers_used = 0;
-  std::vector<void*> allocs;
+  std::vector<std::pair<void*, size_t>> allocs;  // device allocations and their sizes
+  size_t device_bytes = 0;
   std::string err;
 
   int fail(int code, const std::string& m) {
@@ -144,13 +145,29 @@ struct irt_ctx {
 
 namespace {
 
+// Bytes dev_alloc allocates for a request (never zero).
+size_t alloc_size(size_t bytes) { return bytes ? bytes : 256; }
+
 int dev_alloc(irt_ctx* c, void** p, size_t bytes) {
-  if (bytes == 0) bytes = 256;
+  bytes = alloc_size(bytes);
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return c->fail(IRT_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
-  c->allocs.push_back(*p);
+  c->allocs.emplace_back(*p, bytes);
+  c->device_bytes += bytes;
   e = hipMemsetAsync(*p, 0, bytes, c->stream);
   if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return IRT_OK;
+}
+
+// Frees one allocation of the context (stream work using it must be done).
+int dev_free(irt_ctx* c, void* p) {
+  if (!p) return IRT_OK;
+  for (auto& a : c->allocs)
+    if (a.first == p) {
+      c->device_bytes -= a.second;
+      a = {nullptr, 0};
+    }
+  HIP_CHECK(c, hipFree(p));
   return IRT_OK;
 }
 
@@ -161,9 +178,8 @@ int ensure_array(irt_ctx* c, T** p, uint32_t* cap, uint32_t count) {
   if (*p && *cap >= need) return IRT_OK;
   if (*p) {
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    for (auto& a : c->allocs)
-      if (a == *p) a = nullptr;
-    HIP_CHECK(c, hipFree(*p));
+    int rc = dev_free(c, *p);
+    if (rc) return rc;
     *p = nullptr;
   }
   int rc = dev_alloc(c, (void**)p, sizeof(T) * need);
@@ -173,6 +189,64 @@ int ensure_array(irt_ctx* c, T** p, uint32_t* cap, uint32_t count) {
 }
 
 bool mixed(const irt_ctx* c) { return c->precision == IRT_PRECISION_MIXED; }
+
+uint32_t compute_units(int device) {
+  static int cached_device = -1;
+  static uint32_t cached = 0;
+  if (device != cached_device) {
+    hipDeviceProp_t prop;
+    cached = 256;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      cached = (uint32_t)prop.multiProcessorCount;
+    cached_device = device;
+  }
+  return cached;
+}
+
+// Every per-view device buffer (G-buffer, history, reservoirs, VPL ring,
+// SVGF targets, guide records, ray requests) with its size; the scene
+// records and the SDF atlas are allocated by their setters.
+std::vector<std::pair<void**, size_t>> view_alloc_plan(irt_ctx* c, bool m) {
+  const size_t n = c->npx;
+  const size_t n4 = m ? 8 : 16, p2 = m ? 4 : 8, r4 = m ? 8 : 16, l4 = m ? 8 : 16;
+  std::vector<std::pair<void**, size_t>> v;
+  auto A = [&](void* slot, size_t bytes) { v.emplace_back((void**)slot, bytes); };
+  A(&c->depth, n * 4);
+  A(&c->normal, n * n4);
+  A(&c->color, n * n4);
+  A(&c->params, n * p2);
+  A(&c->motion, n * p2);
+  A(&c->prev_depth, n * 4);
+  A(&c->prev_normal, n * n4);
+  if (!m) {
+    A(&c->stage[0], n * 8);
+    A(&c->stage[1], n * 8);
+    A(&c->stage[2], n * 4);
+    A(&c->stage[3], n * 4);
+  }
+  for (int i = 0; i < 3; ++i) {
+    A(&c->direct[i], n * 32);
+    A(&c->indirect[i], n * 32);
+  }
+  A(&c->ring, (size_t)c->slots * c->V * sizeof(irt_vpl));
+  A(&c->direct_lighting, n * l4);
+  A(&c->lighting, n * r4);
+  A(&c->temporal, n * r4);
+  A(&c->scratch[0], n * r4);
+  A(&c->scratch[1], n * r4);
+  A(&c->feedback[0], n * r4);
+  A(&c->feedback[1], n * r4);
+  A(&c->history, n * r4);
+  A(&c->moments[0], n * 16);
+  A(&c->moments[1], n * 16);
+  A(&c->counters, 8 * sizeof(unsigned long long));
+  A(&c->guide, n * sizeof(GuidePx));
+  A(&c->resolve_rec, n * sizeof(ResolveRec));
+  A(&c->rays, 2 * n * sizeof(RayReq));
+  A(&c->vis, 2 * n * sizeof(float));
+  A(&c->queue, 64);
+  return v;
+}
 
 struct BufDesc {
   void* ptr;
@@ -405,11 +479,10 @@ int ensure_grid(irt_ctx* c) {
   const size_t total = (size_t)c->grid.nx * c->grid.ny * c->grid.nz;
   if (c->cells_cap < total) {
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    for (auto& a : c->allocs)
-      if (a == c->cells) a = nullptr;
-    if (c->cells) HIP_CHECK(c, hipFree(c->cells));
+    int rc = dev_free(c, c->cells);
+    if (rc) return rc;
     c->cells = nullptr;
-    int rc = dev_alloc(c, (void**)&c->cells, total * sizeof(CellList));
+    rc = dev_alloc(c, (void**)&c->cells, total * sizeof(CellList));
     if (rc) return rc;
     c->cells_cap = total;
   }
@@ -684,61 +757,16 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   c->npx = pixels;
   c->V = s->vpl_count;
   c->slots = s->restir_max_history + 1;
-  const bool m = precision == IRT_PRECISION_MIXED;
-  const size_t n = c->npx;
-  const size_t n4 = m ? 8 : 16, p2 = m ? 4 : 8, r4 = m ? 8 : 16, l4 = m ? 8 : 16;
   rc = IRT_OK;
-  auto A = [&](void** p, size_t bytes) {
-    if (!rc) rc = dev_alloc(c, p, bytes);
-  };
-  A((void**)&c->depth, n * 4);
-  A(&c->normal, n * n4);
-  A(&c->color, n * n4);
-  A(&c->params, n * p2);
-  A(&c->motion, n * p2);
-  A((void**)&c->prev_depth, n * 4);
-  A(&c->prev_normal, n * n4);
-  if (!m) {
-    A((void**)&c->stage[0], n * 8);
-    A((void**)&c->stage[1], n * 8);
-    A((void**)&c->stage[2], n * 4);
-    A((void**)&c->stage[3], n * 4);
-  }
-  for (int i = 0; i < 3; ++i) {
-    A((void**)&c->direct[i], n * 32);
-    A((void**)&c->indirect[i], n * 32);
-  }
-  A((void**)&c->ring, (size_t)c->slots * c->V * sizeof(irt_vpl));
-  A(&c->direct_lighting, n * l4);
-  A(&c->lighting, n * r4);
-  A(&c->temporal, n * r4);
-  A(&c->scratch[0], n * r4);
-  A(&c->scratch[1], n * r4);
-  A(&c->feedback[0], n * r4);
-  A(&c->feedback[1], n * r4);
-  A(&c->history, n * r4);
-  A((void**)&c->moments[0], n * 16);
-  A((void**)&c->moments[1], n * 16);
-  A((void**)&c->counters, 8 * sizeof(unsigned long long));
-  A((void**)&c->guide, n * sizeof(GuidePx));
-  A((void**)&c->resolve_rec, n * sizeof(ResolveRec));
-  A((void**)&c->rays, 2 * n * sizeof(RayReq));
-  A((void**)&c->vis, 2 * n * sizeof(float));
-  A((void**)&c->queue, 64);
-  if (!rc) {
-    hipDeviceProp_t prop;
-    int cus = 256;
-    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
-      cus = prop.multiProcessorCount;
-    c->march_blocks = (uint32_t)cus * 8u;
-  }
+  for (const auto& a : view_alloc_plan(c, precision == IRT_PRECISION_MIXED))
+    if (!rc) rc = dev_alloc(c, a.first, a.second);
+  if (!rc) c->march_blocks = compute_units(c->device) * 8u;
   // Depth targets start cleared to 1 (far plane) as the reference's depth pass clears them.
   if (!rc) {
-    std::vector<float> ones(n, 1.0f);
-    if (hipMemcpyAsync(c->depth, ones.data(), n * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->prev_depth, ones.data(), n * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-      rc = c->fail(IRT_ERR_DEVICE, "initial depth clear failed");
+    const uint32_t n = (uint32_t)c->npx, one = 0x3f800000u;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->depth, one, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->prev_depth, one, n);
+    if (hipGetLastError() != hipSuccess) rc = c->fail(IRT_ERR_DEVICE, "initial depth clear failed");
   }
   if (rc) {
     irt_host::set_global_error(c->err);
@@ -752,8 +780,8 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
 void irt_destroy(irt_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void* p : c->allocs)
-    if (p) (void)hipFree(p);
+  for (auto& a : c->allocs)
+    if (a.first) (void)hipFree(a.first);
   for (auto& t : c->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -778,9 +806,8 @@ int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, u
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
   for (void** slot : {(void**)&c->atlas, (void**)&c->atlas16}) {
     if (*slot) {
-      for (auto& a : c->allocs)
-        if (a == *slot) a = nullptr;
-      HIP_CHECK(c, hipFree(*slot));
+      int rc = dev_free(c, *slot);
+      if (rc) return rc;
       *slot = nullptr;
     }
   }
@@ -901,6 +928,113 @@ int irt_set_lights(irt_ctx* c, const irt_light* lights, uint32_t count) {
   }
   c->n_lights = count;
   return IRT_OK;
+}
+
+int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
+  if (!dst || !src) return IRT_ERR_INVALID_ARGUMENT;
+  if (dst == src) return IRT_OK;
+  if (dst->precision != src->precision) return dst->fail(IRT_ERR_INVALID_ARGUMENT, "scene copy across precision profiles");
+  if ((!src->atlas && !src->atlas16) || !src->meshes || !src->objects)
+    return dst->fail(IRT_ERR_STATE, "source context has no complete scene");
+  int rc = ensure_grid(src);
+  if (rc) return dst->fail(rc, std::string("source scene: ") + src->err);
+  HIP_CHECK(dst, hipStreamSynchronize(src->stream));
+  HIP_CHECK(dst, hipStreamSynchronize(dst->stream));
+  const size_t texels = (size_t)src->atlas_dims[0] * src->atlas_dims[1] * src->atlas_dims[2];
+  const size_t tb = src->atlas16 ? 2 : 4;
+  void** slot = src->atlas16 ? (void**)&dst->atlas16 : (void**)&dst->atlas;
+  const void* from = src->atlas16 ? (const void*)src->atlas16 : (const void*)src->atlas;
+  const size_t have = (dst->atlas16 || dst->atlas) ? (size_t)dst->atlas_dims[0] * dst->atlas_dims[1] * dst->atlas_dims[2] : 0;
+  if (!*slot || have != texels) {
+    rc = dev_free(dst, *slot);
+    if (!rc) *slot = nullptr;
+    if (!rc) rc = dev_alloc(dst, slot, texels * tb);
+    if (rc) return rc;
+  }
+  HIP_CHECK(dst, hipMemcpyAsync(*slot, from, texels * tb, hipMemcpyDeviceToDevice, dst->stream));
+  for (int a = 0; a < 3; ++a) dst->atlas_dims[a] = src->atlas_dims[a];
+  rc = ensure_array(dst, &dst->meshes, &dst->cap_meshes, src->n_meshes);
+  if (!rc) rc = ensure_array(dst, &dst->objects, &dst->cap_objects, src->n_objects);
+  if (!rc) rc = ensure_array(dst, &dst->nodes, &dst->cap_nodes, src->n_nodes);
+  if (!rc) rc = ensure_array(dst, &dst->lights, &dst->cap_lights, src->n_lights);
+  if (!rc) rc = ensure_array(dst, &dst->obj_boxes, &dst->obj_boxes_cap, 2 * src->n_objects);
+  if (!rc) rc = ensure_array(dst, &dst->mesh_k2, &dst->mesh_k2_cap, src->n_meshes);
+  if (rc) return rc;
+  HIP_CHECK(dst, hipMemcpyAsync(dst->meshes, src->meshes, sizeof(irt_mesh) * src->n_meshes, hipMemcpyDeviceToDevice,
+                                dst->stream));
+  HIP_CHECK(dst, hipMemcpyAsync(dst->objects, src->objects, sizeof(irt_object) * src->n_objects,
+                                hipMemcpyDeviceToDevice, dst->stream));
+  HIP_CHECK(dst, hipMemcpyAsync(dst->nodes, src->nodes, sizeof(irt_node) * src->n_nodes, hipMemcpyDeviceToDevice,
+                                dst->stream));
+  if (src->n_lights)
+    HIP_CHECK(dst, hipMemcpyAsync(dst->lights, src->lights, sizeof(irt_light) * src->n_lights,
+                                  hipMemcpyDeviceToDevice, dst->stream));
+  HIP_CHECK(dst, hipMemcpyAsync(dst->obj_boxes, src->obj_boxes, sizeof(float4) * 2 * src->n_objects,
+                                hipMemcpyDeviceToDevice, dst->stream));
+  HIP_CHECK(dst, hipMemcpyAsync(dst->mesh_k2, src->mesh_k2, sizeof(unsigned int) * src->n_meshes,
+                                hipMemcpyDeviceToDevice, dst->stream));
+  dst->n_meshes = src->n_meshes;
+  dst->n_objects = src->n_objects;
+  dst->n_nodes = src->n_nodes;
+  dst->n_lights = src->n_lights;
+  dst->host_meshes = src->host_meshes;
+  dst->max_mesh_ref = src->max_mesh_ref;
+  dst->grid = src->grid;
+  dst->grid.cells = nullptr;
+  dst->grid.boxes = dst->obj_boxes;
+  if (src->grid.cells) {
+    const size_t total = (size_t)src->grid.nx * src->grid.ny * src->grid.nz;
+    if (dst->cells_cap < total) {
+      rc = dev_free(dst, dst->cells);
+      dst->cells = nullptr;
+      if (!rc) rc = dev_alloc(dst, (void**)&dst->cells, total * sizeof(CellList));
+      if (rc) return rc;
+      dst->cells_cap = total;
+    }
+    HIP_CHECK(dst, hipMemcpyAsync(dst->cells, src->cells, total * sizeof(CellList), hipMemcpyDeviceToDevice,
+                                  dst->stream));
+    dst->grid.cells = dst->cells;
+  }
+  dst->mesh_k_dirty = src->mesh_k_dirty;
+  dst->grid_dirty = false;
+  dst->guide_dirty = true;
+  return IRT_OK;
+}
+
+int irt_reset(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  for (const auto& a : view_alloc_plan(c, mixed(c)))
+    HIP_CHECK(c, hipMemsetAsync(*a.first, 0, alloc_size(a.second), c->stream));
+  const uint32_t n = (uint32_t)c->npx, one = 0x3f800000u;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->depth, one, n);
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->prev_depth, one, n);
+  HIP_CHECK(c, hipGetLastError());
+  std::memset(&c->frame, 0, sizeof(c->frame));
+  c->have_frame = false;
+  c->guide_dirty = true;
+  c->counting = false;
+  c->n_timers_used = 0;
+  return IRT_OK;
+}
+
+size_t irt_device_bytes(const irt_ctx* c) { return c ? c->device_bytes : 0; }
+
+size_t irt_scene_bytes(const irt_ctx* c) {
+  if (!c) return 0;
+  size_t view = 0;
+  for (const auto& a : view_alloc_plan(const_cast<irt_ctx*>(c), mixed(c))) view += alloc_size(a.second);
+  return c->device_bytes > view ? c->device_bytes - view : 0;
+}
+
+size_t irt_estimate_bytes(const irt_settings* s, uint32_t precision) {
+  if (!s) return 0;
+  irt_ctx tmp;
+  tmp.npx = (size_t)s->render_width * s->render_height;
+  tmp.V = s->vpl_count;
+  tmp.slots = s->restir_max_history + 1;
+  size_t total = 0;
+  for (const auto& a : view_alloc_plan(&tmp, precision == IRT_PRECISION_MIXED)) total += alloc_size(a.second);
+  return total;
 }
 
 int irt_set_frame(irt_ctx* c, const irt_frame* f) {
@@ -1087,3 +1221,4 @@ int irt_read_counters(irt_ctx* c, uint64_t counts[IRT_COUNTER_COUNT]) {
 }  // extern "C"
 
 #include "irt_batch_impl.h"
+#include "irt_server_impl.h"

@@ -8,8 +8,9 @@
 // (the dispatcher balances tiles across CUs); the march is one persistent
 // kernel that walks the views in order, stages each view's scene in LDS and
 // drains that view's request slots with the per-wave refill of k_march.
-// The launch sequence can be captured once in a HIP graph and replayed: the
-// kernels read everything that changes per frame from device memory.
+// The launch sequence can be captured once in a HIP graph and replayed while
+// the membership stays the same: the kernels read everything that changes
+// per frame from device memory.
 
 namespace {
 
@@ -24,9 +25,9 @@ enum BatchKind {
   BK_END = 7,
   BK_COUNT = 8
 };
-static const char* kBatchNames[BK_COUNT] = {"guide", "reservoirs_a", "spatial_setup", "resolve_combine",
-                                           "svgf_temporal", "svgf_variance", "svgf_atrous", "history"};
 constexpr int kShapes[3][2] = {{8, 8}, {16, 8}, {16, 16}};
+constexpr uint32_t kMaxBatchViews = 255;  // 8-bit view index in a tile entry
+constexpr uint32_t kMaxTileCoord = 4096;  // 12-bit tile coordinates
 
 template <class P>
 struct BatchView {
@@ -59,6 +60,9 @@ __global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict
   const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
   const uint32_t v = e >> 24, tx = (e >> 12) & 0xfffu, ty = e & 0xfffu;
   const BatchView<P>& bv = views[v];
+  if constexpr (K == BK_SVGF_A) {
+    if (iter >= bv.iterations) return;  // this view has finished its a-trous iterations
+  }
   Px p;
   p.x = (int32_t)(tx * TW + threadIdx.x % TW);
   p.y = (int32_t)(ty * TH + threadIdx.x / TW);
@@ -125,7 +129,8 @@ struct irt_batch {
   uint32_t precision = 0;
   int device = 0;
   hipStream_t stream = nullptr;
-  size_t view_bytes = 0;
+  size_t view_size = 0;       // sizeof(BatchView<P>) of the precision profile
+  uint32_t cap_views = 0;
   void* d_views = nullptr;
   void* h_views[2] = {nullptr, nullptr};
   MarchView* d_march = nullptr;
@@ -133,18 +138,26 @@ struct irt_batch {
   hipEvent_t staged[2] = {nullptr, nullptr};
   bool staged_used[2] = {false, false};
   int flip = 0;
-  uint32_t* d_tiles[3] = {nullptr, nullptr, nullptr};
+  // tile tables of the three shapes, concatenated
+  uint32_t* d_tiles = nullptr;
+  size_t cap_tiles = 0;
+  uint32_t* h_tiles[2] = {nullptr, nullptr};
+  size_t cap_h_tiles[2] = {0, 0};
+  hipEvent_t tiles_staged[2] = {nullptr, nullptr};
+  bool tiles_used[2] = {false, false};
+  int tflip = 0;
+  uint32_t tile_off[3] = {0, 0, 0};
   uint32_t n_tiles[3] = {0, 0, 0};
   uint32_t shape[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t max_iterations = 0;
   uint32_t march_blocks = 0;
   bool use_graph = false;
   hipGraphExec_t exec = nullptr;
-  uint32_t graph_iterations = 0;
   hipEvent_t done = nullptr;
   hipEvent_t view_ready = nullptr;
   bool timing = false;
-  hipEvent_t tev[BK_COUNT + 2][2];
-  float last_ms[BK_COUNT + 1];
+  hipEvent_t tev[BK_COUNT + 1][2];
+  float last_ms[BK_COUNT + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   std::string err;
   int fail(int code, const std::string& m) {
     err = m;
@@ -232,13 +245,14 @@ int launch_kind(irt_batch* b, uint32_t iter) {
   const uint32_t sh = b->shape[K];
   const BatchView<P>* views = (const BatchView<P>*)b->d_views;
   const uint32_t n = b->n_tiles[sh];
+  const uint32_t* tiles = b->d_tiles + b->tile_off[sh];
   if (n == 0) return IRT_OK;
   if (sh == 0)
-    hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, b->stream, views, b->d_tiles[0], iter);
+    hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, b->stream, views, tiles, iter);
   else if (sh == 1)
-    hipLaunchKernelGGL((k_batch<K, P, 16, 8>), dim3(n), dim3(128), 0, b->stream, views, b->d_tiles[1], iter);
+    hipLaunchKernelGGL((k_batch<K, P, 16, 8>), dim3(n), dim3(128), 0, b->stream, views, tiles, iter);
   else
-    hipLaunchKernelGGL((k_batch<K, P, 16, 16>), dim3(n), dim3(256), 0, b->stream, views, b->d_tiles[2], iter);
+    hipLaunchKernelGGL((k_batch<K, P, 16, 16>), dim3(n), dim3(256), 0, b->stream, views, tiles, iter);
   BHIP(b, hipGetLastError());
   return IRT_OK;
 }
@@ -286,7 +300,7 @@ int batch_sequence(irt_batch* b) {
     bmark(b, BK_SVGF_V, 1);
   }
   bmark(b, BK_SVGF_A, 0);
-  for (uint32_t i = 0; !rc && i < b->graph_iterations; ++i) rc = launch_kind<BK_SVGF_A, P>(b, i);
+  for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) rc = launch_kind<BK_SVGF_A, P>(b, i);
   bmark(b, BK_SVGF_A, 1);
   if (!rc) {
     bmark(b, BK_END, 0);
@@ -296,9 +310,17 @@ int batch_sequence(irt_batch* b) {
   return rc;
 }
 
+void drop_graph(irt_batch* b) {
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+}
+
 template <class P>
 int batch_encode(irt_batch* b) {
   const uint32_t nv = (uint32_t)b->views.size();
+  if (nv == 0) return b->fail(IRT_ERR_STATE, "batch has no views");
   // per-view preconditions and lazy structures on the views' own streams
   for (irt_ctx* c : b->views) {
     int rc = check_ready(c);
@@ -307,8 +329,6 @@ int batch_encode(irt_batch* b) {
     if (!c->grid.cells || c->n_objects > LDS_MAX_OBJECTS || c->n_meshes > LDS_MAX_OBJECTS)
       return b->fail(IRT_ERR_LIMIT, "batched views need scenes of at most 64 objects and meshes");
     if (c->frame.flags[2] >= 3u) return b->fail(IRT_ERR_STATE, "batched views render debug modes < 3");
-    if (c->s.svgf_iterations != b->graph_iterations)
-      return b->fail(IRT_ERR_INVALID_ARGUMENT, "batched views share one SVGF iteration count");
     BHIP(b, hipEventRecord(b->view_ready, c->stream));
     BHIP(b, hipStreamWaitEvent(b->stream, b->view_ready, 0));
   }
@@ -317,7 +337,7 @@ int batch_encode(irt_batch* b) {
   if (b->staged_used[f]) BHIP(b, hipEventSynchronize(b->staged[f]));
   BatchView<P>* hv = (BatchView<P>*)b->h_views[f];
   for (uint32_t i = 0; i < nv; ++i) fill_view<P>(b->views[i], hv[i], b->h_march[f][i]);
-  BHIP(b, hipMemcpyAsync(b->d_views, hv, b->view_bytes, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->d_views, hv, sizeof(BatchView<P>) * nv, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipMemcpyAsync(b->d_march, b->h_march[f], sizeof(MarchView) * nv, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipEventRecord(b->staged[f], b->stream));
   b->staged_used[f] = true;
@@ -351,11 +371,99 @@ int batch_encode(irt_batch* b) {
   return IRT_OK;
 }
 
-void drop_graph(irt_batch* b) {
-  if (b->exec) {
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
+// Replaces the batch's views: argument buffers grow as needed and the tile
+// tables are rebuilt and uploaded in stream order (no host wait unless a
+// device buffer has to grow).
+int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
+  if (count == 0 || count > kMaxBatchViews) return b->fail(IRT_ERR_INVALID_ARGUMENT, "batch needs 1..255 views");
+  uint32_t max_it = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const irt_ctx* c = views[i];
+    if (!c) return b->fail(IRT_ERR_INVALID_ARGUMENT, "null view");
+    if (c->precision != b->precision)
+      return b->fail(IRT_ERR_INVALID_ARGUMENT, "batched views must share a precision profile");
+    if (c->device != b->device) return b->fail(IRT_ERR_INVALID_ARGUMENT, "batched views must share a device");
+    if ((c->W + 7) / 8 > kMaxTileCoord || (c->H + 7) / 8 > kMaxTileCoord)
+      return b->fail(IRT_ERR_LIMIT, "view too large for the tile table");
+    for (uint32_t j = 0; j < i; ++j)
+      if (views[j] == c) return b->fail(IRT_ERR_INVALID_ARGUMENT, "a view appears twice in the batch");
+    max_it = std::max(max_it, c->s.svgf_iterations);
   }
+  if (count > b->cap_views) {
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->d_views) BHIP(b, hipFree(b->d_views));
+    if (b->d_march) BHIP(b, hipFree(b->d_march));
+    b->d_views = nullptr;
+    b->d_march = nullptr;
+    for (int f = 0; f < 2; ++f) {
+      if (b->h_views[f]) BHIP(b, hipHostFree(b->h_views[f]));
+      if (b->h_march[f]) BHIP(b, hipHostFree(b->h_march[f]));
+      b->h_views[f] = nullptr;
+      b->h_march[f] = nullptr;
+      b->staged_used[f] = false;
+    }
+    const uint32_t cap = std::max(count, std::min<uint32_t>(kMaxBatchViews, 2 * b->cap_views));
+    BHIP(b, hipMalloc(&b->d_views, b->view_size * cap));
+    BHIP(b, hipMalloc((void**)&b->d_march, sizeof(MarchView) * cap));
+    for (int f = 0; f < 2; ++f) {
+      BHIP(b, hipHostMalloc(&b->h_views[f], b->view_size * cap, hipHostMallocDefault));
+      BHIP(b, hipHostMalloc((void**)&b->h_march[f], sizeof(MarchView) * cap, hipHostMallocDefault));
+    }
+    b->cap_views = cap;
+  }
+  // tile tables: shape s occupies [tile_off[s], tile_off[s] + n_tiles[s])
+  size_t total = 0;
+  uint32_t n[3];
+  for (int s = 0; s < 3; ++s) {
+    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
+    size_t k = 0;
+    for (uint32_t v = 0; v < count; ++v) k += (size_t)((views[v]->W + tw - 1) / tw) * ((views[v]->H + th - 1) / th);
+    if (k > 0x7fffffffull) return b->fail(IRT_ERR_LIMIT, "too many tiles in one batch");
+    n[s] = (uint32_t)k;
+    total += k;
+  }
+  const int f = b->tflip;
+  if (b->tiles_used[f]) BHIP(b, hipEventSynchronize(b->tiles_staged[f]));
+  if (b->cap_h_tiles[f] < total) {
+    if (b->h_tiles[f]) BHIP(b, hipHostFree(b->h_tiles[f]));
+    b->h_tiles[f] = nullptr;
+    BHIP(b, hipHostMalloc((void**)&b->h_tiles[f], total * 4, hipHostMallocDefault));
+    b->cap_h_tiles[f] = total;
+  }
+  if (b->cap_tiles < total) {
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->d_tiles) BHIP(b, hipFree(b->d_tiles));
+    b->d_tiles = nullptr;
+    BHIP(b, hipMalloc((void**)&b->d_tiles, total * 4));
+    b->cap_tiles = total;
+  }
+  uint32_t* h = b->h_tiles[f];
+  size_t k = 0;
+  for (int s = 0; s < 3; ++s) {
+    b->tile_off[s] = (uint32_t)k;
+    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
+    for (uint32_t v = 0; v < count; ++v) {
+      const uint32_t nx = (views[v]->W + tw - 1) / tw, ny = (views[v]->H + th - 1) / th;
+      for (uint32_t ty = 0; ty < ny; ++ty)
+        for (uint32_t tx = 0; tx < nx; ++tx) h[k++] = (v << 24) | (tx << 12) | ty;
+    }
+    b->n_tiles[s] = n[s];
+  }
+  BHIP(b, hipMemcpyAsync(b->d_tiles, h, total * 4, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipEventRecord(b->tiles_staged[f], b->stream));
+  b->tiles_used[f] = true;
+  b->tflip = 1 - f;
+  b->views.assign(views, views + count);
+  b->max_iterations = max_it;
+  drop_graph(b);
+  return IRT_OK;
+}
+
+bool same_views(const irt_batch* b, irt_ctx* const* views, uint32_t count) {
+  if (b->views.size() != count) return false;
+  for (uint32_t i = 0; i < count; ++i)
+    if (b->views[i] != views[i]) return false;
+  return true;
 }
 
 }  // namespace
@@ -363,62 +471,37 @@ void drop_graph(irt_batch* b) {
 extern "C" {
 
 int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out) {
-  if (!views || !out || count == 0 || count > 255) {
+  if (!views || !out || count == 0 || count > kMaxBatchViews || !views[0]) {
     irt_host::set_global_error("batch needs 1..255 views");
     return IRT_ERR_INVALID_ARGUMENT;
   }
   *out = nullptr;
   irt_batch* b = new irt_batch();
-  b->views.assign(views, views + count);
+  for (auto& pair : b->tev)
+    for (auto& e : pair) e = nullptr;
   b->precision = views[0]->precision;
   b->device = views[0]->device;
-  b->graph_iterations = views[0]->s.svgf_iterations;
-  auto bail = [&](const std::string& m) {
+  b->view_size = b->precision == IRT_PRECISION_MIXED ? sizeof(BatchView<ProfileMixed>) : sizeof(BatchView<ProfileF32>);
+  b->march_blocks = views[0]->march_blocks;
+  auto bail = [&](int code, const std::string& m) {
     irt_host::set_global_error(m);
     irt_batch_destroy(b);
-    return IRT_ERR_DEVICE;
+    return code;
   };
-  for (uint32_t i = 0; i < count; ++i) {
-    if (!views[i]) return bail("null view");
-    if (views[i]->precision != b->precision) {
-      irt_host::set_global_error("batched views must share a precision profile");
-      irt_batch_destroy(b);
-      return IRT_ERR_INVALID_ARGUMENT;
-    }
-    if (views[i]->W > 4095 * 8 || views[i]->H > 4095 * 8) return bail("view too large for the tile table");
-  }
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed");
-  b->view_bytes = (b->precision == IRT_PRECISION_MIXED ? sizeof(BatchView<ProfileMixed>)
-                                                        : sizeof(BatchView<ProfileF32>)) * count;
-  if (hipMalloc(&b->d_views, b->view_bytes) != hipSuccess ||
-      hipMalloc((void**)&b->d_march, sizeof(MarchView) * count) != hipSuccess)
-    return bail("hipMalloc failed");
-  for (int f = 0; f < 2; ++f) {
-    if (hipHostMalloc(&b->h_views[f], b->view_bytes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&b->h_march[f], sizeof(MarchView) * count, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&b->staged[f], hipEventDisableTiming) != hipSuccess)
-      return bail("pinned staging allocation failed");
-  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(IRT_ERR_DEVICE, "hipStreamCreate failed");
+  for (int f = 0; f < 2; ++f)
+    if (hipEventCreateWithFlags(&b->staged[f], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->tiles_staged[f], hipEventDisableTiming) != hipSuccess)
+      return bail(IRT_ERR_DEVICE, "hipEventCreate failed");
   if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->view_ready, hipEventDisableTiming) != hipSuccess)
-    return bail("hipEventCreate failed");
+    return bail(IRT_ERR_DEVICE, "hipEventCreate failed");
   for (auto& pair : b->tev)
     for (auto& e : pair)
-      if (hipEventCreate(&e) != hipSuccess) return bail("hipEventCreate failed");
-  for (int s = 0; s < 3; ++s) {
-    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
-    std::vector<uint32_t> tiles;
-    for (uint32_t v = 0; v < count; ++v) {
-      const uint32_t nx = (views[v]->W + tw - 1) / tw, ny = (views[v]->H + th - 1) / th;
-      for (uint32_t ty = 0; ty < ny; ++ty)
-        for (uint32_t tx = 0; tx < nx; ++tx) tiles.push_back((v << 24) | (tx << 12) | ty);
-    }
-    b->n_tiles[s] = (uint32_t)tiles.size();
-    if (hipMalloc((void**)&b->d_tiles[s], tiles.size() * 4) != hipSuccess ||
-        hipMemcpy(b->d_tiles[s], tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return bail("tile table upload failed");
-  }
-  b->march_blocks = views[0]->march_blocks;
+      if (hipEventCreate(&e) != hipSuccess) return bail(IRT_ERR_DEVICE, "hipEventCreate failed");
+  int rc = set_views(b, views, count);
+  if (rc) return bail(rc, b->err);
   *out = b;
   return IRT_OK;
 }
@@ -429,13 +512,14 @@ void irt_batch_destroy(irt_batch* b) {
   drop_graph(b);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_march) (void)hipFree(b->d_march);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
   for (int f = 0; f < 2; ++f) {
     if (b->h_views[f]) (void)hipHostFree(b->h_views[f]);
     if (b->h_march[f]) (void)hipHostFree(b->h_march[f]);
+    if (b->h_tiles[f]) (void)hipHostFree(b->h_tiles[f]);
     if (b->staged[f]) (void)hipEventDestroy(b->staged[f]);
+    if (b->tiles_staged[f]) (void)hipEventDestroy(b->tiles_staged[f]);
   }
-  for (auto& t : b->d_tiles)
-    if (t) (void)hipFree(t);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->view_ready) (void)hipEventDestroy(b->view_ready);
   for (auto& pair : b->tev)
@@ -446,6 +530,13 @@ void irt_batch_destroy(irt_batch* b) {
 }
 
 const char* irt_batch_last_error(const irt_batch* b) { return b ? b->err.c_str() : irt_host::global_error(); }
+
+int irt_batch_set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  if (!views) return b->fail(IRT_ERR_INVALID_ARGUMENT, "null view list");
+  if (same_views(b, views, count)) return IRT_OK;
+  return set_views(b, views, count);
+}
 
 int irt_batch_encode(irt_batch* b) {
   if (!b) return IRT_ERR_INVALID_ARGUMENT;
@@ -481,52 +572,6 @@ int irt_batch_pass_times(irt_batch* b, float ms[9]) {
   return BK_COUNT + 1;
 }
 
-// Times one batched frame per tile shape (all pixel passes at that shape)
-// and keeps, per pass, the fastest shape.  Each trial renders a frame of
-// every view from its current inputs: run it during warm-up.
-int irt_batch_autotune(irt_batch* b, uint32_t repeats, uint32_t shapes_out[8]) {
-  if (!b) return IRT_ERR_INVALID_ARGUMENT;
-  if (repeats == 0) repeats = 1;
-  float best[BK_COUNT];
-  uint32_t pick[BK_COUNT];
-  for (int k = 0; k < BK_COUNT; ++k) {
-    best[k] = INFINITY;
-    pick[k] = 0;
-  }
-  drop_graph(b);
-  const bool graph = b->use_graph;
-  b->use_graph = false;
-  b->timing = true;
-  for (uint32_t s = 0; s < 3; ++s) {
-    for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = s;
-    float acc[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t r = 0; r < repeats; ++r) {
-      int rc = irt_batch_encode(b);
-      if (rc) {
-        b->timing = false;
-        b->use_graph = graph;
-        return rc;
-      }
-      BHIP(b, hipStreamSynchronize(b->stream));
-      for (int k = 0; k < BK_COUNT; ++k) {
-        float t = 0.f;
-        if (hipEventElapsedTime(&t, b->tev[k][0], b->tev[k][1]) == hipSuccess) acc[k] += t;
-      }
-    }
-    for (int k = 0; k < BK_COUNT; ++k)
-      if (acc[k] < best[k]) {
-        best[k] = acc[k];
-        pick[k] = s;
-      }
-  }
-  for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = pick[k];
-  b->timing = false;
-  b->use_graph = graph;
-  if (shapes_out)
-    for (int k = 0; k < BK_COUNT; ++k) shapes_out[k] = pick[k];
-  return IRT_OK;
-}
-
 // Per-pass GPU times of one batched frame (no graph), for profiling.
 int irt_batch_encode_timed(irt_batch* b, float ms[9]) {
   if (!b || !ms) return IRT_ERR_INVALID_ARGUMENT;
@@ -537,10 +582,45 @@ int irt_batch_encode_timed(irt_batch* b, float ms[9]) {
   BHIP(b, hipStreamSynchronize(b->stream));
   for (int k = 0; k <= BK_COUNT; ++k) {
     float t = 0.f;
-    (void)hipEventElapsedTime(&t, b->tev[k][0], b->tev[k][1]);
+    BHIP(b, hipEventElapsedTime(&t, b->tev[k][0], b->tev[k][1]));
     ms[k] = t;
     b->last_ms[k] = t;
   }
+  return IRT_OK;
+}
+
+// Times one batched frame per tile shape (all pixel passes at that shape,
+// `repeats` frames each) and keeps, per pass, the fastest shape.  Each trial
+// renders a frame of every view from its current inputs: run it during
+// warm-up.
+int irt_batch_autotune(irt_batch* b, uint32_t repeats, uint32_t shapes_out[8]) {
+  if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  if (repeats == 0) repeats = 1;
+  float best[BK_COUNT];
+  uint32_t pick[BK_COUNT];
+  for (int k = 0; k < BK_COUNT; ++k) {
+    best[k] = INFINITY;
+    pick[k] = 0;
+  }
+  for (uint32_t s = 0; s < 3; ++s) {
+    for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = s;
+    float acc[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t r = 0; r < repeats; ++r) {
+      float ms[BK_COUNT + 1];
+      int rc = irt_batch_encode_timed(b, ms);
+      if (rc) return rc;
+      for (int k = 0; k < BK_COUNT; ++k) acc[k] += ms[k];
+    }
+    for (int k = 0; k < BK_COUNT; ++k)
+      if (acc[k] < best[k]) {
+        best[k] = acc[k];
+        pick[k] = s;
+      }
+  }
+  for (int k = 0; k < BK_COUNT; ++k) b->shape[k] = pick[k];
+  drop_graph(b);
+  if (shapes_out)
+    for (int k = 0; k < BK_COUNT; ++k) shapes_out[k] = pick[k];
   return IRT_OK;
 }
 

@@ -502,6 +502,10 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // ------------------------------------------------------------------- SVGF
 template <class P, class C, bool FAST>

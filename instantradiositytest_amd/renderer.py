@@ -53,6 +53,21 @@ _BUFFER_FORMAT = {
 }
 
 
+def read_buffer(ctx, name: str, width: int, height: int) -> np.ndarray:
+    """Read-back of one buffer of a context in the reference format
+    (irt_read), shaped [H, W(, C)] for images."""
+    L = _lib.irt()
+    fmt = _BUFFER_FORMAT[name]
+    dt, ch = _FORMAT[fmt]
+    nbytes = L.irt_buffer_bytes(ctx, BUFFERS[name])
+    itemsize = np.dtype(dt).itemsize * (ch or 1)
+    out = np.empty(nbytes // itemsize * (ch or 1), dtype=dt)
+    check(L.irt_read(ctx, BUFFERS[name], ptr(out), nbytes), ctx)
+    if fmt == "vpl":
+        return out
+    return out.reshape((height, width) + ((ch,) if ch else ()))
+
+
 def pack_frame(settings: Settings, vp, previous_vp, camera, frame_index: int, generation: int,
                previous_generation: int, history_enabled: bool, debug_mode: int, light_count: int,
                object_count: int) -> np.ndarray:
@@ -157,16 +172,7 @@ class Renderer:
 
     # -- buffers ------------------------------------------------------------
     def read(self, name: str) -> np.ndarray:
-        fmt = _BUFFER_FORMAT[name]
-        dt, ch = _FORMAT[fmt]
-        nbytes = self.L.irt_buffer_bytes(self.ctx, BUFFERS[name])
-        itemsize = np.dtype(dt).itemsize * (ch or 1)
-        out = np.empty(nbytes // itemsize * (ch or 1), dtype=dt)
-        check(self.L.irt_read(self.ctx, BUFFERS[name], ptr(out), nbytes), self.ctx)
-        if fmt == "vpl":
-            return out
-        shape = (self.H, self.W) + ((ch,) if ch else ())
-        return out.reshape(shape)
+        return read_buffer(self.ctx, name, self.W, self.H)
 
     def write(self, name: str, data: np.ndarray):
         data = np.ascontiguousarray(data)

@@ -129,3 +129,33 @@ def ptr(a: np.ndarray) -> ctypes.c_void_p:
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
     return ctypes.c_void_p(a.ctypes.data)
+
+
+class ServerConfig(ctypes.Structure):
+    """irt_server_config (include/irt.h)."""
+    _fields_ = [("max_sessions", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64), ("max_in_flight", ctypes.c_uint32),
+                ("use_graph", ctypes.c_uint32), ("tile_shapes", ctypes.c_uint32 * 8)]
+
+
+class FrameRequest(ctypes.Structure):
+    """irt_frame_request: one frame of one session; G-buffer planes and VPLs are device pointers."""
+    _fields_ = [("vp", ctypes.c_float * 16), ("camera", ctypes.c_float * 3), ("generation", ctypes.c_uint32),
+                ("light_count", ctypes.c_uint32), ("debug_mode", ctypes.c_uint32), ("vpl_count", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("depth", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("color", ctypes.c_void_p), ("params", ctypes.c_void_p), ("motion", ctypes.c_void_p),
+                ("vpls", ctypes.c_void_p), ("tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    """irt_completion: a finished frame."""
+    _fields_ = [("tag", ctypes.c_uint64), ("session", ctypes.c_uint32), ("frame_index", ctypes.c_uint32),
+                ("pixels", ctypes.c_uint64), ("submit_s", ctypes.c_double), ("launch_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("gpu_ms", ctypes.c_float), ("batch_views", ctypes.c_uint32)]
+
+
+class ServerStatus(ctypes.Structure):
+    """irt_server_status."""
+    _fields_ = [("open_sessions", ctypes.c_uint32), ("idle_contexts", ctypes.c_uint32),
+                ("steps_in_flight", ctypes.c_uint32), ("pending_requests", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64)]

@@ -1,0 +1,191 @@
+"""Host logic of the serving layer on the CPU: the arrival process, the
+simulation loop against a stand-in server (admission order, lifetimes, one
+outstanding request per session, cadence, latency accounting), the C record
+layouts and the per-session device-memory estimate."""
+from __future__ import annotations
+
+import ctypes
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from instantradiositytest_amd import _lib
+from instantradiositytest_amd.serving import (SERVING_MIX, Arrival, batched_view_configs, poisson_arrivals,
+                                              session_config, simulate)
+from instantradiositytest_amd.types import Completion, FrameRequest, ServerConfig, ServerStatus, make_settings
+
+
+class FakeServer:
+    """Completes every launched frame `gpu_s` seconds after its step."""
+
+    def __init__(self, max_sessions=3, gpu_s=0.002):
+        self.max_sessions = max_sessions
+        self.gpu_s = gpu_s
+        self.open_ids = set()
+        self.pending = {}
+        self.inflight = []
+        self.opened = []
+        self.closed = []
+        self.max_outstanding = 0
+        self.submits = {}
+
+    @staticmethod
+    def now():
+        return time.perf_counter()
+
+    def open(self, settings, scene_ctx):
+        if len(self.open_ids) >= self.max_sessions:
+            return None
+        sid = min(set(range(self.max_sessions)) - self.open_ids)
+        self.open_ids.add(sid)
+        self.opened.append((sid, scene_ctx))
+        return sid
+
+    def close(self, sid):
+        self.open_ids.discard(sid)
+        self.closed.append(sid)
+        self.pending.pop(sid, None)
+
+    def submit(self, sid, frame, generation, light_count, vpl_count, tag):
+        assert sid in self.open_ids
+        q = self.pending.setdefault(sid, [])
+        q.append((tag, self.now(), frame))
+        outstanding = len(q) + sum(1 for s in self.inflight for m in s[1] if m[0] == sid)
+        self.max_outstanding = max(self.max_outstanding, outstanding)
+        self.submits.setdefault(sid, []).append(self.now())
+
+    def step(self):
+        members = []
+        for sid, q in self.pending.items():
+            if q:
+                tag, t, _ = q.pop(0)
+                members.append((sid, tag, t))
+        if not members:
+            return 0
+        self.inflight.append((self.now() + self.gpu_s, members))
+        return len(members)
+
+    def poll(self, wait=False):
+        if wait and self.inflight:
+            dt = self.inflight[0][0] - self.now()
+            if dt > 0:
+                time.sleep(dt)
+        out = []
+        while self.inflight and self.inflight[0][0] <= self.now():
+            t_done, members = self.inflight.pop(0)
+            for sid, tag, t in members:
+                out.append(SimpleNamespace(tag=tag, session=sid, pixels=100, submit_s=t, done_s=self.now(),
+                                           gpu_ms=self.gpu_s * 1e3, batch_views=len(members)))
+        return out
+
+    def stats(self):
+        return {"steps_in_flight": len(self.inflight)}
+
+    def sync(self):
+        while self.inflight:
+            self.poll(wait=True)
+
+
+def fake_streams(n=2, frames=4):
+    out = []
+    for i in range(n):
+        out.append(SimpleNamespace(settings=None, scene_ctx=f"scene{i}", frames=[f"s{i}f{k}" for k in range(frames)],
+                                   cs=SimpleNamespace(scene=SimpleNamespace(generation=0, lights=[1, 2])),
+                                   cfg=SimpleNamespace(vpl_count=12)))
+    return out
+
+
+def test_poisson_arrivals_rate_classes_and_lifetimes():
+    rng = np.random.default_rng(7)
+    arr = poisson_arrivals(40.0, 50.0, [2, 2, 1], [p for _, _, p in SERVING_MIX], (120, 600), 24, rng)
+    # 2000 expected arrivals; a Poisson count is within 4 sigma (~180)
+    assert abs(len(arr) - 2000) < 180
+    times = np.array([a.t for a in arr])
+    assert np.all(np.diff(times) > 0) and times[-1] < 50.0
+    gaps = np.diff(times)
+    assert abs(gaps.mean() - 1 / 40) < 0.004
+    streams = np.array([a.stream for a in arr])
+    cls = np.where(streams < 2, 0, np.where(streams < 4, 1, 2))
+    frac = np.bincount(cls, minlength=3) / len(arr)
+    assert np.allclose(frac, [0.4, 0.4, 0.2], atol=0.04)
+    life = np.array([a.lifetime for a in arr])
+    assert life.min() >= 120 and life.max() <= 600 and abs(life.mean() - 360) < 15
+    assert all(0 <= a.offset < 24 for a in arr)
+
+
+def test_simulation_admits_in_order_and_retires_sessions():
+    srv = FakeServer(max_sessions=2, gpu_s=0.002)
+    streams = fake_streams()
+    arrivals = [Arrival(0.0, 0, 3, 1), Arrival(0.0, 1, 2, 0), Arrival(0.001, 0, 2, 3), Arrival(0.002, 1, 1, 2)]
+    rep = simulate(srv, streams, arrivals, duration_s=0.6, warmup_s=0.0, cadence_hz=500.0)
+    # every session ran its lifetime, in arrival order as slots freed up
+    assert rep.sessions_admitted == 4 and rep.sessions_finished == 4
+    assert [s for s, _ in srv.opened][:2] == [0, 1]
+    assert [c for _, c in srv.opened] == ["scene0", "scene1", "scene0", "scene1"]
+    assert rep.frames == 3 + 2 + 2 + 1
+    assert srv.max_outstanding == 1  # never more than one request per session
+    assert rep.peak_sessions == 2
+    assert all(l >= 1.9 for l in rep.latencies_ms)  # at least the stand-in's GPU time
+    assert rep.p99_ms >= rep.p50_ms > 0
+
+
+def test_simulation_cadence_spaces_requests():
+    srv = FakeServer(max_sessions=1, gpu_s=0.0005)
+    rep = simulate(srv, fake_streams(1), [Arrival(0.0, 0, 6, 0)], duration_s=0.4, warmup_s=0.0, cadence_hz=50.0)
+    t = np.array(srv.submits[0])
+    assert len(t) == 6 and rep.frames == 6
+    assert np.all(np.diff(t) >= 0.02 - 1e-3)  # one request per 1/cadence at most
+
+
+def test_simulation_reports_only_after_warmup():
+    srv = FakeServer(max_sessions=1, gpu_s=0.001)
+    rep = simulate(srv, fake_streams(1), [Arrival(0.0, 0, 1000, 0)], duration_s=0.3, warmup_s=0.15,
+                   cadence_hz=100.0)
+    # ~100 frames/s over a 0.15 s window
+    assert 8 <= rep.frames <= 17
+    assert rep.pixels == 100 * rep.frames
+    assert rep.mpix_s == pytest.approx(rep.pixels / 0.15 / 1e6)
+
+
+def test_workload_configs_match_the_stated_ranges():
+    rng = np.random.default_rng(3)
+    for i in range(50):
+        c = session_config(i, 640, 360, 24, rng)
+        assert 8 <= c.n_meshes <= 48 and 3 <= c.vpl_bounces <= 5
+        assert 500 - 5 <= c.vpl_count <= 2000 + 5
+    views = batched_view_configs(4)
+    sizes = [(v.width, v.height) for v in views]
+    assert sizes.count((1920, 1200)) == 4 and sizes.count((1280, 720)) == 4 and sizes.count((640, 360)) == 8
+    assert sum(w * h for w, h in sizes) == 14_745_600
+    assert len({v.seed for v in views}) == 16
+    assert all(8 <= v.n_meshes <= 48 and 300 - 5 <= v.vpl_count <= 2000 + 5 for v in views)
+
+
+def test_server_records_match_the_c_layouts():
+    assert ctypes.sizeof(ServerConfig) == 56
+    assert ctypes.sizeof(FrameRequest) == 152
+    assert ctypes.sizeof(Completion) == 56
+    assert ctypes.sizeof(ServerStatus) == 32
+    assert FrameRequest.depth.offset == 96 and FrameRequest.tag.offset == 144
+    assert Completion.submit_s.offset == 24 and Completion.gpu_ms.offset == 48
+
+
+def test_state_estimate_counts_every_per_view_buffer():
+    L = _lib.irt()
+    W, H, V, hist = 160, 100, 300, 4
+    n = W * H
+    s = make_settings(render_width=W, render_height=H, vpl_count=V, restir_max_history=hist)
+    mixed = L.irt_estimate_bytes(ctypes.byref(s), 1)
+    fp32 = L.irt_estimate_bytes(ctypes.byref(s), 0)
+    ring = (hist + 1) * V * 48
+    # reservoirs 6x32, guide 32, resolve record 32, ray requests 2x64, visibility 2x4, moments 2x16,
+    # depth + previous depth 2x4 = 436 B/px in both profiles
+    common = 6 * 32 + 32 + 32 + 128 + 8 + 32 + 8
+    # mixed: normal/color/previous normal 3x8, params/motion 2x4, direct lighting + 7 radiance targets 8x8
+    assert mixed == n * (common + 24 + 8 + 64) + ring + 64 + 64
+    # fp32: the same planes at twice the width plus 24 B/px of binary16 staging for ingest
+    assert fp32 == n * (common + 48 + 16 + 128 + 24) + ring + 64 + 64
+    s2 = make_settings(render_width=2 * W, render_height=H, vpl_count=V, restir_max_history=hist)
+    assert L.irt_estimate_bytes(ctypes.byref(s2), 1) - mixed == n * (common + 96)
