@@ -1,18 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the per-pixel lighting + denoising path on one MI355X.
 
-A step is one frame of the path for one view: the eight ReSTIR/resolve
-passes and SVGF (temporal, variance, four a-trous) at 1920x1200 with the
+A step is one frame of the path for one view: the ReSTIR candidate /
+temporal / spatial passes, both resolves with the SDF soft-shadow march,
+and SVGF (temporal, variance, four a-trous) at 1920x1200 with the
 reference's default settings, on the synthetic 24-mesh / 128^3-SDF /
 16-light / 1000-VPL room (orbit + 0.5 m dolly).  Every frame's G-buffer and
 VPL pool are generated up front and staged in HBM; the timed region starts
-with them resident and covers ingest + lighting + SVGF + history per frame.
+with them resident and covers G-buffer ingest + lighting + SVGF + history
+per frame.
 
-Prints one JSON line (see the repository's measurement notes for fields).
+Legs (one JSON line carries all of them):
+  headline   fused kernels, mixed storage (16-bit G-buffer / radiance / SDF
+             samples, FP32 reservoirs, moments and march), single view
+  plain      one kernel per reference pass, FP32 storage: the plain FP32
+             path the headline is compared with
+  fused_fp32 fused kernels, FP32 storage
+  batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
+             launch per pass, autotuned tiles, HIP graph replay
+  serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
+             1920x1200 at 0.4 / 0.4 / 0.2, lifetimes U[120,600] frames),
+             up to 32 resident sessions in a 200 GB budget, 60 Hz clients,
+             continuous batching: p50 / p99 frame latency and Mpix/s
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -38,6 +52,7 @@ FLOP_WEIGHTS = {
     "tap": 85,       # SVGF geometry_weight incl. position reconstruction (svgf.wgsl:32-44)
     "march": 35,     # visibility() set-up (sdf.wgsl:61-69)
 }
+MARCH_EVENTS = ("node", "obj_box", "obj_grid", "step", "march")
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
@@ -46,8 +61,9 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def flops_of(counts: dict) -> float:
-    return float(sum(FLOP_WEIGHTS[k] * counts.get(k, 0) for k in FLOP_WEIGHTS))
+def flops_of(counts: dict, keys=None) -> float:
+    keys = FLOP_WEIGHTS if keys is None else keys
+    return float(sum(FLOP_WEIGHTS[k] * counts.get(k, 0) for k in keys))
 
 
 def cpu_baseline(cs, settings, frames: int, threads: int) -> dict:
@@ -71,31 +87,218 @@ def cpu_baseline(cs, settings, frames: int, threads: int) -> dict:
         log(f"[cpu] frame {fi} cumulative {elapsed:.1f}s")
     px = cfg.width * cfg.height * frames
     return {"value": round(px / elapsed / 1e6, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 CPU restatement (oracle) of frames 0-{frames - 1} of the same workload, "
-                      f"{cfg.width}x{cfg.height}, all 8 lighting passes + SVGF, {elapsed:.1f} s"}
+            "sample": f"FP32 CPU restatement (oracle, OpenMP) of frames 0-{frames - 1} of the same workload, "
+                      f"{cfg.width}x{cfg.height}, all lighting passes + SVGF, {elapsed:.1f} s"}
+
+
+def pass_time_entries(r) -> list:
+    cap = 8192
+    names = (ctypes.c_char * 32 * cap)()
+    ms = (ctypes.c_float * cap)()
+    n = r.L.irt_pass_times(r.ctx, ctypes.cast(names, ctypes.c_void_p), ms, cap)
+    return [(bytes(names[i]).split(b"\0", 1)[0].decode(), float(ms[i])) for i in range(max(n, 0))]
+
+
+def run_single(cs, settings, staged, host_frames, precision: str, schedule: str, warmup: int, steps: int):
+    """Frames of one view from staged inputs; per-frame GPU time from events
+    on the view's stream, per-kernel times from the library's timers."""
+    from instantradiositytest_amd import hip
+    from instantradiositytest_amd._lib import check
+    from instantradiositytest_amd.renderer import Renderer
+    r = Renderer(cs.scene, settings, precision=precision, schedule=schedule)
+    L, ctx = r.L, r.ctx
+    stream = L.irt_stream(ctx)
+    gen, nl, no = cs.scene.generation, len(cs.scene.lights), len(cs.scene.objects)
+
+    def submit(fi: int):
+        vp, cam = host_frames[fi]
+        r.begin_frame_device(staged[fi], vp, cam, gen, nl, no)
+        check(L.irt_encode(ctx), ctx)
+        r.finish_frame()
+
+    for fi in range(warmup):
+        submit(fi)
+    r.sync()
+    r.enable_timing(True)
+    starts = [hip.Event() for _ in range(steps)]
+    ends = [hip.Event() for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        submit(warmup + k)
+        ends[k].record(stream)
+    r.sync()
+    wall = time.perf_counter() - t0
+    frame_ms = np.array([starts[k].elapsed_ms(ends[k]) for k in range(steps)])
+    per = {}
+    for name, ms in pass_time_entries(r):
+        per.setdefault(name, []).append(ms)
+    r.enable_timing(False)
+    W, H = settings.render_width, settings.render_height
+    out = {"value": round(W * H * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 4),
+           "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
+           "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
+           "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in per.items()},
+           "kernel_launches_per_frame": {k: len(v) // steps for k, v in per.items()}}
+    log(f"[bench] {schedule}/{precision}: {out['value']} Mpix/s, {out['ms_per_step']} ms/frame, kernels {out['kernel_ms']}")
+    return out, r
+
+
+def reference_counts(r, cs, settings, staged, host_frames, fi: int) -> dict:
+    """Reference-algorithm event counts of one frame, pass by pass (the
+    per-pass kernels walk the hierarchy as sdf.wgsl does), on the frame
+    after the timed ones."""
+    from instantradiositytest_amd._lib import check
+    L, ctx = r.L, r.ctx
+    r.schedule("passes")
+    vp, cam = host_frames[fi]
+    r.begin_frame_device(staged[fi], vp, cam, cs.scene.generation, len(cs.scene.lights), len(cs.scene.objects))
+    out = {}
+    names = ["candidates_direct", "candidates_indirect", "temporal_direct", "temporal_indirect", "spatial_direct",
+             "spatial_indirect", "resolve_direct", "resolve_indirect"]
+    for i, name in enumerate(names):
+        r.enable_counters(True)
+        check(L.irt_pass(ctx, i, 0), ctx)
+        out[name] = r.counters()
+    r.enable_counters(True)
+    check(L.irt_pass(ctx, 8, 0), ctx)
+    out["svgf_temporal"] = r.counters()
+    r.enable_counters(True)
+    check(L.irt_pass(ctx, 9, 0), ctx)
+    out["svgf_variance"] = r.counters()
+    for it in range(settings.svgf_iterations):
+        r.enable_counters(True)
+        check(L.irt_pass(ctx, 10, it), ctx)
+        out[f"svgf_atrous{it}"] = r.counters()
+    r.enable_counters(False)
+    r.sync()
+    return out
+
+
+def kernel_flops(ref: dict) -> dict:
+    """Algorithmic FLOPs of each kernel of the fused schedule (and of the
+    per-pass kernels) from the reference-algorithm counts."""
+    add = lambda *ds: {k: sum(d.get(k, 0) for d in ds) for k in FLOP_WEIGHTS}
+    res_d, res_i = ref["resolve_direct"], ref["resolve_indirect"]
+    out = {
+        "reservoirs_a": flops_of(add(ref["candidates_direct"], ref["candidates_indirect"], ref["temporal_direct"],
+                                     ref["temporal_indirect"])),
+        "spatial_setup": flops_of(add(ref["spatial_direct"], ref["spatial_indirect"])) +
+        FLOP_WEIGHTS["brdf"] * (res_d.get("brdf", 0) + res_i.get("brdf", 0)),
+        "march": flops_of(add(res_d, res_i), MARCH_EVENTS),
+        "visibility_direct": flops_of(res_d),
+        "visibility_indirect": flops_of(res_i),
+    }
+    for k, v in ref.items():
+        if k.startswith("svgf") or k.startswith("candidates") or k.startswith("temporal") or k.startswith("spatial"):
+            out[k] = flops_of(v)
+    return out
+
+
+def run_batched(frames: int, warmup: int, steps: int, graph: bool = True) -> dict:
+    """16 views of the batched workload in one launch per pass."""
+    from instantradiositytest_amd.renderer import Batch, Renderer
+    from instantradiositytest_amd.serving import SceneStream, batched_view_configs
+    t0 = time.perf_counter()
+    cfgs = batched_view_configs(frames)
+    streams = [SceneStream(c, "mixed", frames) for c in cfgs]
+    log(f"[batch] 16 streams staged in {time.perf_counter() - t0:.1f}s")
+    views = []
+    for st in streams:
+        r = Renderer(st.cs.scene, st.settings, precision="mixed", schedule="fused")
+        views.append(r)
+    batch = Batch(views, graph=graph)
+    px = sum(st.pixels for st in streams)
+
+    def begin(fi):
+        for r, st in zip(views, streams):
+            f = st.frames[fi % frames]
+            r.begin_frame_device(f.bufs, f.vp, f.camera, st.cs.scene.generation, len(st.cs.scene.lights),
+                                 len(st.cs.scene.objects))
+
+    begin(0)
+    picks = batch.autotune_current(repeats=2)
+    batch.finish()
+    for fi in range(1, warmup):
+        begin(fi)
+        batch.encode_current()
+        batch.finish()
+    batch.sync()
+    timed = None
+    begin(warmup)
+    timed = batch.encode_timed_current()
+    batch.finish()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        begin(warmup + 1 + k)
+        batch.encode_current()
+        batch.finish()
+    batch.sync()
+    wall = time.perf_counter() - t0
+    out = {"workload": "16 views: 4x1920x1200 + 4x1280x720 + 8x640x360, own seeds, 8-48 meshes at 64^3-128^3, "
+                       "300-2000 VPLs; mixed storage; one launch per pass; HIP graph replay",
+           "pixels_per_frame": px, "frames": steps, "mpix_s": round(px * steps / wall / 1e6, 3),
+           "ms_per_batched_frame": round(wall * 1e3 / steps, 4), "tile_shapes": picks,
+           "kernel_ms": {k: round(v, 4) for k, v in timed.items()}, "graph": graph}
+    log(f"[batch] {out['mpix_s']} Mpix/s, {out['ms_per_batched_frame']} ms per batched frame, tiles {picks}")
+    batch.close()
+    for r in views:
+        r.close()
+    return out
+
+
+def run_serving(seconds: float, warmup_s: float, sessions: int, variants: int, frames: int, seed: int,
+                tile_shapes=None, max_in_flight: int = 2) -> dict:
+    from instantradiositytest_amd.serving import (SERVING_MIX, SceneStream, Server, poisson_arrivals,
+                                                  session_config, simulate)
+    rng = np.random.default_rng(seed)
+    streams = []
+    t0 = time.perf_counter()
+    for w, h, _ in SERVING_MIX:
+        for v in range(variants):
+            streams.append(SceneStream(session_config(seed * 100 + len(streams), w, h, frames, rng), "mixed",
+                                       frames))
+    log(f"[serve] {len(streams)} session streams staged in {time.perf_counter() - t0:.1f}s")
+    arrivals = poisson_arrivals(40.0, seconds, [variants] * 3, [p for _, _, p in SERVING_MIX], (120, 600), frames,
+                                rng)
+    srv = Server(max_sessions=sessions, precision="mixed", budget_bytes=200e9, max_in_flight=max_in_flight,
+                 tile_shapes=tile_shapes)
+    rep = simulate(srv, streams, arrivals, seconds, warmup_s, 60.0, log=log)
+    d = rep.as_dict()
+    d["workload"] = (f"Poisson arrivals 40/s over {seconds:.0f} s, sessions 640x360/1280x720/1920x1200 at "
+                     f"0.4/0.4/0.2, lifetimes U[120,600] frames, <= {sessions} resident sessions, 200 GB budget, "
+                     f"60 Hz clients with one request outstanding, continuous batching ({max_in_flight} steps in "
+                     f"flight); {len(streams)} seeded scene streams of {frames} frames, 8-48 meshes, "
+                     f"500-2000 VPLs; latency = submit to completion; window after {warmup_s:.0f} s warm-up")
+    d["max_sessions"] = sessions
+    srv.close_server()
+    log(f"[serve] {d}")
+    return d
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--config", default="plain-1920x1200")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed"])
-    ap.add_argument("--schedule", default="passes", choices=["passes", "fused"])
     ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=25.0)
+    ap.add_argument("--serve-warmup", type=float, default=5.0)
+    ap.add_argument("--sessions", type=int, default=32)
+    ap.add_argument("--only", default="", help="comma list of legs: headline,plain,fused_fp32,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this path runs one view per GPU with no cross-GPU work; "
-              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU: the path renders each view and each session on one GPU with no "
+              f"cross-GPU work; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
+    legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "batched", "serving"}
 
     from instantradiositytest_amd import hip
-    from instantradiositytest_amd.renderer import Renderer, pack_frame
     from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
-    from instantradiositytest_amd._lib import check
 
     if hip.device_count() < 1:
         print("no HIP device", file=sys.stderr)
@@ -109,9 +312,7 @@ def main() -> int:
     cs = ConfigScene(cfg)
     log(f"[bench] scene {args.config}: {len(cs.scene.meshes)} meshes, {len(cs.scene.objects)} objects, "
         f"atlas {cs.scene.atlas_dims}, built in {time.perf_counter() - t0:.1f}s")
-    # Generate and stage every frame's inputs in HBM.
-    staged = []
-    host_frames = []
+    staged, host_frames = [], []
     t0 = time.perf_counter()
     for fi in range(total + 1):
         inp = cs.frame_inputs(fi)
@@ -120,161 +321,82 @@ def main() -> int:
         host_frames.append((inp.vp, inp.camera))
         if fi % 16 == 0:
             log(f"[bench] staged frame {fi}/{total + 1} ({time.perf_counter() - t0:.1f}s)")
-    r = Renderer(cs.scene, settings, precision=args.precision, schedule=args.schedule)
-    L, ctx = r.L, r.ctx
-    stream = L.irt_stream(ctx)
 
-    def submit(fi: int):
-        vp, cam = host_frames[fi]
-        pvp = vp if r.previous_vp is None else r.previous_vp
-        f = pack_frame(settings, vp, pvp, cam, r.frame_index, cs.scene.generation, r.previous_generation,
-                       r.frame_index > 0 and r.previous_debug == r.debug_mode, r.debug_mode,
-                       len(cs.scene.lights), len(cs.scene.objects))
-        check(L.irt_set_frame(ctx, f.ctypes.data), ctx)
-        d = staged[fi]
-        check(L.irt_set_gbuffer_device(ctx, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, d[4].ptr), ctx)
-        check(L.irt_set_vpls_device(ctx, d[5].ptr, cfg.vpl_count), ctx)
-        check(L.irt_encode(ctx), ctx)
-        r.previous_vp = vp
-        r.previous_generation = cs.scene.generation
-        r.previous_debug = r.debug_mode
-        r.frame_index += 1
+    results = {}
+    head_r = None
+    for leg, prec, sched in (("plain", "fp32", "passes"), ("fused_fp32", "fp32", "fused"),
+                             ("headline", "mixed", "fused")):
+        if leg not in legs:
+            continue
+        res, r = run_single(cs, settings, staged, host_frames, prec, sched, args.warmup, args.steps)
+        results[leg] = res
+        if leg == "headline":
+            head_r = r
+        else:
+            r.close()
 
-    for fi in range(args.warmup):
-        submit(fi)
-    r.sync()
-    log(f"[bench] warmup {args.warmup} frames done")
-    r.enable_timing(True)
-    starts = [hip.Event() for _ in range(args.steps)]
-    ends = [hip.Event() for _ in range(args.steps)]
-    r.sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        starts[k].record(stream)
-        submit(args.warmup + k)
-        ends[k].record(stream)
-    r.sync()
-    wall = time.perf_counter() - t0
-    frame_ms = np.array([starts[k].elapsed_ms(ends[k]) for k in range(args.steps)])
-    pass_entries = pass_time_entries(r)
-    r.enable_timing(False)
-    log(f"[bench] timed {args.steps} frames in {wall:.3f}s")
-    value = W * H * args.steps / wall / 1e6
-    # per-kernel mean durations over the timed region
-    per_pass = {}
-    for name, ms in pass_entries:
-        per_pass.setdefault(name, []).append(ms)
-    pass_ms = {k: float(np.mean(v)) for k, v in per_pass.items()}
-    dominant = max(pass_ms, key=pass_ms.get) if pass_ms else None
-    roofline = None
-    counts_by_pass = {}
-    if not args.no_counters and dominant is not None:
-        counts_by_pass = instrumented_counts(r, cs, settings, staged, host_frames, total)
-        flop = flops_of(counts_by_pass.get(dominant, {}))
-        achieved = flop / (pass_ms[dominant] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "compute": "fp32 vector ALU (no matrix work in this path)",
-                    "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": None,
-                    "flops_per_launch": flop, "launch_ms": round(pass_ms[dominant], 4)}
-    line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{args.config}: single view {W}x{H}, {args.precision} storage, "
-                               f"{args.schedule} schedule, reference default settings",
-                   "width": W, "height": H, "meshes": len(cs.scene.meshes), "objects": len(cs.scene.objects),
-                   "sdf_res": int(cfg.sdf_res[1]), "lights": len(cs.scene.lights), "vpls": cfg.vpl_count,
-                   "precision": args.precision, "schedule": args.schedule},
-        "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
-        "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
-        "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
-        "roofline": roofline,
-    }
-    if counts_by_pass:
-        executed = counts_by_pass.pop("_executed", None)
-        line["events_per_frame"] = {k: v for k, v in counts_by_pass.items()}
-        if executed:
-            line["executed_events_per_frame"] = executed
-            if roofline is not None and dominant in executed:
-                ex = flops_of(executed[dominant]) / (pass_ms[dominant] * 1e-3) / 1e12
-                roofline["executed_tflops"] = round(ex, 3)
-                roofline["executed_frac"] = round(ex / FP32_VALU_PEAK_TFLOPS, 5)
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    head = results.get("headline")
+    if head:
+        line.update({"value": head["value"], "unit": "Mpix/s", "ms_per_step": head["ms_per_step"],
+                     "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"]})
+        line["config"] = {
+            "workload": f"{args.config}: single view {W}x{H}, fused kernels, mixed storage (16-bit G-buffer, "
+                        f"radiance and SDF samples; FP32 arithmetic, reservoirs, moments and march), reference "
+                        f"default settings; inputs staged in HBM",
+            "width": W, "height": H, "meshes": len(cs.scene.meshes), "objects": len(cs.scene.objects),
+            "sdf_res": int(cfg.sdf_res[1]), "lights": len(cs.scene.lights), "vpls": cfg.vpl_count,
+            "storage": "mixed", "schedule": "fused"}
+        line["kernel_ms"] = head["kernel_ms"]
+    if "plain" in results:
+        line["plain_fp32"] = results["plain"]
+        if head:
+            line["speedup_vs_plain_fp32"] = round(head["value"] / results["plain"]["value"], 3)
+    if "fused_fp32" in results:
+        line["fused_fp32"] = results["fused_fp32"]
+        if "plain" in results:
+            line["fused_fp32"]["speedup_vs_plain_fp32"] = round(
+                results["fused_fp32"]["value"] / results["plain"]["value"], 3)
+
+    # roofline of the headline path's dominant kernel
+    if head and head_r is not None and not args.no_counters:
+        ref = reference_counts(head_r, cs, settings, staged, host_frames, total)
+        kf = kernel_flops(ref)
+        launches = head["kernel_launches_per_frame"]
+        per_launch = {k: v for k, v in head["kernel_ms"].items() if launches.get(k, 1) == 1}
+        dominant = max(per_launch, key=per_launch.get)
+        flop = kf.get(dominant)
+        if flop:
+            achieved = flop / (head["kernel_ms"][dominant] * 1e-3) / 1e12
+            line["roofline"] = {
+                "bound": "mfma", "compute": "FP32 vector ALU (no matrix work on this path; peak is the FP32 "
+                                            "vector figure)",
+                "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": None,
+                "flops_per_launch": flop, "launch_ms": head["kernel_ms"][dominant]}
+        line["kernel_tflops"] = {k: round(kf[k] / (ms * 1e-3) / 1e12, 3) for k, ms in per_launch.items() if kf.get(k)}
+        line["reference_events_per_frame"] = ref
+    if head_r is not None:
+        head_r.close()
+    del staged
+
+    if "batched" in legs:
+        line["batched"] = run_batched(12, 4, args.batch_steps)
+    if "serving" in legs:
+        shapes = None
+        if "batched" in line:
+            order = ("guide", "reservoirs_a", "spatial_setup", "resolve_combine", "svgf_temporal", "svgf_variance",
+                     "svgf_atrous", "history")
+            shapes = [("8x8", "16x8", "16x16").index(line["batched"]["tile_shapes"][k]) for k in order]
+        line["serving"] = run_serving(args.serve_seconds, args.serve_warmup, args.sessions, 2, 24, 0xC5, shapes)
+        if "plain" in results:
+            line["serving"]["speedup_vs_plain_fp32"] = round(line["serving"]["mpix_s"] / results["plain"]["value"], 3)
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         line["cpu_baseline"] = cpu_baseline(cs, settings, args.cpu_frames, threads)
     print(json.dumps(line), flush=True)
     return 0
-
-
-def pass_time_entries(r) -> list:
-    import ctypes
-    cap = 8192
-    names = (ctypes.c_char * 32 * cap)()
-    ms = (ctypes.c_float * cap)()
-    n = r.L.irt_pass_times(r.ctx, ctypes.cast(names, ctypes.c_void_p), ms, cap)
-    out = []
-    for i in range(max(n, 0)):
-        out.append((bytes(names[i]).split(b"\0", 1)[0].decode(), float(ms[i])))
-    return out
-
-
-FUSED_MEMBERS = {
-    "reservoirs_a": ["candidates_direct", "candidates_indirect", "temporal_direct", "temporal_indirect"],
-    "reservoirs_b_resolve": ["spatial_direct", "spatial_indirect", "resolve_direct", "resolve_indirect"],
-}
-PASS_TIMER_NAME = {"resolve_direct": "visibility_direct", "resolve_indirect": "visibility_indirect"}
-
-
-def instrumented_counts(r, cs, settings, staged, host_frames, last: int) -> dict:
-    """Event counts per kernel for one more frame, run pass by pass with the
-    instrumented kernels (outside the timed region)."""
-    from instantradiositytest_amd.renderer import pack_frame
-    from instantradiositytest_amd._lib import check
-    L, ctx = r.L, r.ctx
-    vp, cam = host_frames[last]
-    f = pack_frame(settings, vp, r.previous_vp, cam, r.frame_index, cs.scene.generation, r.previous_generation,
-                   True, r.debug_mode, len(cs.scene.lights), len(cs.scene.objects))
-    check(L.irt_set_frame(ctx, f.ctypes.data), ctx)
-    d = staged[last]
-    check(L.irt_set_gbuffer_device(ctx, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, d[4].ptr), ctx)
-    check(L.irt_set_vpls_device(ctx, d[5].ptr, cs.cfg.vpl_count), ctx)
-    out = {}
-    names = ["candidates_direct", "candidates_indirect", "temporal_direct", "temporal_indirect", "spatial_direct",
-             "spatial_indirect", "resolve_direct", "resolve_indirect"]
-    for i, name in enumerate(names):
-        r.enable_counters(True)
-        check(L.irt_pass(ctx, i, 0), ctx)
-        out[PASS_TIMER_NAME.get(name, name)] = r.counters()
-    # leave the passes' results in place for the executed-work count below
-    r.enable_counters(True)
-    check(L.irt_svgf(ctx, settings.svgf_iterations, 0), ctx)
-    out["svgf"] = r.counters()
-    r.enable_counters(False)
-    for fused, members in FUSED_MEMBERS.items():
-        agg = {}
-        for m in members:
-            for k, v in out[PASS_TIMER_NAME.get(m, m)].items():
-                agg[k] = agg.get(k, 0) + v
-        out[fused] = agg
-    if r.schedule_name == "fused":
-        # Work the fused kernels execute (candidate-grid queries count their
-        # candidate tests as node tests), on the same frame's inputs.
-        executed = {}
-        r.enable_counters(True)
-        r.schedule("fused")
-        check(L.irt_lighting(ctx), ctx)
-        executed["reservoirs"] = r.counters()
-        r.enable_counters(True)
-        check(L.irt_svgf(ctx, settings.svgf_iterations, 0), ctx)
-        executed["svgf"] = r.counters()
-        r.enable_counters(False)
-        # split the lighting counts between the two fused kernels: march events belong to reservoirs_b
-        lit = executed.pop("reservoirs")
-        a_brdf = out["reservoirs_a"]["brdf"]
-        executed["reservoirs_a"] = {k: (a_brdf if k == "brdf" else 0) for k in lit}
-        executed["reservoirs_b_resolve"] = {k: (v - a_brdf if k == "brdf" else v) for k, v in lit.items()}
-        out["_executed"] = executed
-    return out
 
 
 if __name__ == "__main__":

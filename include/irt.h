@@ -291,8 +291,10 @@ void* irt_stream(irt_ctx* ctx);
  * order; reading clears the record.  Names: candidates_direct,
  * candidates_indirect, temporal_direct, temporal_indirect, spatial_direct,
  * spatial_indirect, visibility_direct, visibility_indirect, svgf; in the fused
- * schedule the fused kernels report under reservoirs_a, reservoirs_b_resolve,
- * svgf.  Returns the number of entries written. */
+ * schedule the kernels report under reservoirs_a, spatial_setup, march,
+ * resolve_combine (reservoirs_b_resolve for scenes beyond the LDS staging
+ * limits), svgf_temporal, svgf_variance, svgf_atrous0..4.  Returns the
+ * number of entries written. */
 int irt_enable_timing(irt_ctx* ctx, int enable);
 int irt_pass_times(irt_ctx* ctx, char names[][32], float* ms, int capacity);
 

@@ -619,25 +619,30 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
   if (iterations == 0 || iterations > 5) return c->fail(IRT_ERR_INVALID_ARGUMENT, "SVGF iterations must be 1..5");
   const uint32_t cur = c->frame.size[2] % 2u;
   const size_t rb = r4_bytes(c) * c->npx;
-  timer_begin(c, "svgf");
+  const bool fused = c->schedule == IRT_SCHEDULE_FUSED;
   if (debug >= 3) {
+    timer_begin(c, "svgf");
     HIP_CHECK(c, hipMemcpyAsync(c->history, c->lighting, rb, hipMemcpyDeviceToDevice, c->stream));
     timer_end(c);
     return IRT_OK;
   }
-  const bool fused = c->schedule == IRT_SCHEDULE_FUSED;
   int rc = IRT_OK;
-  const GuidePx* guide = nullptr;
   if (fused) {
+    // One timer per kernel; feedback and final copies folded into the
+    // a-trous stores.
+    static const char* atrous_names[5] = {"svgf_atrous0", "svgf_atrous1", "svgf_atrous2", "svgf_atrous3",
+                                          "svgf_atrous4"};
     rc = ensure_guide<P>(c);
     if (rc) return rc;
-    guide = c->guide;
-  }
-  rc = svgf_temporal_pass<P>(c, guide);
-  if (!rc) rc = svgf_variance_pass<P>(c, guide);
-  if (rc) return rc;
-  if (fused) {
-    // Feedback and final copies folded into the a-trous stores.
+    const GuidePx* guide = c->guide;
+    timer_begin(c, "svgf_temporal");
+    rc = svgf_temporal_pass<P>(c, guide);
+    timer_end(c);
+    if (rc) return rc;
+    timer_begin(c, "svgf_variance");
+    rc = svgf_variance_pass<P>(c, guide);
+    timer_end(c);
+    if (rc) return rc;
     void* in = c->scratch[0];
     for (uint32_t i = 0; i < iterations; ++i) {
       const bool last = i + 1 == iterations;
@@ -651,19 +656,25 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
       } else {
         out = in == c->scratch[1] ? c->scratch[0] : c->scratch[1];
       }
+      timer_begin(c, atrous_names[i]);
       rc = svgf_atrous_pass<P>(c, i, in, out, out2, guide);
+      timer_end(c);
       if (rc) return rc;
       in = out;
     }
-  } else {
-    for (uint32_t i = 0; i < iterations; ++i) {
-      rc = svgf_atrous_pass<P>(c, i, c->scratch[i % 2], c->scratch[1 - i % 2], nullptr);
-      if (rc) return rc;
-      if (i == 0)
-        HIP_CHECK(c, hipMemcpyAsync(c->feedback[cur], c->scratch[1], rb, hipMemcpyDeviceToDevice, c->stream));
-    }
-    HIP_CHECK(c, hipMemcpyAsync(c->history, c->scratch[iterations % 2], rb, hipMemcpyDeviceToDevice, c->stream));
+    return IRT_OK;
   }
+  timer_begin(c, "svgf");
+  rc = svgf_temporal_pass<P>(c);
+  if (!rc) rc = svgf_variance_pass<P>(c);
+  if (rc) return rc;
+  for (uint32_t i = 0; i < iterations; ++i) {
+    rc = svgf_atrous_pass<P>(c, i, c->scratch[i % 2], c->scratch[1 - i % 2], nullptr);
+    if (rc) return rc;
+    if (i == 0)
+      HIP_CHECK(c, hipMemcpyAsync(c->feedback[cur], c->scratch[1], rb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIP_CHECK(c, hipMemcpyAsync(c->history, c->scratch[iterations % 2], rb, hipMemcpyDeviceToDevice, c->stream));
   timer_end(c);
   return IRT_OK;
 }

@@ -138,6 +138,25 @@ class Renderer:
         self._pending = (vp, scene.generation)
         return f
 
+    def begin_frame_device(self, bufs, vp, camera, generation: int, light_count: int, object_count: int):
+        """begin_frame with the G-buffer planes and VPL pool already in device
+        memory (`bufs`: objects with a `.ptr` c_void_p, in the order depth,
+        normal, color, params, motion, vpls); the scene records stay as
+        uploaded."""
+        L, ctx = self.L, self.ctx
+        vp = np.asarray(vp, np.float32)
+        pvp = vp if self.previous_vp is None else self.previous_vp
+        history = self.frame_index > 0 and self.previous_debug == self.debug_mode
+        f = pack_frame(self.settings, vp, pvp, camera, self.frame_index, generation, self.previous_generation,
+                       history, self.debug_mode, light_count, object_count)
+        check(L.irt_set_frame(ctx, ptr(f)), ctx)
+        d = bufs
+        check(L.irt_set_gbuffer_device(ctx, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, d[4].ptr), ctx)
+        check(L.irt_set_vpls_device(ctx, d[5].ptr, self.settings.vpl_count), ctx)
+        self.last_frame = f
+        self._pending = (vp, generation)
+        return f
+
     def finish_frame(self):
         """Bookkeeping at the end of Renderer::encode (renderer.rs:1973-1982)."""
         vp, generation = self._pending
@@ -286,24 +305,39 @@ class Batch:
     def encode(self, scenes, inputs):
         """One frame of every view; ``scenes``/``inputs`` are per view."""
         self.begin(scenes, inputs)
-        self._check(self.L.irt_batch_encode(self.b))
+        self.encode_current()
         self.finish()
+
+    def encode_current(self):
+        """One batched frame from the inputs already set on every view."""
+        self._check(self.L.irt_batch_encode(self.b))
 
     def encode_timed(self, scenes, inputs) -> dict:
         self.begin(scenes, inputs)
+        t = self.encode_timed_current()
+        self.finish()
+        return t
+
+    def encode_timed_current(self) -> dict:
         ms = (ctypes.c_float * 9)()
         self._check(self.L.irt_batch_encode_timed(self.b, ms))
-        self.finish()
         return {**{k: float(ms[i]) for i, k in enumerate(BATCH_KINDS)}, "march": float(ms[8])}
 
     def autotune(self, scenes, inputs, repeats: int = 2) -> dict:
         """Picks a tile shape per pass kind; renders 3 * repeats frames of
         every view from the given inputs (frame bookkeeping advances once)."""
         self.begin(scenes, inputs)
+        picks = self.autotune_current(repeats)
+        self.finish()
+        return picks
+
+    def autotune_current(self, repeats: int = 2) -> dict:
         out = (ctypes.c_uint32 * 8)()
         self._check(self.L.irt_batch_autotune(self.b, repeats, out))
-        self.finish()
         return {k: TILE_SHAPES[out[i]] for i, k in enumerate(BATCH_KINDS)}
+
+    def stream(self):
+        return self.L.irt_batch_stream(self.b)
 
     def sync(self):
         self._check(self.L.irt_batch_sync(self.b))

@@ -104,6 +104,30 @@ CONFIGS = {
 }
 
 
+def place_prop(rng, rad: float, placed: list, hx: float, hy: float, hz: float):
+    """A free spot for a prop of bounding radius `rad` (4000 tries): the
+    centre zone, the ring near the walls or the upper band; None if none."""
+    for _ in range(4000):
+        zone = rng.integers(0, 3)
+        x = rng.uniform(-hx + rad, hx - rad)
+        z = rng.uniform(-hz + rad, hz - rad)
+        r = math.hypot(x, z)
+        if zone == 2:
+            y = rng.uniform(1.95 + rad, 2 * hy - rad) if 1.95 + rad < 2 * hy - rad else None
+        else:
+            if zone == 0 and r > 0.95 - rad:
+                continue
+            if zone == 1 and r < 1.95 + rad:
+                continue
+            y = rad if rng.uniform() < 0.6 else rng.uniform(rad, 2 * hy - rad)
+        if y is None:
+            continue
+        if any(math.dist((x, y, z), q[:3]) < rad + q[3] + 0.02 for q in placed):
+            continue
+        return (x, y, z)
+    return None
+
+
 def build_room(cfg: SceneConfig):
     """Seeded room: 3 wall-slab meshes instanced twice each, then props.
 
@@ -150,32 +174,22 @@ def build_room(cfg: SceneConfig):
         k, p = shapes[mesh]
         scale = float(rng.uniform(0.8, 1.2))
         if k == SG_BOX:
-            rad = math.sqrt(sum(v * v for v in p)) * scale
+            extent = math.sqrt(sum(v * v for v in p))
         elif k == SG_SPHERE:
-            rad = p[0] * scale
+            extent = p[0]
         else:
-            rad = (p[0] + p[1]) * scale
-        for _ in range(4000):
-            zone = rng.integers(0, 3)
-            x = rng.uniform(-hx + rad, hx - rad)
-            z = rng.uniform(-hz + rad, hz - rad)
-            r = math.hypot(x, z)
-            if zone == 2:
-                y = rng.uniform(1.95 + rad, 2 * hy - rad) if 1.95 + rad < 2 * hy - rad else None
-            else:
-                if zone == 0 and r > 0.95 - rad:
-                    continue
-                if zone == 1 and r < 1.95 + rad:
-                    continue
-                y = rad if rng.uniform() < 0.6 else rng.uniform(rad, 2 * hy - rad)
-            if y is None:
-                continue
-            if any(math.dist((x, y, z), q[:3]) < rad + q[3] + 0.02 for q in placed):
-                continue
-            placed.append((x, y, z, rad))
-            break
+            extent = p[0] + p[1]
+        # crowded rooms (many props): shrink a prop that finds no free spot
+        for _shrink in range(12):
+            rad = extent * scale
+            spot = place_prop(rng, rad, placed, hx, hy, hz)
+            if spot is not None:
+                break
+            scale *= 0.8
         else:
             raise RuntimeError(f"could not place prop mesh {mesh}")
+        x, y, z = spot
+        placed.append((x, y, z, rad))
         metal = 1.0 if rng.uniform() < 0.2 else 0.0
         objs.append(dict(mesh=mesh, position=(x, y, z), rotation=quat_y(rng.uniform(0, 2 * math.pi)), scale=scale,
                          albedo=tuple(rng.uniform(0.1, 0.9, 3)), rough=float(rng.uniform(0.3, 0.8)), metal=metal))
