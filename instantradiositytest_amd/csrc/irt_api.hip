@@ -606,7 +606,18 @@ int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, co
   a.out = (typename P::R4*)out;
   a.out2 = (typename P::R4*)out2;
   a.stride = 1u << i;
-  LAUNCH_SVGF(c, k_svgf_atrous, P, guide != nullptr, a);
+  if (guide) {
+    // fused schedule: taps staged in LDS per 16 x 16 patch of a stride-lattice coset
+    const uint32_t span = (uint32_t)AT_TILE << i;
+    const dim3 grid((c->W + span - 1) / span, (c->H + span - 1) / span, 1u << (2 * i));
+    if (c->counting)
+      hipLaunchKernelGGL((k_svgf_atrous_lds<P, true>), grid, dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_svgf_atrous_lds<P, false>), grid, dim3(256), 0, c->stream, a);
+    HIP_CHECK(c, hipGetLastError());
+    return IRT_OK;
+  }
+  LAUNCH_SVGF(c, k_svgf_atrous, P, false, a);
   return IRT_OK;
 }
 

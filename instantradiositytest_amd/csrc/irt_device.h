@@ -115,9 +115,7 @@ static_assert(sizeof(GuidePx) == 32, "guide record is 32 bytes");
 // in registers (field-wise access would issue one 16-bit load per channel).
 __device__ __forceinline__ _Float16 lo16(uint32_t w) { return __builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
 __device__ __forceinline__ _Float16 hi16(uint32_t w) { return __builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
-__device__ __forceinline__ GuidePx ld_guide(const GuidePx* g, uint32_t i) {
-  const uint4* p = reinterpret_cast<const uint4*>(g + i);
-  const uint4 a = p[0], b = p[1];
+__device__ __forceinline__ GuidePx unpack_guide(uint4 a, uint4 b) {
   GuidePx r;
   r.px = __uint_as_float(a.x);
   r.py = __uint_as_float(a.y);
@@ -132,6 +130,10 @@ __device__ __forceinline__ GuidePx ld_guide(const GuidePx* g, uint32_t i) {
   r.m0 = lo16(b.w);
   r.m1 = hi16(b.w);
   return r;
+}
+__device__ __forceinline__ GuidePx ld_guide(const GuidePx* g, uint32_t i) {
+  const uint4* p = reinterpret_cast<const uint4*>(g + i);
+  return unpack_guide(p[0], p[1]);
 }
 
 // Storage profiles.

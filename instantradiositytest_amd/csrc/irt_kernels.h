@@ -546,6 +546,47 @@ __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
   cnt.flush(a.counters);
 }
 
+// A-trous iteration with the tap neighbourhood staged in LDS (guide records
+// present).  Block (bx, by, z) renders a 16 x 16 patch of one coset of the
+// stride-s lattice, x = bx*16s + (z % s) + i*s, y = by*16s + (z / s) + j*s:
+// all 5 x 5 taps of those pixels fall on the same lattice, so the block
+// stages the 20 x 20 lattice points around the patch (guide record + input
+// radiance) once and every tap reads LDS.  Arithmetic as svgf_atrous.
+constexpr int32_t AT_TILE = 16, AT_HALO = 2, AT_SIDE = AT_TILE + 2 * AT_HALO;
+template <class P, bool CNT>
+__global__ __launch_bounds__(256) void k_svgf_atrous_lds(SvgfArgs<P> a) {
+  __shared__ uint4 s_g[AT_SIDE * AT_SIDE][2];
+  __shared__ typename P::R4 s_r[AT_SIDE * AT_SIDE];
+  const int32_t W = (int32_t)a.F.size[0], H = (int32_t)a.F.size[1], st = (int32_t)a.stride;
+  const int32_t cx = (int32_t)blockIdx.z % st, cy = (int32_t)blockIdx.z / st;
+  const int32_t x0 = (int32_t)blockIdx.x * AT_TILE * st + cx, y0 = (int32_t)blockIdx.y * AT_TILE * st + cy;
+  for (int32_t e = (int32_t)threadIdx.x; e < AT_SIDE * AT_SIDE; e += 256) {
+    const int32_t gx = x0 + (e % AT_SIDE - AT_HALO) * st, gy = y0 + (e / AT_SIDE - AT_HALO) * st;
+    if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+      const uint32_t q = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+      const uint4* gp = reinterpret_cast<const uint4*>(a.guide + q);
+      s_g[e][0] = gp[0];
+      s_g[e][1] = gp[1];
+      s_r[e] = a.input[q];
+    } else {
+      s_g[e][0] = make_uint4(0u, 0u, 0u, 0u);  // valid = 0: outside the image
+      s_g[e][1] = make_uint4(0u, 0u, 0u, 0u);
+      s_r[e] = typename P::R4{};
+    }
+  }
+  __syncthreads();
+  const int32_t i = (int32_t)threadIdx.x % AT_TILE, j = (int32_t)threadIdx.x / AT_TILE;
+  const int32_t x = x0 + i * st, y = y0 + j * st;
+  Count<CNT> cnt;
+  if (x < W && y < H) {
+    const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+    const float4 r = svgf_atrous_staged<P>(a, x, y, s_g, s_r, i + AT_HALO, j + AT_HALO, AT_SIDE, st == 1, cnt);
+    st_r16(a.out, id, r);
+    if (a.out2) st_r16(a.out2, id, r);
+  }
+  cnt.flush(a.counters);
+}
+
 // Guide records of one frame: world position (world(), common.wgsl:46) and
 // the binary16 channels of the G-buffer.
 template <class P>

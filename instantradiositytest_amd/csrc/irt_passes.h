@@ -375,26 +375,29 @@ __device__ __forceinline__ bool history_ok(const SvgfArgs<P>& a, int32_t px, int
   return length(pa - pb) < F.temporal[1] * fmaxf(length(pa - cam), 1.0f) &&
          dot(normal_px(a, (uint32_t)py * W + (uint32_t)px), xyz(ld4(a.old_normal, q))) > F.temporal[2];
 }
+// Guide of a pixel from its guide record.
+__device__ __forceinline__ Guide guide_of(const irt_frame& F, const GuidePx& r) {
+  Guide g;
+  g.position = v3{r.px, r.py, r.pz};
+  g.normal = v3{(float)r.nx, (float)r.ny, (float)r.nz};
+  g.color = v3{(float)r.cr, (float)r.cg, (float)r.cb};
+  g.m0 = (float)r.m0;
+  g.m1 = (float)r.m1;
+  g.pixel_scale = length(g.position - ld3(F.camera)) / (float)F.size[1];
+  return g;
+}
 template <class P>
 __device__ __forceinline__ Guide guide(const SvgfArgs<P>& a, int32_t x, int32_t y) {
   const uint32_t W = a.F.size[0];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  if (a.guide) return guide_of(a.F, ld_guide(a.guide, id));
   Guide g;
-  if (a.guide) {
-    const GuidePx r = ld_guide(a.guide, id);
-    g.position = v3{r.px, r.py, r.pz};
-    g.normal = v3{(float)r.nx, (float)r.ny, (float)r.nz};
-    g.color = v3{(float)r.cr, (float)r.cg, (float)r.cb};
-    g.m0 = (float)r.m0;
-    g.m1 = (float)r.m1;
-  } else {
-    g.position = position(a, x, y);
-    g.normal = xyz(ld4(a.normal, id));
-    g.color = xyz(ld4(a.color, id));
-    float2 rm = ld2(a.params, id);
-    g.m0 = rm.x;
-    g.m1 = rm.y;
-  }
+  g.position = position(a, x, y);
+  g.normal = xyz(ld4(a.normal, id));
+  g.color = xyz(ld4(a.color, id));
+  float2 rm = ld2(a.params, id);
+  g.m0 = rm.x;
+  g.m1 = rm.y;
   g.pixel_scale = length(g.position - ld3(a.F.camera)) / (float)a.F.size[1];
   return g;
 }
@@ -410,30 +413,11 @@ __device__ __forceinline__ float tap_tolerance(const Guide& g, float stride) {
   return fmaxf(0.0001f, g.pixel_scale * stride * 0.5f);
 }
 
-template <class P, class C, bool FAST = false>
-__device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Guide& g, int32_t qx, int32_t qy,
-                                                 float stride, float inv_tol, C& cnt) {
-  cnt.add(C_TAP, 1);
-  if (!inside(a, qx, qy)) return 0.0f;
-  const uint32_t W = a.F.size[0];
-  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
-  v3 nq, wq, cq;
-  float2 rm;
-  if (a.guide) {
-    const GuidePx r = ld_guide(a.guide, q);
-    if (!r.valid) return 0.0f;
-    nq = v3{(float)r.nx, (float)r.ny, (float)r.nz};
-    wq = v3{r.px, r.py, r.pz};
-    cq = v3{(float)r.cr, (float)r.cg, (float)r.cb};
-    rm = make_float2((float)r.m0, (float)r.m1);
-  } else {
-    float d = a.depth[q];
-    if (d >= 1.0f) return 0.0f;
-    nq = xyz(ld4(a.normal, q));
-    wq = world(pixel_u(qx, W), pixel_u(qy, a.F.size[1]), d, a.F.inv_vp);
-    cq = xyz(ld4(a.color, q));
-    rm = ld2(a.params, q);
-  }
+// The weight of geometry_weight for a tap of normal nq, position wq,
+// colour cq and material rm.
+template <bool FAST>
+__device__ __forceinline__ float tap_weight(const Guide& g, v3 nq, v3 wq, v3 cq, float2 rm, float stride,
+                                            float inv_tol) {
   float wn = pow64(fmaxf(dot(g.normal, nq), 0.0f));
   float plane = fabsf(dot(wq - g.position, g.normal));
   float wz;
@@ -446,6 +430,27 @@ __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Gui
   v3 dc = g.color - cq;
   float wc = __expf(-wsqrt<FAST>(dot(dc, dc)) * 8.0f - wsqrt<FAST>(dm0 * dm0 + dm1 * dm1) * 8.0f);
   return wn * wz * wc;
+}
+// Tap weight from a guide record (records of pixels without geometry weigh 0).
+template <bool FAST>
+__device__ __forceinline__ float guide_tap_weight(const Guide& g, const GuidePx& r, float stride, float inv_tol) {
+  if (!r.valid) return 0.0f;
+  return tap_weight<FAST>(g, v3{(float)r.nx, (float)r.ny, (float)r.nz}, v3{r.px, r.py, r.pz},
+                          v3{(float)r.cr, (float)r.cg, (float)r.cb}, make_float2((float)r.m0, (float)r.m1), stride,
+                          inv_tol);
+}
+template <class P, class C, bool FAST = false>
+__device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Guide& g, int32_t qx, int32_t qy,
+                                                 float stride, float inv_tol, C& cnt) {
+  cnt.add(C_TAP, 1);
+  if (!inside(a, qx, qy)) return 0.0f;
+  const uint32_t W = a.F.size[0];
+  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+  if (a.guide) return guide_tap_weight<FAST>(g, ld_guide(a.guide, q), stride, inv_tol);
+  float d = a.depth[q];
+  if (d >= 1.0f) return 0.0f;
+  return tap_weight<FAST>(g, xyz(ld4(a.normal, q)), world(pixel_u(qx, W), pixel_u(qy, a.F.size[1]), d, a.F.inv_vp),
+                          xyz(ld4(a.color, q)), ld2(a.params, q), stride, inv_tol);
 }
 
 // svgf_temporal — svgf.wgsl:45-76.
@@ -586,6 +591,59 @@ __device__ __forceinline__ float4 svgf_atrous(const SvgfArgs<P>& a, int32_t x, i
       float4 smp = ld4(a.input, (uint32_t)qy * W + (uint32_t)qx);
       float dl = fabsf(luminance(xyz(smp)) - lc);
       float w = bspline(i) * bspline(j) * geom * __expf(FAST ? -dl * inv_phi : -dl / phi);
+      sum = sum + xyz(smp) * w;
+      v += smp.w * smp.w * w * w;
+      total += w;
+    }
+  }
+  float t = fmaxf(total, 0.00001f);
+  return make_float4(sum.x / t, sum.y / t, sum.z / t, fminf(sqrtf(v) / t, 65000.0f));
+}
+
+// svgf_atrous (FAST form) for a pixel whose tap neighbourhood is staged:
+// `sg`/`sr` hold the guide records (two 16-byte halves; valid = 0 outside the
+// image) and input radiance of an n x n patch of the stride-`step` lattice
+// around the pixel, which sits at patch coordinates (li, lj).  `var_from_patch`
+// (unit stride) reads the 3x3 variance neighbourhood from the patch as well;
+// otherwise it is loaded from `a.input`.  Same arithmetic, same order.
+template <class P, class C>
+__device__ __forceinline__ float4 svgf_atrous_staged(const SvgfArgs<P>& a, int32_t x, int32_t y, const uint4 (*sg)[2],
+                                                     const typename P::R4* sr, int32_t li, int32_t lj, int32_t n,
+                                                     bool var_from_patch, C& cnt) {
+  const uint32_t W = a.F.size[0], H = a.F.size[1];
+  const int32_t e0 = lj * n + li;
+  float4 c = ld4(sr, (uint32_t)e0);
+  const GuidePx self = unpack_guide(sg[e0][0], sg[e0][1]);
+  if (!self.valid) return c;
+  float variance = 0.f, guide_weight = 0.f;
+  const Guide g = guide_of(a.F, self);
+  for (int32_t j = -1; j <= 1; ++j) {
+    for (int32_t i = -1; i <= 1; ++i) {
+      int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+      int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+      float w = (i == 0 ? 2.0f : 1.0f) * (j == 0 ? 2.0f : 1.0f);
+      float sv = var_from_patch ? ld4(sr, (uint32_t)((lj + qy - y) * n + li + qx - x)).w
+                                : ld4(a.input, (uint32_t)qy * W + (uint32_t)qx).w;
+      variance += sv * sv * w;
+      guide_weight += w;
+    }
+  }
+  float phi = 4.0f * sqrtf(variance / fmaxf(guide_weight, 0.00001f)) + 0.0001f;
+  float lc = luminance(xyz(c));
+  v3 sum = v3{0.f, 0.f, 0.f};
+  float total = 0.f, v = 0.f;
+  const int32_t step = (int32_t)a.stride;
+  const float inv_tol = 1.0f / tap_tolerance(g, (float)step);
+  const float inv_phi = 1.0f / phi;
+  for (int32_t j = -2; j <= 2; ++j) {
+    for (int32_t i = -2; i <= 2; ++i) {
+      cnt.add(C_TAP, 1);
+      const int32_t e = (lj + j) * n + li + i;
+      float geom = guide_tap_weight<true>(g, unpack_guide(sg[e][0], sg[e][1]), (float)step, inv_tol);
+      if (geom <= 0.0f) continue;
+      float4 smp = ld4(sr, (uint32_t)e);
+      float dl = fabsf(luminance(xyz(smp)) - lc);
+      float w = bspline(i) * bspline(j) * geom * __expf(-dl * inv_phi);
       sum = sum + xyz(smp) * w;
       v += smp.w * smp.w * w * w;
       total += w;
