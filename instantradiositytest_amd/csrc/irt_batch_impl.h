@@ -92,6 +92,48 @@ __global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict
   }
 }
 
+// Batched a-trous iteration `iter` with LDS-staged taps: tile entries are
+// 16s x 16s pixel patches (s = 2^iter) of a view, blockIdx.y the lattice
+// coset; the body is k_svgf_atrous_lds's.
+template <class P>
+__global__ __launch_bounds__(256) void k_batch_atrous(const BatchView<P>* __restrict__ views,
+                                                     const uint32_t* __restrict__ tiles, uint32_t iter) {
+  __shared__ uint4 s_g[AT_SIDE * AT_SIDE][2];
+  __shared__ typename P::R4 s_r[AT_SIDE * AT_SIDE];
+  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+  const BatchView<P>& bv = views[e >> 24];
+  if (iter >= bv.iterations) return;  // this view has finished its a-trous iterations
+  const SvgfArgs<P>& a = bv.sa[iter];
+  const int32_t W = (int32_t)bv.W, H = (int32_t)bv.H, st = (int32_t)a.stride;
+  const int32_t cx = (int32_t)blockIdx.y % st, cy = (int32_t)blockIdx.y / st;
+  const int32_t x0 = (int32_t)((e >> 12) & 0xfffu) * AT_TILE * st + cx;
+  const int32_t y0 = (int32_t)(e & 0xfffu) * AT_TILE * st + cy;
+  for (int32_t k = (int32_t)threadIdx.x; k < AT_SIDE * AT_SIDE; k += 256) {
+    const int32_t gx = x0 + (k % AT_SIDE - AT_HALO) * st, gy = y0 + (k / AT_SIDE - AT_HALO) * st;
+    if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+      const uint32_t q = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+      const uint4* gp = reinterpret_cast<const uint4*>(a.guide + q);
+      s_g[k][0] = gp[0];
+      s_g[k][1] = gp[1];
+      s_r[k] = a.input[q];
+    } else {
+      s_g[k][0] = make_uint4(0u, 0u, 0u, 0u);
+      s_g[k][1] = make_uint4(0u, 0u, 0u, 0u);
+      s_r[k] = typename P::R4{};
+    }
+  }
+  __syncthreads();
+  const int32_t i = (int32_t)threadIdx.x % AT_TILE, j = (int32_t)threadIdx.x / AT_TILE;
+  const int32_t x = x0 + i * st, y = y0 + j * st;
+  Count<false> cnt;
+  if (x < W && y < H) {
+    const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+    const float4 r = svgf_atrous_staged<P>(a, x, y, s_g, s_r, i + AT_HALO, j + AT_HALO, AT_SIDE, st == 1, cnt);
+    st_r16(a.out, id, r);
+    if (a.out2) st_r16(a.out2, id, r);
+  }
+}
+
 __global__ __launch_bounds__(64) void k_zero_queues(const MarchView* __restrict__ views, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n * 16u) views[i / 16u].queue[i % 16u] = 0u;
@@ -148,6 +190,8 @@ struct irt_batch {
   int tflip = 0;
   uint32_t tile_off[3] = {0, 0, 0};
   uint32_t n_tiles[3] = {0, 0, 0};
+  uint32_t at_off[5] = {0, 0, 0, 0, 0};  // a-trous patch tables, iteration i: 16*2^i pixel patches
+  uint32_t at_n[5] = {0, 0, 0, 0, 0};
   uint32_t shape[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t max_iterations = 0;
   uint32_t march_blocks = 0;
@@ -300,7 +344,12 @@ int batch_sequence(irt_batch* b) {
     bmark(b, BK_SVGF_V, 1);
   }
   bmark(b, BK_SVGF_A, 0);
-  for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) rc = launch_kind<BK_SVGF_A, P>(b, i);
+  for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) {
+    if (b->at_n[i] == 0) continue;
+    hipLaunchKernelGGL((k_batch_atrous<P>), dim3(b->at_n[i], 1u << (2 * i)), dim3(256), 0, b->stream,
+                       (const BatchView<P>*)b->d_views, b->d_tiles + b->at_off[i], i);
+    BHIP(b, hipGetLastError());
+  }
   bmark(b, BK_SVGF_A, 1);
   if (!rc) {
     bmark(b, BK_END, 0);
@@ -411,15 +460,24 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
     }
     b->cap_views = cap;
   }
-  // tile tables: shape s occupies [tile_off[s], tile_off[s] + n_tiles[s])
+  // tile tables: shape s occupies [tile_off[s], tile_off[s] + n_tiles[s]),
+  // then the a-trous patch tables of iterations 0..4
   size_t total = 0;
-  uint32_t n[3];
+  uint32_t n[3], na[5];
   for (int s = 0; s < 3; ++s) {
     const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
     size_t k = 0;
     for (uint32_t v = 0; v < count; ++v) k += (size_t)((views[v]->W + tw - 1) / tw) * ((views[v]->H + th - 1) / th);
     if (k > 0x7fffffffull) return b->fail(IRT_ERR_LIMIT, "too many tiles in one batch");
     n[s] = (uint32_t)k;
+    total += k;
+  }
+  for (uint32_t i = 0; i < 5; ++i) {
+    const uint32_t span = (uint32_t)AT_TILE << i;
+    size_t k = 0;
+    for (uint32_t v = 0; v < count; ++v)
+      if (i < views[v]->s.svgf_iterations) k += (size_t)((views[v]->W + span - 1) / span) * ((views[v]->H + span - 1) / span);
+    na[i] = (uint32_t)k;
     total += k;
   }
   const int f = b->tflip;
@@ -448,6 +506,17 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
         for (uint32_t tx = 0; tx < nx; ++tx) h[k++] = (v << 24) | (tx << 12) | ty;
     }
     b->n_tiles[s] = n[s];
+  }
+  for (uint32_t i = 0; i < 5; ++i) {
+    b->at_off[i] = (uint32_t)k;
+    const uint32_t span = (uint32_t)AT_TILE << i;
+    for (uint32_t v = 0; v < count; ++v) {
+      if (i >= views[v]->s.svgf_iterations) continue;
+      const uint32_t nx = (views[v]->W + span - 1) / span, ny = (views[v]->H + span - 1) / span;
+      for (uint32_t ty = 0; ty < ny; ++ty)
+        for (uint32_t tx = 0; tx < nx; ++tx) h[k++] = (v << 24) | (tx << 12) | ty;
+    }
+    b->at_n[i] = na[i];
   }
   BHIP(b, hipMemcpyAsync(b->d_tiles, h, total * 4, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipEventRecord(b->tiles_staged[f], b->stream));
