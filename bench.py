@@ -108,6 +108,8 @@ def run_single(cs, settings, staged, host_frames, precision: str, schedule: str,
     r = Renderer(cs.scene, settings, precision=precision, schedule=schedule)
     L, ctx = r.L, r.ctx
     stream = L.irt_stream(ctx)
+    # a frame's last work: SVGF's stream under the fused schedule
+    done_stream = L.irt_stream_svgf(ctx) if schedule == "fused" else stream
     gen, nl, no = cs.scene.generation, len(cs.scene.lights), len(cs.scene.objects)
 
     def submit(fi: int):
@@ -126,7 +128,7 @@ def run_single(cs, settings, staged, host_frames, precision: str, schedule: str,
     for k in range(steps):
         starts[k].record(stream)
         submit(warmup + k)
-        ends[k].record(stream)
+        ends[k].record(done_stream)
     r.sync()
     wall = time.perf_counter() - t0
     frame_ms = np.array([starts[k].elapsed_ms(ends[k]) for k in range(steps)])

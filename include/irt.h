@@ -20,8 +20,10 @@
  *   - Calls on one context are serialised on the context's HIP stream and are
  *     asynchronous unless stated; irt_read* and irt_sync block.  Different
  *     contexts are independent (one per view / session).
- *   - No pointer passed in is retained after a call returns, except device
- *     pointers handed to irt_set_gbuffer_device (read during that call only).
+ *   - No pointer passed in is retained after a call returns.  Device
+ *     pointers handed to irt_set_gbuffer_device / irt_set_vpls_device are
+ *     read by work the call enqueues: keep them valid until that work is
+ *     done (irt_sync, or the frame's completion).
  */
 #ifndef IRT_H
 #define IRT_H
@@ -283,8 +285,12 @@ int irt_write(irt_ctx* ctx, uint32_t buffer, const void* src, size_t bytes);
 /* Block until every call enqueued on the context has finished. */
 int irt_sync(irt_ctx* ctx);
 
-/* HIP stream of the context (hipStream_t as void*). */
+/* HIP streams of the context (hipStream_t as void*): irt_stream carries the
+ * ingest, lighting and read-backs; under IRT_SCHEDULE_FUSED, SVGF runs on
+ * irt_stream_svgf after everything irt_stream has queued, so it overlaps the
+ * next frame's lighting (a frame's last work is on irt_stream_svgf). */
 void* irt_stream(irt_ctx* ctx);
+void* irt_stream_svgf(irt_ctx* ctx);
 
 /* GPU time of every pass recorded since timing was enabled or last read
  * (read_gpu_time, renderer.rs:2066-2108), one entry per launch in launch

@@ -67,6 +67,7 @@ def irt() -> ctypes.CDLL:
         "irt_write": (c_int, [P, u32, P, c_size_t]),
         "irt_sync": (c_int, [P]),
         "irt_stream": (c_void_p, [P]),
+        "irt_stream_svgf": (c_void_p, [P]),
         "irt_enable_timing": (c_int, [P, c_int]),
         "irt_pass_times": (c_int, [P, P, POINTER(c_float), c_int]),
         "irt_enable_counters": (c_int, [P, c_int]),

@@ -65,7 +65,11 @@ struct irt_ctx {
   uint32_t precision = IRT_PRECISION_FP32;
   uint32_t schedule = IRT_SCHEDULE_PASSES;
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // ingest, lighting, read-back
+  hipStream_t svgf_stream = nullptr;  // SVGF of the fused schedule (overlaps the next frame's lighting)
+  hipEvent_t ev_lit = nullptr;        // lighting-stream progress the SVGF stream waits for
+  hipEvent_t ev_frame[2] = {nullptr, nullptr};  // SVGF work of frames, by frame parity
+  uint64_t frames_ended = 0;
   uint32_t W = 0, H = 0, V = 0, slots = 0;
   size_t npx = 0;
   // scene
@@ -84,7 +88,18 @@ struct irt_ctx {
   bool have_frame = false;
   std::vector<irt_mesh> host_meshes;     // host copies for launch-time validation
   uint32_t max_mesh_ref = 0;             // largest mesh index referenced by an object
-  // G-buffer (internal formats) and history
+  // G-buffer slots (internal formats).  The current and previous frames'
+  // G-buffers are slots gb_cur / gb_prev (the same slot right after a frame
+  // ends, as when the reference copies the G-buffer); a write into the
+  // current G-buffer first moves it to the slot that is neither gb_prev nor
+  // gb_pp, the one no SVGF still in flight reads.  The pointers below are
+  // the current bindings (bind_slots).
+  float* gb_depth[3] = {nullptr, nullptr, nullptr};
+  void* gb_normal[3] = {nullptr, nullptr, nullptr};
+  void* gb_color[3] = {nullptr, nullptr, nullptr};
+  void* gb_params[3] = {nullptr, nullptr, nullptr};
+  void* gb_motion[3] = {nullptr, nullptr, nullptr};
+  int gb_cur = 0, gb_prev = 0, gb_pp = 0;
   float* depth = nullptr;
   void* normal = nullptr;
   void* color = nullptr;
@@ -97,12 +112,16 @@ struct irt_ctx {
   irt_reservoir* indirect[3] = {nullptr, nullptr, nullptr};
   irt_vpl* ring = nullptr;
   void* direct_lighting = nullptr;
+  void* lit_slot[2] = {nullptr, nullptr};  // noisy radiance, alternating per frame
+  int lit_cur = 0;
   void* lighting = nullptr;
   void* temporal = nullptr;
   void* scratch[2] = {nullptr, nullptr};
   void* feedback[2] = {nullptr, nullptr};
   void* history = nullptr;
   float4* moments[2] = {nullptr, nullptr};
+  GuidePx* guide_slot[2] = {nullptr, nullptr};  // alternating per rebuild
+  int guide_cur = 0;
   GuidePx* guide = nullptr;  // per-pixel guide records of the current G-buffer (fused schedule)
   bool guide_dirty = true;
   // candidate grid over the scene (fused schedule)
@@ -190,6 +209,73 @@ int ensure_array(irt_ctx* c, T** p, uint32_t* cap, uint32_t count) {
 
 bool mixed(const irt_ctx* c) { return c->precision == IRT_PRECISION_MIXED; }
 
+// Current bindings of the slot buffers.
+void bind_slots(irt_ctx* c) {
+  c->depth = c->gb_depth[c->gb_cur];
+  c->normal = c->gb_normal[c->gb_cur];
+  c->color = c->gb_color[c->gb_cur];
+  c->params = c->gb_params[c->gb_cur];
+  c->motion = c->gb_motion[c->gb_cur];
+  c->prev_depth = c->gb_depth[c->gb_prev];
+  c->prev_normal = c->gb_normal[c->gb_prev];
+  c->lighting = c->lit_slot[c->lit_cur];
+  c->guide = c->guide_slot[c->guide_cur];
+}
+
+// Before the lighting stream writes a slot an SVGF still in flight may read
+// (the G-buffer slot of two frames back, the other guide and noisy-radiance
+// slots): wait for the SVGF work of the frame before the previous one.
+int wait_for_slots(irt_ctx* c) {
+  if (c->frames_ended >= 2) HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_frame[c->frames_ended % 2], 0));
+  return IRT_OK;
+}
+
+// Makes the current G-buffer a slot of its own (not the previous frame's
+// and not the one before), copying the planes across when `keep` is set.
+int make_current_writable(irt_ctx* c, bool keep) {
+  if (c->gb_cur != c->gb_prev && c->gb_cur != c->gb_pp) return IRT_OK;
+  int w = 0;
+  while (w == c->gb_prev || w == c->gb_pp) ++w;
+  int rc = wait_for_slots(c);
+  if (rc) return rc;
+  if (keep) {
+    const size_t n = c->npx, n4 = c->precision == IRT_PRECISION_MIXED ? 8 : 16, p2 = n4 / 2;
+    const int f = c->gb_cur;
+    HIP_CHECK(c, hipMemcpyAsync(c->gb_depth[w], c->gb_depth[f], n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->gb_normal[w], c->gb_normal[f], n * n4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->gb_color[w], c->gb_color[f], n * n4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->gb_params[w], c->gb_params[f], n * p2, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(c->gb_motion[w], c->gb_motion[f], n * p2, hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->gb_cur = w;
+  bind_slots(c);
+  return IRT_OK;
+}
+
+// The end of a frame (renderer.rs:1946-1960 copies the G-buffer into the
+// previous-frame textures): the current slot becomes the previous one.
+int end_frame_slots(irt_ctx* c) {
+  HIP_CHECK(c, hipEventRecord(c->ev_frame[c->frames_ended % 2], c->svgf_stream));
+  c->frames_ended++;
+  c->gb_pp = c->gb_prev;
+  c->gb_prev = c->gb_cur;
+  bind_slots(c);
+  return IRT_OK;
+}
+
+// Orders `to` after everything enqueued so far on `from`.
+int join(irt_ctx* c, hipStream_t from, hipStream_t to) {
+  HIP_CHECK(c, hipEventRecord(c->ev_lit, from));
+  HIP_CHECK(c, hipStreamWaitEvent(to, c->ev_lit, 0));
+  return IRT_OK;
+}
+
+int sync_all(irt_ctx* c) {
+  HIP_CHECK(c, hipStreamSynchronize(c->svgf_stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+
 uint32_t compute_units(int device) {
   static int cached_device = -1;
   static uint32_t cached = 0;
@@ -211,13 +297,13 @@ std::vector<std::pair<void**, size_t>> view_alloc_plan(irt_ctx* c, bool m) {
   const size_t n4 = m ? 8 : 16, p2 = m ? 4 : 8, r4 = m ? 8 : 16, l4 = m ? 8 : 16;
   std::vector<std::pair<void**, size_t>> v;
   auto A = [&](void* slot, size_t bytes) { v.emplace_back((void**)slot, bytes); };
-  A(&c->depth, n * 4);
-  A(&c->normal, n * n4);
-  A(&c->color, n * n4);
-  A(&c->params, n * p2);
-  A(&c->motion, n * p2);
-  A(&c->prev_depth, n * 4);
-  A(&c->prev_normal, n * n4);
+  for (int k = 0; k < 3; ++k) {
+    A(&c->gb_depth[k], n * 4);
+    A(&c->gb_normal[k], n * n4);
+    A(&c->gb_color[k], n * n4);
+    A(&c->gb_params[k], n * p2);
+    A(&c->gb_motion[k], n * p2);
+  }
   if (!m) {
     A(&c->stage[0], n * 8);
     A(&c->stage[1], n * 8);
@@ -230,7 +316,8 @@ std::vector<std::pair<void**, size_t>> view_alloc_plan(irt_ctx* c, bool m) {
   }
   A(&c->ring, (size_t)c->slots * c->V * sizeof(irt_vpl));
   A(&c->direct_lighting, n * l4);
-  A(&c->lighting, n * r4);
+  A(&c->lit_slot[0], n * r4);
+  A(&c->lit_slot[1], n * r4);
   A(&c->temporal, n * r4);
   A(&c->scratch[0], n * r4);
   A(&c->scratch[1], n * r4);
@@ -240,7 +327,8 @@ std::vector<std::pair<void**, size_t>> view_alloc_plan(irt_ctx* c, bool m) {
   A(&c->moments[0], n * 16);
   A(&c->moments[1], n * 16);
   A(&c->counters, 8 * sizeof(unsigned long long));
-  A(&c->guide, n * sizeof(GuidePx));
+  A(&c->guide_slot[0], n * sizeof(GuidePx));
+  A(&c->guide_slot[1], n * sizeof(GuidePx));
   A(&c->resolve_rec, n * sizeof(ResolveRec));
   A(&c->rays, 2 * n * sizeof(RayReq));
   A(&c->vis, 2 * n * sizeof(float));
@@ -447,6 +535,11 @@ int check_ready(irt_ctx* c) {
 template <class P>
 int ensure_guide(irt_ctx* c) {
   if (!c->guide_dirty) return IRT_OK;
+  // the other slot: an SVGF of the previous frame may still read this one
+  int rc = wait_for_slots(c);
+  if (rc) return rc;
+  c->guide_cur ^= 1;
+  bind_slots(c);
   const uint32_t n = (uint32_t)c->npx;
   hipLaunchKernelGGL((k_guide<P>), dim3((n + 255) / 256), dim3(256), 0, c->stream, c->frame, c->depth,
                      (const typename P::N4*)c->normal, (const typename P::N4*)c->color,
@@ -519,6 +612,12 @@ template <class P>
 int run_lighting(irt_ctx* c) {
   int rc = check_ready(c);
   if (rc) return rc;
+  // this frame's noisy radiance goes to the other slot (the previous frame's
+  // SVGF may still read the current one)
+  rc = wait_for_slots(c);
+  if (rc) return rc;
+  c->lit_cur ^= 1;
+  bind_slots(c);
   if (c->schedule == IRT_SCHEDULE_FUSED) {
     timer_begin(c, "reservoirs_a");
     rc = ensure_guide<P>(c);
@@ -623,11 +722,38 @@ int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, co
 
 size_t r4_bytes(const irt_ctx* c) { return mixed(c) ? 8 : 16; }
 
+// Points c->stream at the SVGF stream for its lifetime (launch helpers and
+// timers use c->stream).
+struct StreamSwap {
+  irt_ctx* c;
+  hipStream_t saved;
+  explicit StreamSwap(irt_ctx* ctx) : c(ctx), saved(ctx->stream) { c->stream = c->svgf_stream; }
+  ~StreamSwap() { c->stream = saved; }
+};
+
+template <class P>
+int run_svgf_on(irt_ctx* c, uint32_t iterations, uint32_t debug);
+
 // Svgf::encode — svgf.rs:140-261.
 template <class P>
 int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
   if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
   if (iterations == 0 || iterations > 5) return c->fail(IRT_ERR_INVALID_ARGUMENT, "SVGF iterations must be 1..5");
+  if (c->schedule == IRT_SCHEDULE_FUSED) {
+    // The fused schedule runs SVGF on its own stream, after everything the
+    // lighting stream has queued, so it overlaps the next frame's lighting.
+    int rc = ensure_guide<P>(c);
+    if (!rc) rc = join(c, c->stream, c->svgf_stream);
+    if (rc) return rc;
+    StreamSwap swap(c);
+    return run_svgf_on<P>(c, iterations, debug);
+  }
+  return run_svgf_on<P>(c, iterations, debug);
+}
+
+// The SVGF passes on c->stream (the SVGF stream while a StreamSwap holds).
+template <class P>
+int run_svgf_on(irt_ctx* c, uint32_t iterations, uint32_t debug) {
   const uint32_t cur = c->frame.size[2] % 2u;
   const size_t rb = r4_bytes(c) * c->npx;
   const bool fused = c->schedule == IRT_SCHEDULE_FUSED;
@@ -643,8 +769,6 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
     // a-trous stores.
     static const char* atrous_names[5] = {"svgf_atrous0", "svgf_atrous1", "svgf_atrous2", "svgf_atrous3",
                                           "svgf_atrous4"};
-    rc = ensure_guide<P>(c);
-    if (rc) return rc;
     const GuidePx* guide = c->guide;
     timer_begin(c, "svgf_temporal");
     rc = svgf_temporal_pass<P>(c, guide);
@@ -692,6 +816,8 @@ int run_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
 
 template <class P>
 int run_pass(irt_ctx* c, uint32_t pass, uint32_t arg) {
+  int jrc = join(c, c->svgf_stream, c->stream);  // after any SVGF still in flight
+  if (jrc) return jrc;
   if (pass < 8) {
     int rc = check_ready(c);
     if (rc) return rc;
@@ -769,9 +895,13 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   c->s = *s;
   c->precision = precision;
   (void)hipGetDevice(&c->device);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    irt_host::set_global_error("hipStreamCreate failed");
-    delete c;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->svgf_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_lit, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_frame[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_frame[1], hipEventDisableTiming) != hipSuccess) {
+    irt_host::set_global_error("hipStreamCreate / hipEventCreate failed");
+    irt_destroy(c);
     return IRT_ERR_DEVICE;
   }
   c->W = s->render_width;
@@ -785,9 +915,10 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   if (!rc) c->march_blocks = compute_units(c->device) * 8u;
   // Depth targets start cleared to 1 (far plane) as the reference's depth pass clears them.
   if (!rc) {
+    bind_slots(c);
     const uint32_t n = (uint32_t)c->npx, one = 0x3f800000u;
-    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->depth, one, n);
-    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->prev_depth, one, n);
+    for (int k = 0; k < 3; ++k)
+      hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->gb_depth[k], one, n);
     if (hipGetLastError() != hipSuccess) rc = c->fail(IRT_ERR_DEVICE, "initial depth clear failed");
   }
   if (rc) {
@@ -801,6 +932,7 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
 
 void irt_destroy(irt_ctx* c) {
   if (!c) return;
+  if (c->svgf_stream) (void)hipStreamSynchronize(c->svgf_stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& a : c->allocs)
     if (a.first) (void)hipFree(a.first);
@@ -808,6 +940,10 @@ void irt_destroy(irt_ctx* c) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
   }
+  if (c->ev_lit) (void)hipEventDestroy(c->ev_lit);
+  for (hipEvent_t e : c->ev_frame)
+    if (e) (void)hipEventDestroy(e);
+  if (c->svgf_stream) (void)hipStreamDestroy(c->svgf_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1025,12 +1161,18 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
 
 int irt_reset(irt_ctx* c) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  int rc = sync_all(c);
+  if (rc) return rc;
   for (const auto& a : view_alloc_plan(c, mixed(c)))
     HIP_CHECK(c, hipMemsetAsync(*a.first, 0, alloc_size(a.second), c->stream));
   const uint32_t n = (uint32_t)c->npx, one = 0x3f800000u;
-  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->depth, one, n);
-  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->prev_depth, one, n);
+  for (int k = 0; k < 3; ++k)
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->gb_depth[k], one, n);
   HIP_CHECK(c, hipGetLastError());
+  c->gb_cur = c->gb_prev = c->gb_pp = 0;
+  c->lit_cur = c->guide_cur = 0;
+  c->frames_ended = 0;
+  bind_slots(c);
   std::memset(&c->frame, 0, sizeof(c->frame));
   c->have_frame = false;
   c->guide_dirty = true;
@@ -1077,6 +1219,8 @@ int irt_set_gbuffer(irt_ctx* c, const float* depth, const uint16_t* normal, cons
                     const uint16_t* params, const uint16_t* motion) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
+  int wrc = make_current_writable(c, false);
+  if (wrc) return wrc;
   const size_t n = c->npx;
   c->guide_dirty = true;
   HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyHostToDevice, c->stream));
@@ -1102,6 +1246,8 @@ int irt_set_gbuffer_device(irt_ctx* c, const float* depth, const uint16_t* norma
                            const uint16_t* params, const uint16_t* motion) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!depth || !normal || !color || !params || !motion) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane");
+  int wrc = make_current_writable(c, false);
+  if (wrc) return wrc;
   c->guide_dirty = true;
   if (mixed(c)) {
     const size_t n = c->npx;
@@ -1142,10 +1288,7 @@ int irt_svgf(irt_ctx* c, uint32_t iterations, uint32_t debug) {
 
 int irt_end_frame(irt_ctx* c) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
-  const size_t n = c->npx;
-  HIP_CHECK(c, hipMemcpyAsync(c->prev_depth, c->depth, n * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIP_CHECK(c, hipMemcpyAsync(c->prev_normal, c->normal, n * (mixed(c) ? 8 : 16), hipMemcpyDeviceToDevice, c->stream));
-  return IRT_OK;
+  return end_frame_slots(c);
 }
 
 int irt_encode(irt_ctx* c) {
@@ -1171,6 +1314,8 @@ size_t irt_buffer_bytes(const irt_ctx* c, uint32_t b) {
 int irt_read(irt_ctx* c, uint32_t b, void* dst, size_t bytes) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (b >= IRT_BUF_COUNT || !dst) return c->fail(IRT_ERR_INVALID_ARGUMENT, "bad buffer id or null destination");
+  int rrc = sync_all(c);
+  if (rrc) return rrc;
   BufDesc d = describe(c, b);
   if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "read size mismatch");
   std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
@@ -1183,6 +1328,11 @@ int irt_read(irt_ctx* c, uint32_t b, void* dst, size_t bytes) {
 int irt_write(irt_ctx* c, uint32_t b, const void* src, size_t bytes) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (b >= IRT_BUF_COUNT || !src) return c->fail(IRT_ERR_INVALID_ARGUMENT, "bad buffer id or null source");
+  int wrc = sync_all(c);
+  if (!wrc && (b == IRT_BUF_DEPTH || b == IRT_BUF_NORMAL || b == IRT_BUF_COLOR || b == IRT_BUF_PARAMS ||
+               b == IRT_BUF_MOTION || b == IRT_BUF_PREVIOUS_DEPTH || b == IRT_BUF_PREVIOUS_NORMAL))
+    wrc = make_current_writable(c, true);  // the previous frame's planes stay as they were
+  if (wrc) return wrc;
   BufDesc d = describe(c, b);
   if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "write size mismatch");
   std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
@@ -1195,11 +1345,13 @@ int irt_write(irt_ctx* c, uint32_t b, const void* src, size_t bytes) {
 
 int irt_sync(irt_ctx* c) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  int rc = sync_all(c);
+  if (rc) return rc;
   return IRT_OK;
 }
 
 void* irt_stream(irt_ctx* c) { return c ? (void*)c->stream : nullptr; }
+void* irt_stream_svgf(irt_ctx* c) { return c ? (void*)c->svgf_stream : nullptr; }
 
 int irt_enable_timing(irt_ctx* c, int enable) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
@@ -1210,7 +1362,8 @@ int irt_enable_timing(irt_ctx* c, int enable) {
 
 int irt_pass_times(irt_ctx* c, char names[][32], float* ms, int capacity) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  int rc = sync_all(c);
+  if (rc) return rc;
   int n = 0;
   for (size_t i = 0; i < c->n_timers_used && n < capacity; ++i, ++n) {
     float t = 0.f;
@@ -1226,6 +1379,8 @@ int irt_pass_times(irt_ctx* c, char names[][32], float* ms, int capacity) {
 
 int irt_enable_counters(irt_ctx* c, int enable) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  int rc = join(c, c->svgf_stream, c->stream);
+  if (rc) return rc;
   c->counting = enable != 0;
   HIP_CHECK(c, hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
   return IRT_OK;
@@ -1234,6 +1389,8 @@ int irt_enable_counters(irt_ctx* c, int enable) {
 int irt_read_counters(irt_ctx* c, uint64_t counts[IRT_COUNTER_COUNT]) {
   if (!c || !counts) return IRT_ERR_INVALID_ARGUMENT;
   unsigned long long tmp[8];
+  int rc = join(c, c->svgf_stream, c->stream);
+  if (rc) return rc;
   HIP_CHECK(c, hipMemcpyAsync(tmp, c->counters, sizeof(tmp), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < IRT_COUNTER_COUNT; ++i) counts[i] = tmp[i];

@@ -40,8 +40,6 @@ struct BatchView {
   RayReq* reqs;
   float* vis;
   GuidePx* guide;
-  float* prev_depth;
-  typename P::N4* prev_normal;
   uint32_t W, H, iterations, pad;
 };
 
@@ -84,11 +82,6 @@ __global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict
     body_svgf_variance<P, Count<false>, true>(bv.sv, p, cnt);
   } else if constexpr (K == BK_SVGF_A) {
     body_svgf_atrous<P, Count<false>, true>(bv.sa[iter], p, cnt);
-  } else if constexpr (K == BK_END) {
-    if (p.active) {
-      bv.prev_depth[p.id] = bv.la.depth[p.id];
-      bv.prev_normal[p.id] = bv.la.normal[p.id];
-    }
   }
 }
 
@@ -234,8 +227,6 @@ void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
   bv.reqs = c->rays;
   bv.vis = c->vis;
   bv.guide = c->guide;
-  bv.prev_depth = c->prev_depth;
-  bv.prev_normal = (typename P::N4*)c->prev_normal;
   bv.W = c->W;
   bv.H = c->H;
   bv.iterations = c->s.svgf_iterations;
@@ -351,11 +342,9 @@ int batch_sequence(irt_batch* b) {
     BHIP(b, hipGetLastError());
   }
   bmark(b, BK_SVGF_A, 1);
-  if (!rc) {
-    bmark(b, BK_END, 0);
-    rc = launch_kind<BK_END, P>(b, 0);
-    bmark(b, BK_END, 1);
-  }
+  // BK_END: the previous-frame G-buffer is a slot rotation on the host
+  bmark(b, BK_END, 0);
+  bmark(b, BK_END, 1);
   return rc;
 }
 
@@ -378,6 +367,8 @@ int batch_encode(irt_batch* b) {
     if (!c->grid.cells || c->n_objects > LDS_MAX_OBJECTS || c->n_meshes > LDS_MAX_OBJECTS)
       return b->fail(IRT_ERR_LIMIT, "batched views need scenes of at most 64 objects and meshes");
     if (c->frame.flags[2] >= 3u) return b->fail(IRT_ERR_STATE, "batched views render debug modes < 3");
+    rc = join(c, c->svgf_stream, c->stream);
+    if (rc) return b->fail(rc, std::string("view: ") + c->err);
     BHIP(b, hipEventRecord(b->view_ready, c->stream));
     BHIP(b, hipStreamWaitEvent(b->stream, b->view_ready, 0));
   }
@@ -415,7 +406,10 @@ int batch_encode(irt_batch* b) {
   BHIP(b, hipEventRecord(b->done, b->stream));
   for (irt_ctx* c : b->views) {
     BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
+    BHIP(b, hipStreamWaitEvent(c->svgf_stream, b->done, 0));
     c->guide_dirty = false;  // the batch built this frame's guide records
+    int rc2 = end_frame_slots(c);  // the frame's G-buffer becomes the previous one
+    if (rc2) return b->fail(rc2, std::string("view: ") + c->err);
   }
   return IRT_OK;
 }

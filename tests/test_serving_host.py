@@ -180,12 +180,13 @@ def test_state_estimate_counts_every_per_view_buffer():
     mixed = L.irt_estimate_bytes(ctypes.byref(s), 1)
     fp32 = L.irt_estimate_bytes(ctypes.byref(s), 0)
     ring = (hist + 1) * V * 48
-    # reservoirs 6x32, guide 32, resolve record 32, ray requests 2x64, visibility 2x4, moments 2x16,
-    # depth + previous depth 2x4 = 436 B/px in both profiles
-    common = 6 * 32 + 32 + 32 + 128 + 8 + 32 + 8
-    # mixed: normal/color/previous normal 3x8, params/motion 2x4, direct lighting + 7 radiance targets 8x8
-    assert mixed == n * (common + 24 + 8 + 64) + ring + 64 + 64
+    # per pixel, both profiles: reservoirs 6 x 32, guide records 2 slots x 32, resolve record 32, ray
+    # requests 2 x 64, visibility 2 x 4, moments 2 x 16, depth 3 G-buffer slots x 4
+    common = 6 * 32 + 2 * 32 + 32 + 128 + 8 + 32 + 3 * 4
+    # mixed: normal + colour 3 slots x 2 x 8, params + motion 3 slots x 2 x 4, direct lighting +
+    # noisy radiance (2 slots) + temporal + 2 scratch + 2 feedback + history = 9 x 8
+    assert mixed == n * (common + 48 + 24 + 72) + ring + 64 + 64
     # fp32: the same planes at twice the width plus 24 B/px of binary16 staging for ingest
-    assert fp32 == n * (common + 48 + 16 + 128 + 24) + ring + 64 + 64
+    assert fp32 == n * (common + 96 + 48 + 144 + 24) + ring + 64 + 64
     s2 = make_settings(render_width=2 * W, render_height=H, vpl_count=V, restir_max_history=hist)
-    assert L.irt_estimate_bytes(ctypes.byref(s2), 1) - mixed == n * (common + 96)
+    assert L.irt_estimate_bytes(ctypes.byref(s2), 1) - mixed == n * (common + 48 + 24 + 72)
