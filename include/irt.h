@@ -358,7 +358,11 @@ int irt_batch_autotune(irt_batch* batch, uint32_t repeats, uint32_t shapes_out[8
  * (per-frame arguments live in device memory uploaded before the replay). */
 int irt_batch_use_graph(irt_batch* batch, int enable);
 int irt_batch_sync(irt_batch* batch);
+/* The batch's two streams: a step's lighting passes run on the first, its
+ * SVGF passes on the second (where they overlap the next step's lighting);
+ * a step's last work is on the second. */
 void* irt_batch_stream(irt_batch* batch);
+void* irt_batch_stream_svgf(irt_batch* batch);
 /* One encode with per-pass GPU times (ms[0..7] by pass kind, ms[8] march). */
 int irt_batch_encode_timed(irt_batch* batch, float ms[9]);
 int irt_batch_pass_times(irt_batch* batch, float ms[9]);

@@ -99,6 +99,7 @@ def irt() -> ctypes.CDLL:
         "irt_batch_use_graph": (c_int, [P, c_int]),
         "irt_batch_sync": (c_int, [P]),
         "irt_batch_stream": (c_void_p, [P]),
+        "irt_batch_stream_svgf": (c_void_p, [P]),
         "irt_batch_encode_timed": (c_int, [P, POINTER(c_float)]),
         "irt_batch_pass_times": (c_int, [P, POINTER(c_float)]),
         "irt_host_is_box_geometry": (c_int, [P, u32, u32, P, u32, P, P]),

@@ -163,13 +163,19 @@ struct irt_batch {
   std::vector<irt_ctx*> views;
   uint32_t precision = 0;
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // lighting passes
+  hipStream_t svgf_stream = nullptr;  // SVGF passes (overlap the next step's lighting)
   size_t view_size = 0;       // sizeof(BatchView<P>) of the precision profile
   uint32_t cap_views = 0;
-  void* d_views = nullptr;
+  // device arguments by step parity: a step's SVGF still reads its own while
+  // the next step uploads
+  void* d_views[2] = {nullptr, nullptr};
   void* h_views[2] = {nullptr, nullptr};
-  MarchView* d_march = nullptr;
+  MarchView* d_march[2] = {nullptr, nullptr};
   MarchView* h_march[2] = {nullptr, nullptr};
+  uint64_t step = 0;
+  hipEvent_t ev_lit = nullptr;                   // a step's lighting done
+  hipEvent_t ev_svgf[2] = {nullptr, nullptr};    // a step's SVGF done, by step parity
   hipEvent_t staged[2] = {nullptr, nullptr};
   bool staged_used[2] = {false, false};
   int flip = 0;
@@ -189,7 +195,8 @@ struct irt_batch {
   uint32_t max_iterations = 0;
   uint32_t march_blocks = 0;
   bool use_graph = false;
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec_l[2] = {nullptr, nullptr};  // lighting launches, by step parity
+  hipGraphExec_t exec_s[2] = {nullptr, nullptr};  // SVGF launches, by step parity
   hipEvent_t done = nullptr;
   hipEvent_t view_ready = nullptr;
   bool timing = false;
@@ -276,82 +283,112 @@ void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
 }
 
 template <int K, class P>
-int launch_kind(irt_batch* b, uint32_t iter) {
+int launch_kind(irt_batch* b, int q, hipStream_t st, uint32_t iter) {
   const uint32_t sh = b->shape[K];
-  const BatchView<P>* views = (const BatchView<P>*)b->d_views;
+  const BatchView<P>* views = (const BatchView<P>*)b->d_views[q];
   const uint32_t n = b->n_tiles[sh];
   const uint32_t* tiles = b->d_tiles + b->tile_off[sh];
   if (n == 0) return IRT_OK;
   if (sh == 0)
-    hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, b->stream, views, tiles, iter);
+    hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, st, views, tiles, iter);
   else if (sh == 1)
-    hipLaunchKernelGGL((k_batch<K, P, 16, 8>), dim3(n), dim3(128), 0, b->stream, views, tiles, iter);
+    hipLaunchKernelGGL((k_batch<K, P, 16, 8>), dim3(n), dim3(128), 0, st, views, tiles, iter);
   else
-    hipLaunchKernelGGL((k_batch<K, P, 16, 16>), dim3(n), dim3(256), 0, b->stream, views, tiles, iter);
+    hipLaunchKernelGGL((k_batch<K, P, 16, 16>), dim3(n), dim3(256), 0, st, views, tiles, iter);
   BHIP(b, hipGetLastError());
   return IRT_OK;
 }
 
-void bmark(irt_batch* b, int slot, int end) {
-  if (b->timing) (void)hipEventRecord(b->tev[slot][end], b->stream);
+void bmark(irt_batch* b, hipStream_t st, int slot, int end) {
+  if (b->timing) (void)hipEventRecord(b->tev[slot][end], st);
 }
 
-// The launch sequence of one batched frame (also what the graph captures).
+// The lighting launches of one batched frame (b->stream; what graph
+// exec_l[q] holds).
 template <class P>
-int batch_sequence(irt_batch* b) {
+int batch_lighting(irt_batch* b, int q) {
   const uint32_t nv = (uint32_t)b->views.size();
+  hipStream_t st = b->stream;
   int rc = IRT_OK;
-  bmark(b, BK_GUIDE, 0);
-  rc = launch_kind<BK_GUIDE, P>(b, 0);
-  bmark(b, BK_GUIDE, 1);
+  bmark(b, st, BK_GUIDE, 0);
+  rc = launch_kind<BK_GUIDE, P>(b, q, st, 0);
+  bmark(b, st, BK_GUIDE, 1);
   if (!rc) {
-    bmark(b, BK_RES_A, 0);
-    rc = launch_kind<BK_RES_A, P>(b, 0);
-    bmark(b, BK_RES_A, 1);
+    bmark(b, st, BK_RES_A, 0);
+    rc = launch_kind<BK_RES_A, P>(b, q, st, 0);
+    bmark(b, st, BK_RES_A, 1);
   }
   if (!rc) {
-    bmark(b, BK_SETUP, 0);
-    hipLaunchKernelGGL(k_zero_queues, dim3((nv * 16 + 63) / 64), dim3(64), 0, b->stream, b->d_march, nv);
-    rc = launch_kind<BK_SETUP, P>(b, 0);
-    bmark(b, BK_SETUP, 1);
+    bmark(b, st, BK_SETUP, 0);
+    hipLaunchKernelGGL(k_zero_queues, dim3((nv * 16 + 63) / 64), dim3(64), 0, st, b->d_march[q], nv);
+    rc = launch_kind<BK_SETUP, P>(b, q, st, 0);
+    bmark(b, st, BK_SETUP, 1);
   }
   if (!rc) {
-    bmark(b, BK_COUNT, 0);  // march
-    hipLaunchKernelGGL(k_march_views, dim3(b->march_blocks), dim3(256), 0, b->stream, b->d_march, nv);
+    bmark(b, st, BK_COUNT, 0);  // march
+    hipLaunchKernelGGL(k_march_views, dim3(b->march_blocks), dim3(256), 0, st, b->d_march[q], nv);
     BHIP(b, hipGetLastError());
-    bmark(b, BK_COUNT, 1);
-    bmark(b, BK_COMBINE, 0);
-    rc = launch_kind<BK_COMBINE, P>(b, 0);
-    bmark(b, BK_COMBINE, 1);
+    bmark(b, st, BK_COUNT, 1);
+    bmark(b, st, BK_COMBINE, 0);
+    rc = launch_kind<BK_COMBINE, P>(b, q, st, 0);
+    bmark(b, st, BK_COMBINE, 1);
   }
-  if (!rc) {
-    bmark(b, BK_SVGF_T, 0);
-    rc = launch_kind<BK_SVGF_T, P>(b, 0);
-    bmark(b, BK_SVGF_T, 1);
-  }
-  if (!rc) {
-    bmark(b, BK_SVGF_V, 0);
-    rc = launch_kind<BK_SVGF_V, P>(b, 0);
-    bmark(b, BK_SVGF_V, 1);
-  }
-  bmark(b, BK_SVGF_A, 0);
-  for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) {
-    if (b->at_n[i] == 0) continue;
-    hipLaunchKernelGGL((k_batch_atrous<P>), dim3(b->at_n[i], 1u << (2 * i)), dim3(256), 0, b->stream,
-                       (const BatchView<P>*)b->d_views, b->d_tiles + b->at_off[i], i);
-    BHIP(b, hipGetLastError());
-  }
-  bmark(b, BK_SVGF_A, 1);
-  // BK_END: the previous-frame G-buffer is a slot rotation on the host
-  bmark(b, BK_END, 0);
-  bmark(b, BK_END, 1);
   return rc;
 }
 
+// The SVGF launches of one batched frame (b->svgf_stream; graph exec_s[q]).
+template <class P>
+int batch_svgf(irt_batch* b, int q) {
+  hipStream_t st = b->svgf_stream;
+  int rc = IRT_OK;
+  bmark(b, st, BK_SVGF_T, 0);
+  rc = launch_kind<BK_SVGF_T, P>(b, q, st, 0);
+  bmark(b, st, BK_SVGF_T, 1);
+  if (!rc) {
+    bmark(b, st, BK_SVGF_V, 0);
+    rc = launch_kind<BK_SVGF_V, P>(b, q, st, 0);
+    bmark(b, st, BK_SVGF_V, 1);
+  }
+  bmark(b, st, BK_SVGF_A, 0);
+  for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) {
+    if (b->at_n[i] == 0) continue;
+    hipLaunchKernelGGL((k_batch_atrous<P>), dim3(b->at_n[i], 1u << (2 * i)), dim3(256), 0, st,
+                       (const BatchView<P>*)b->d_views[q], b->d_tiles + b->at_off[i], i);
+    BHIP(b, hipGetLastError());
+  }
+  bmark(b, st, BK_SVGF_A, 1);
+  // BK_END: the previous-frame G-buffer is a slot rotation on the host
+  bmark(b, st, BK_END, 0);
+  bmark(b, st, BK_END, 1);
+  return rc;
+}
+
+// Captures `launch` on `st` into *exec (first use) and replays it.
+template <class F>
+int replay(irt_batch* b, hipStream_t st, hipGraphExec_t* exec, F launch) {
+  if (!*exec) {
+    hipGraph_t graph = nullptr;
+    BHIP(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = launch();
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  BHIP(b, hipGraphLaunch(*exec, st));
+  return IRT_OK;
+}
+
 void drop_graph(irt_batch* b) {
-  if (b->exec) {
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
+  for (int q = 0; q < 2; ++q) {
+    if (b->exec_l[q]) (void)hipGraphExecDestroy(b->exec_l[q]);
+    if (b->exec_s[q]) (void)hipGraphExecDestroy(b->exec_s[q]);
+    b->exec_l[q] = b->exec_s[q] = nullptr;
   }
 }
 
@@ -368,49 +405,59 @@ int batch_encode(irt_batch* b) {
       return b->fail(IRT_ERR_LIMIT, "batched views need scenes of at most 64 objects and meshes");
     if (c->frame.flags[2] >= 3u) return b->fail(IRT_ERR_STATE, "batched views render debug modes < 3");
     rc = join(c, c->svgf_stream, c->stream);
+    // this step writes the view's other guide and noisy-radiance slots: wait
+    // for the SVGF (batched or not) of the frame before the previous one
+    if (!rc) rc = wait_for_slots(c);
     if (rc) return b->fail(rc, std::string("view: ") + c->err);
     BHIP(b, hipEventRecord(b->view_ready, c->stream));
     BHIP(b, hipStreamWaitEvent(b->stream, b->view_ready, 0));
   }
+  // The argument buffers of this step's parity: the SVGF of two steps back
+  // read them.
+  const int q = (int)(b->step % 2);
+  BHIP(b, hipStreamWaitEvent(b->stream, b->ev_svgf[q], 0));
   // per-frame arguments into the free pinned buffer, then to the device
   const int f = b->flip;
   if (b->staged_used[f]) BHIP(b, hipEventSynchronize(b->staged[f]));
   BatchView<P>* hv = (BatchView<P>*)b->h_views[f];
-  for (uint32_t i = 0; i < nv; ++i) fill_view<P>(b->views[i], hv[i], b->h_march[f][i]);
-  BHIP(b, hipMemcpyAsync(b->d_views, hv, sizeof(BatchView<P>) * nv, hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipMemcpyAsync(b->d_march, b->h_march[f], sizeof(MarchView) * nv, hipMemcpyHostToDevice, b->stream));
+  for (uint32_t i = 0; i < nv; ++i) {
+    irt_ctx* c = b->views[i];
+    c->lit_cur ^= 1;
+    c->guide_cur ^= 1;
+    bind_slots(c);
+    fill_view<P>(c, hv[i], b->h_march[f][i]);
+  }
+  BHIP(b, hipMemcpyAsync(b->d_views[q], hv, sizeof(BatchView<P>) * nv, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync(b->d_march[q], b->h_march[f], sizeof(MarchView) * nv, hipMemcpyHostToDevice, b->stream));
   BHIP(b, hipEventRecord(b->staged[f], b->stream));
   b->staged_used[f] = true;
   b->flip = 1 - f;
   int rc = IRT_OK;
-  if (b->use_graph && !b->timing) {
-    if (!b->exec) {
-      hipGraph_t graph = nullptr;
-      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      rc = batch_sequence<P>(b);
-      hipError_t e = hipStreamEndCapture(b->stream, &graph);
-      if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-      }
-      if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) return b->fail(IRT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    }
-    BHIP(b, hipGraphLaunch(b->exec, b->stream));
-  } else {
-    rc = batch_sequence<P>(b);
-    if (rc) return rc;
-  }
-  BHIP(b, hipEventRecord(b->done, b->stream));
+  const bool graph = b->use_graph && !b->timing;
+  if (graph)
+    rc = replay(b, b->stream, &b->exec_l[q], [&] { return batch_lighting<P>(b, q); });
+  else
+    rc = batch_lighting<P>(b, q);
+  if (rc) return rc;
+  BHIP(b, hipEventRecord(b->ev_lit, b->stream));
+  BHIP(b, hipStreamWaitEvent(b->svgf_stream, b->ev_lit, 0));
+  if (graph)
+    rc = replay(b, b->svgf_stream, &b->exec_s[q], [&] { return batch_svgf<P>(b, q); });
+  else
+    rc = batch_svgf<P>(b, q);
+  if (rc) return rc;
+  BHIP(b, hipEventRecord(b->ev_svgf[q], b->svgf_stream));
+  BHIP(b, hipEventRecord(b->done, b->svgf_stream));
   for (irt_ctx* c : b->views) {
-    BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
+    // the view's next ingest and VPL upload wait for this step's lighting,
+    // its frame end for this step's SVGF
+    BHIP(b, hipStreamWaitEvent(c->stream, b->ev_lit, 0));
     BHIP(b, hipStreamWaitEvent(c->svgf_stream, b->done, 0));
     c->guide_dirty = false;  // the batch built this frame's guide records
     int rc2 = end_frame_slots(c);  // the frame's G-buffer becomes the previous one
     if (rc2) return b->fail(rc2, std::string("view: ") + c->err);
   }
+  b->step++;
   return IRT_OK;
 }
 
@@ -433,11 +480,14 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
     max_it = std::max(max_it, c->s.svgf_iterations);
   }
   if (count > b->cap_views) {
+    BHIP(b, hipStreamSynchronize(b->svgf_stream));
     BHIP(b, hipStreamSynchronize(b->stream));
-    if (b->d_views) BHIP(b, hipFree(b->d_views));
-    if (b->d_march) BHIP(b, hipFree(b->d_march));
-    b->d_views = nullptr;
-    b->d_march = nullptr;
+    for (int q = 0; q < 2; ++q) {
+      if (b->d_views[q]) BHIP(b, hipFree(b->d_views[q]));
+      if (b->d_march[q]) BHIP(b, hipFree(b->d_march[q]));
+      b->d_views[q] = nullptr;
+      b->d_march[q] = nullptr;
+    }
     for (int f = 0; f < 2; ++f) {
       if (b->h_views[f]) BHIP(b, hipHostFree(b->h_views[f]));
       if (b->h_march[f]) BHIP(b, hipHostFree(b->h_march[f]));
@@ -446,8 +496,10 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
       b->staged_used[f] = false;
     }
     const uint32_t cap = std::max(count, std::min<uint32_t>(kMaxBatchViews, 2 * b->cap_views));
-    BHIP(b, hipMalloc(&b->d_views, b->view_size * cap));
-    BHIP(b, hipMalloc((void**)&b->d_march, sizeof(MarchView) * cap));
+    for (int q = 0; q < 2; ++q) {
+      BHIP(b, hipMalloc(&b->d_views[q], b->view_size * cap));
+      BHIP(b, hipMalloc((void**)&b->d_march[q], sizeof(MarchView) * cap));
+    }
     for (int f = 0; f < 2; ++f) {
       BHIP(b, hipHostMalloc(&b->h_views[f], b->view_size * cap, hipHostMallocDefault));
       BHIP(b, hipHostMalloc((void**)&b->h_march[f], sizeof(MarchView) * cap, hipHostMallocDefault));
@@ -483,6 +535,7 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
     b->cap_h_tiles[f] = total;
   }
   if (b->cap_tiles < total) {
+    BHIP(b, hipStreamSynchronize(b->svgf_stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     if (b->d_tiles) BHIP(b, hipFree(b->d_tiles));
     b->d_tiles = nullptr;
@@ -490,6 +543,8 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
     b->cap_tiles = total;
   }
   uint32_t* h = b->h_tiles[f];
+  // the a-trous launches of the last step (SVGF stream) still read the tables
+  BHIP(b, hipStreamWaitEvent(b->stream, b->done, 0));
   size_t k = 0;
   for (int s = 0; s < 3; ++s) {
     b->tile_off[s] = (uint32_t)k;
@@ -551,14 +606,18 @@ int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out) {
     irt_batch_destroy(b);
     return code;
   };
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->svgf_stream, hipStreamNonBlocking) != hipSuccess)
     return bail(IRT_ERR_DEVICE, "hipStreamCreate failed");
   for (int f = 0; f < 2; ++f)
     if (hipEventCreateWithFlags(&b->staged[f], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->tiles_staged[f], hipEventDisableTiming) != hipSuccess)
       return bail(IRT_ERR_DEVICE, "hipEventCreate failed");
   if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->view_ready, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&b->view_ready, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_lit, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_svgf[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_svgf[1], hipEventDisableTiming) != hipSuccess)
     return bail(IRT_ERR_DEVICE, "hipEventCreate failed");
   for (auto& pair : b->tev)
     for (auto& e : pair)
@@ -571,12 +630,14 @@ int irt_batch_create(irt_ctx* const* views, uint32_t count, irt_batch** out) {
 
 void irt_batch_destroy(irt_batch* b) {
   if (!b) return;
+  if (b->svgf_stream) (void)hipStreamSynchronize(b->svgf_stream);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  if (b->d_views) (void)hipFree(b->d_views);
-  if (b->d_march) (void)hipFree(b->d_march);
   if (b->d_tiles) (void)hipFree(b->d_tiles);
   for (int f = 0; f < 2; ++f) {
+    if (b->d_views[f]) (void)hipFree(b->d_views[f]);
+    if (b->d_march[f]) (void)hipFree(b->d_march[f]);
+    if (b->ev_svgf[f]) (void)hipEventDestroy(b->ev_svgf[f]);
     if (b->h_views[f]) (void)hipHostFree(b->h_views[f]);
     if (b->h_march[f]) (void)hipHostFree(b->h_march[f]);
     if (b->h_tiles[f]) (void)hipHostFree(b->h_tiles[f]);
@@ -585,10 +646,12 @@ void irt_batch_destroy(irt_batch* b) {
   }
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->view_ready) (void)hipEventDestroy(b->view_ready);
+  if (b->ev_lit) (void)hipEventDestroy(b->ev_lit);
   for (auto& pair : b->tev)
     for (auto& e : pair)
       if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->svgf_stream) (void)hipStreamDestroy(b->svgf_stream);
   delete b;
 }
 
@@ -623,11 +686,14 @@ int irt_batch_use_graph(irt_batch* b, int enable) {
 
 int irt_batch_sync(irt_batch* b) {
   if (!b) return IRT_ERR_INVALID_ARGUMENT;
+  BHIP(b, hipStreamSynchronize(b->svgf_stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   return IRT_OK;
 }
 
 void* irt_batch_stream(irt_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+void* irt_batch_stream_svgf(irt_batch* b) { return b ? (void*)b->svgf_stream : nullptr; }
 
 int irt_batch_pass_times(irt_batch* b, float ms[9]) {
   if (!b || !ms) return IRT_ERR_INVALID_ARGUMENT;
@@ -642,6 +708,7 @@ int irt_batch_encode_timed(irt_batch* b, float ms[9]) {
   int rc = irt_batch_encode(b);
   b->timing = false;
   if (rc) return rc;
+  BHIP(b, hipStreamSynchronize(b->svgf_stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   for (int k = 0; k <= BK_COUNT; ++k) {
     float t = 0.f;

@@ -340,7 +340,7 @@ int irt_server_step(irt_server* s) {
   if (e != hipSuccess) return s->fail(IRT_ERR_DEVICE, "hipEventRecord failed");
   rc = irt_batch_encode(s->batch);
   if (rc) return s->fail(rc, std::string("batch: ") + s->batch->err);
-  e = hipEventRecord(st.done, s->batch->stream);
+  e = hipEventRecord(st.done, s->batch->svgf_stream);  // a step's last work
   if (e != hipSuccess) return s->fail(IRT_ERR_DEVICE, "hipEventRecord failed");
   st.launch_s = server_clock();
   for (auto& m : st.members) {

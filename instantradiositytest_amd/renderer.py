@@ -337,7 +337,12 @@ class Batch:
         return {k: TILE_SHAPES[out[i]] for i, k in enumerate(BATCH_KINDS)}
 
     def stream(self):
+        """The lighting stream (a step's first work)."""
         return self.L.irt_batch_stream(self.b)
+
+    def stream_svgf(self):
+        """The SVGF stream (a step's last work)."""
+        return self.L.irt_batch_stream_svgf(self.b)
 
     def sync(self):
         self._check(self.L.irt_batch_sync(self.b))
