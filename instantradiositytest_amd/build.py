@@ -45,16 +45,27 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+# The mixed profile's march (irt_march_contracted.hip): multiply-add
+# contraction inside each expression and the hardware's 1-ulp division and
+# square root instead of the exact contract above.
+CONTRACTED_FLAGS = [f for f in HIP_FLAGS
+                    if f not in ("-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt")]
+CONTRACTED_FLAGS += ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+
+
 def build_irt(verbose_asm: bool = False) -> Path:
     """Compile libirt.so (HIP kernels + C ABI + host precompute)."""
     LIB_DIR.mkdir(exist_ok=True)
     out = LIB_DIR / "libirt.so"
     tmp = LIB_DIR / "libirt.so.tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}",
-           str(CSRC / "irt_api.hip"), str(CSRC / "irt_host.cpp"), "-o", str(tmp)]
-    if verbose_asm:
-        cmd.insert(1, "-Rpass-analysis=kernel-resource-usage")
-    _run(cmd)
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_asm else []
+    objs = []
+    for src, flags in (("irt_api.hip", HIP_FLAGS), ("irt_host.cpp", HIP_FLAGS),
+                       ("irt_march_contracted.hip", CONTRACTED_FLAGS)):
+        obj = LIB_DIR / (Path(src).stem + ".o")
+        _run([HIPCC, *flags, *extra, "-c", f"-I{INCLUDE}", f"-I{CSRC}", str(CSRC / src), "-o", str(obj)])
+        objs.append(str(obj))
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

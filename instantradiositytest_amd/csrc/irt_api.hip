@@ -643,12 +643,15 @@ int run_lighting(irt_ctx* c) {
     LAUNCH(c, k_spatial_setup, P, b, c->resolve_rec, c->rays);
     timer_end(c);
     timer_begin(c, "march");
-    if (c->counting)
-      hipLaunchKernelGGL(k_march<true>, dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc, c->frame, c->n_meshes,
-                         c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
+    if (mixed(c))
+      HIP_CHECK(c, launch_march_contracted(c->counting, dim3(c->march_blocks), c->stream, b.sc, c->frame, c->n_meshes,
+                                           c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters));
+    else if (c->counting)
+      hipLaunchKernelGGL((k_march<true, IRT_MARCH_EXACT>), dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc,
+                         c->frame, c->n_meshes, c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
     else
-      hipLaunchKernelGGL(k_march<false>, dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc, c->frame,
-                         c->n_meshes, c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
+      hipLaunchKernelGGL((k_march<false, IRT_MARCH_EXACT>), dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc,
+                         c->frame, c->n_meshes, c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
     HIP_CHECK(c, hipGetLastError());
     timer_end(c);
     timer_begin(c, "resolve_combine");

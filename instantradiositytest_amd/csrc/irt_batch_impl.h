@@ -43,15 +43,6 @@ struct BatchView {
   uint32_t W, H, iterations, pad;
 };
 
-struct MarchView {
-  SceneView sc;
-  irt_frame F;
-  const RayReq* reqs;
-  uint32_t* queue;
-  float* vis;
-  uint32_t n_meshes, n_rays;
-};
-
 template <int K, class P, int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict__ views,
                                                   const uint32_t* __restrict__ tiles, uint32_t iter) {
@@ -130,31 +121,6 @@ __global__ __launch_bounds__(256) void k_batch_atrous(const BatchView<P>* __rest
 __global__ __launch_bounds__(64) void k_zero_queues(const MarchView* __restrict__ views, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n * 16u) views[i / 16u].queue[i % 16u] = 0u;
-}
-
-__global__ __launch_bounds__(256) void k_march_views(const MarchView* __restrict__ views, uint32_t n_views) {
-  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
-  __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
-  __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
-  Count<false> cnt;
-  for (uint32_t v = 0; v < n_views; ++v) {
-    const MarchView& mv = views[v];
-    __syncthreads();  // the previous view's readers are done with the LDS records
-    const uint32_t n_obj = mv.F.counts[3];
-    const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
-    for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
-      reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(mv.sc.objects)[i];
-    const uint32_t words_mesh = mv.n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
-    for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
-      reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(mv.sc.meshes)[i];
-    for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = mv.sc.grid.boxes[i];
-    __syncthreads();
-    SceneView lsc = mv.sc;
-    lsc.objects = s_obj;
-    lsc.meshes = s_mesh;
-    lsc.grid.boxes = s_box;
-    march_queue(lsc, mv.F, mv.reqs, mv.n_rays, mv.queue, mv.vis, cnt);
-  }
 }
 
 }  // namespace
@@ -326,8 +292,12 @@ int batch_lighting(irt_batch* b, int q) {
   }
   if (!rc) {
     bmark(b, st, BK_COUNT, 0);  // march
-    hipLaunchKernelGGL(k_march_views, dim3(b->march_blocks), dim3(256), 0, st, b->d_march[q], nv);
-    BHIP(b, hipGetLastError());
+    if (b->precision == IRT_PRECISION_MIXED) {
+      BHIP(b, launch_march_views_contracted(dim3(b->march_blocks), st, b->d_march[q], nv));
+    } else {
+      hipLaunchKernelGGL((k_march_views<IRT_MARCH_EXACT>), dim3(b->march_blocks), dim3(256), 0, st, b->d_march[q], nv);
+      BHIP(b, hipGetLastError());
+    }
     bmark(b, st, BK_COUNT, 1);
     bmark(b, st, BK_COMBINE, 0);
     rc = launch_kind<BK_COMBINE, P>(b, q, st, 0);

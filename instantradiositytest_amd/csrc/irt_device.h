@@ -349,12 +349,18 @@ __device__ __forceinline__ float signed_box_distance(v3 p, v3 lo, v3 hi) {
   return length(vmaxs(delta, 0.0f)) + fminf(fmaxf(delta.x, fmaxf(delta.y, delta.z)), 0.0f);
 }
 
-// sample_field (manual 8-tap FP32 variant) — common.wgsl:31-38.
+// sample_field (manual 8-tap trilinear) — common.wgsl:31-38.
 __device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_t* origin, v3 q, uint32_t res) {
-  uint32_t cx = min(f2u_sat(floorf(q.x)), res - 2u);
-  uint32_t cy = min(f2u_sat(floorf(q.y)), res - 2u);
-  uint32_t cz = min(f2u_sat(floorf(q.z)), res - 2u);
-  float fx = q.x - (float)cx, fy = q.y - (float)cy, fz = q.z - (float)cz;
+  // cell = min(floor(q), res - 2) with the clamp taken on the float (a
+  // median of three: also 0 below, the saturation of the u32 conversion);
+  // res - 2 and the floors are exact integers, so cell and f = q - cell are
+  // the values of the reference's integer clamp.
+  const float top = (float)(res - 2u);
+  const float ux = __builtin_amdgcn_fmed3f(floorf(q.x), 0.0f, top);
+  const float uy = __builtin_amdgcn_fmed3f(floorf(q.y), 0.0f, top);
+  const float uz = __builtin_amdgcn_fmed3f(floorf(q.z), 0.0f, top);
+  const uint32_t cx = (uint32_t)ux, cy = (uint32_t)uy, cz = (uint32_t)uz;
+  float fx = q.x - ux, fy = q.y - uy, fz = q.z - uz;
   const uint32_t w = sc.atlas_w, h = sc.atlas_h;
   const size_t off = ((size_t)(origin[2] + cz) * h + (origin[1] + cy)) * w + (origin[0] + cx);
   const size_t sy = w, sz = (size_t)w * h;
