@@ -249,10 +249,8 @@ def run_batched(frames: int, warmup: int, steps: int, graph: bool = True) -> dic
     return out
 
 
-def run_serving(seconds: float, warmup_s: float, sessions: int, variants: int, frames: int, seed: int,
-                tile_shapes=None, max_in_flight: int = 2) -> dict:
-    from instantradiositytest_amd.serving import (SERVING_MIX, SceneStream, Server, poisson_arrivals,
-                                                  session_config, simulate)
+def stage_serving_streams(variants: int, frames: int, seed: int):
+    from instantradiositytest_amd.serving import SERVING_MIX, SceneStream, session_config
     rng = np.random.default_rng(seed)
     streams = []
     t0 = time.perf_counter()
@@ -261,6 +259,12 @@ def run_serving(seconds: float, warmup_s: float, sessions: int, variants: int, f
             streams.append(SceneStream(session_config(seed * 100 + len(streams), w, h, frames, rng), "mixed",
                                        frames))
     log(f"[serve] {len(streams)} session streams staged in {time.perf_counter() - t0:.1f}s")
+    return streams, rng
+
+
+def run_serving(streams, rng, seconds: float, warmup_s: float, sessions: int, variants: int, frames: int,
+                tile_shapes=None, max_in_flight: int = 2) -> dict:
+    from instantradiositytest_amd.serving import SERVING_MIX, Server, poisson_arrivals, simulate
     arrivals = poisson_arrivals(40.0, seconds, [variants] * 3, [p for _, _, p in SERVING_MIX], (120, 600), frames,
                                 rng)
     srv = Server(max_sessions=sessions, precision="mixed", budget_bytes=200e9, max_in_flight=max_in_flight,
@@ -289,6 +293,9 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=25.0)
     ap.add_argument("--serve-warmup", type=float, default=5.0)
     ap.add_argument("--sessions", type=int, default=32)
+    ap.add_argument("--serve-sweep", type=lambda v: [int(x) for x in v.split(",") if x], default=[4, 8, 16],
+                    help="resident-session caps of the short serving runs after the main one")
+    ap.add_argument("--sweep-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="", help="comma list of legs: headline,plain,fused_fp32,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
@@ -391,9 +398,18 @@ def main() -> int:
             order = ("guide", "reservoirs_a", "spatial_setup", "resolve_combine", "svgf_temporal", "svgf_variance",
                      "svgf_atrous", "history")
             shapes = [("8x8", "16x8", "16x16").index(line["batched"]["tile_shapes"][k]) for k in order]
-        line["serving"] = run_serving(args.serve_seconds, args.serve_warmup, args.sessions, 2, 24, 0xC5, shapes)
+        streams, rng = stage_serving_streams(2, 24, 0xC5)
+        line["serving"] = run_serving(streams, rng, args.serve_seconds, args.serve_warmup, args.sessions, 2, 24, shapes)
         if "plain" in results:
             line["serving"]["speedup_vs_plain_fp32"] = round(line["serving"]["mpix_s"] / results["plain"]["value"], 3)
+        # the same workload with fewer resident sessions: where 60 Hz demand fits the GPU
+        sweep = {}
+        for n in args.serve_sweep:
+            d = run_serving(streams, rng, args.sweep_seconds, 3.0, n, 2, 24, shapes)
+            sweep[str(n)] = {k: d[k] for k in ("mpix_s", "p50_ms", "p99_ms", "mean_batch", "frames")}
+        if sweep:
+            line["serving"]["by_sessions"] = sweep
+        del streams
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         line["cpu_baseline"] = cpu_baseline(cs, settings, args.cpu_frames, threads)
