@@ -1253,12 +1253,32 @@ int irt_set_gbuffer_device(irt_ctx* c, const float* depth, const uint16_t* norma
   if (wrc) return wrc;
   c->guide_dirty = true;
   if (mixed(c)) {
+    // the reference formats are this profile's storage: plain copies, one
+    // launch for the five planes when every pointer is 16-byte aligned
     const size_t n = c->npx;
-    HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpyAsync(c->normal, normal, n * 8, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpyAsync(c->color, color, n * 8, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpyAsync(c->params, params, n * 4, hipMemcpyDeviceToDevice, c->stream));
-    HIP_CHECK(c, hipMemcpyAsync(c->motion, motion, n * 4, hipMemcpyDeviceToDevice, c->stream));
+    const void* src[5] = {depth, normal, color, params, motion};
+    void* dst[5] = {c->depth, c->normal, c->color, c->params, c->motion};
+    const size_t bytes[5] = {n * 4, n * 8, n * 8, n * 4, n * 4};
+    bool aligned = bytes[1] / 16 <= 0xffffffffull;
+    for (int k = 0; k < 5; ++k) aligned = aligned && ((uintptr_t)src[k] % 16 == 0) && ((uintptr_t)dst[k] % 16 == 0);
+    if (aligned) {
+      PlaneCopy pc;
+      pc.planes = 5;
+      size_t most = 0;
+      for (int k = 0; k < 5; ++k) {
+        pc.src[k] = (const uint4*)src[k];
+        pc.dst[k] = (uint4*)dst[k];
+        pc.units[k] = (uint32_t)(bytes[k] / 16);
+        pc.tail[k] = (uint32_t)(bytes[k] % 16 / 4);
+        most = std::max(most, bytes[k] / 16);
+      }
+      const uint32_t blocks = (uint32_t)std::min<size_t>((most + 255) / 256, 8u * compute_units(c->device));
+      hipLaunchKernelGGL(k_copy_planes, dim3(std::max(blocks, 1u)), dim3(256), 0, c->stream, pc);
+      HIP_CHECK(c, hipGetLastError());
+    } else {
+      for (int k = 0; k < 5; ++k)
+        HIP_CHECK(c, hipMemcpyAsync(dst[k], src[k], bytes[k], hipMemcpyDeviceToDevice, c->stream));
+    }
     return IRT_OK;
   }
   return ingest<ProfileF32>(c, depth, normal, color, params, motion);

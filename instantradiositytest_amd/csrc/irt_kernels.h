@@ -313,6 +313,30 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint3
   if (i < n) p[i] = v;
 }
 
+// Up to five device-to-device plane copies in one launch (the mixed
+// profile's G-buffer ingest: the reference formats are its storage
+// formats).  Each plane is copied in 16-byte units by a grid-stride loop;
+// the host passes 16-byte aligned pointers and sizes in whole units, with
+// any dword tail in `tail` (at most 3 dwords per plane, copied by block 0).
+struct PlaneCopy {
+  const uint4* src[5];
+  uint4* dst[5];
+  uint32_t units[5];
+  uint32_t tail[5];
+  uint32_t planes;
+};
+__global__ __launch_bounds__(256) void k_copy_planes(PlaneCopy pc) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = 0; k < pc.planes; ++k) {
+    const uint4* __restrict__ s = pc.src[k];
+    uint4* __restrict__ d = pc.dst[k];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < pc.units[k]; i += stride) d[i] = s[i];
+    if (blockIdx.x == 0 && threadIdx.x < pc.tail[k])
+      reinterpret_cast<uint32_t*>(d + pc.units[k])[threadIdx.x] =
+          reinterpret_cast<const uint32_t*>(s + pc.units[k])[threadIdx.x];
+  }
+}
+
 // ------------------------------------------------------------------- SVGF
 template <class P, class C, bool FAST>
 __device__ __forceinline__ void body_svgf_temporal(const SvgfArgs<P>& a, const Px& p, C& cnt) {
