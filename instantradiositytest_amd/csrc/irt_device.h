@@ -194,6 +194,11 @@ __device__ __forceinline__ float pixel_u(int32_t x, uint32_t w) { return ((float
 struct Surface {
   v3 p, n, color;
   float rough, metal;
+  // The parts of brdf (common.wgsl:57-74) that depend only on the surface
+  // and the camera, evaluated once per pixel by surface_view() with the
+  // same operations the per-sample evaluation would repeat.
+  v3 wo, f0;
+  float nv, a2, g1v, im;
 };
 
 // pow(x, 5) and pow(x, 64) for x >= 0 as fixed multiply chains: (x^2)^2 * x
@@ -212,21 +217,32 @@ __device__ __forceinline__ float pow64(float x) {
 }
 
 // brdf — common.wgsl:67-74.
-__device__ __forceinline__ v3 brdf(const Surface& s, v3 wo, v3 wi) {
-  float nv = fmaxf(dot(s.n, wo), 0.00001f);
+// The camera-dependent per-pixel terms of brdf for s (wo = normalize(camera
+// - p), n.v, alpha^2, the view Smith term, 1 - metal and F0).
+__device__ __forceinline__ void surface_view(Surface& s, v3 camera) {
+  s.wo = normalize(camera - s.p);
+  s.nv = fmaxf(dot(s.n, s.wo), 0.00001f);
+  const float a = s.rough * s.rough;
+  s.a2 = a * a;
+  s.g1v = 2.0f * s.nv / (s.nv + sqrtf(s.a2 + (1.0f - s.a2) * s.nv * s.nv));
+  s.im = 1.0f - s.metal;
+  s.f0 = v3{0.04f * s.im + s.color.x * s.metal, 0.04f * s.im + s.color.y * s.metal,
+            0.04f * s.im + s.color.z * s.metal};
+}
+
+// brdf toward wi for a surface whose view terms surface_view() filled.
+__device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {
+  const v3 wo = s.wo;
+  const float nv = s.nv, a2 = s.a2, g1v = s.g1v, im = s.im;
+  const v3 f0 = s.f0;
   float nl = fmaxf(dot(s.n, wi), 0.0f);
   if (nl <= 0.0f) return v3{0.f, 0.f, 0.f};
   v3 h = normalize(wo + wi);
   float nh = fmaxf(dot(s.n, h), 0.0f);
   float vh = fmaxf(dot(wo, h), 0.0f);
-  float a = s.rough * s.rough;
-  float a2 = a * a;
   float q = nh * nh * (a2 - 1.0f) + 1.0f;
   float d = a2 / (IRT_PI * q * q);
-  float g1v = 2.0f * nv / (nv + sqrtf(a2 + (1.0f - a2) * nv * nv));
   float g1l = 2.0f * nl / (nl + sqrtf(a2 + (1.0f - a2) * nl * nl));
-  float im = 1.0f - s.metal;
-  v3 f0 = v3{0.04f * im + s.color.x * s.metal, 0.04f * im + s.color.y * s.metal, 0.04f * im + s.color.z * s.metal};
   float p5 = pow5(1.0f - vh);
   v3 f = v3{f0.x + (1.0f - f0.x) * p5, f0.y + (1.0f - f0.y) * p5, f0.z + (1.0f - f0.z) * p5};
   // (1-f)*(1-metal)*color/PI + f*d*g1v*g1l/max(4*nv*nl, 1e-5), evaluated left to right.
@@ -239,7 +255,7 @@ __device__ __forceinline__ v3 brdf(const Surface& s, v3 wo, v3 wi) {
 
 // direct — common.wgsl:75-79.
 template <class C>
-__device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, v3 camera, C& cnt) {
+__device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, C& cnt) {
   v3 lp = v3{l.position_strength[0], l.position_strength[1], l.position_strength[2]};
   v3 delta = lp - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0001f);
@@ -249,7 +265,7 @@ __device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, v3 ca
     if (dot(-wi, axis) < cosf(l.direction_angle[3] * 0.5f)) return v3{0.f, 0.f, 0.f};
   }
   cnt.add(C_BRDF, 1);
-  v3 b = brdf(s, normalize(camera - s.p), wi);
+  v3 b = brdf(s, wi);  // view terms of the frame's camera (surface_view)
   float k = fmaxf(dot(s.n, wi), 0.0f);
   float strength = l.position_strength[3];
   return v3{b.x * k * l.color_type[0] * strength / r2, b.y * k * l.color_type[1] * strength / r2,
@@ -258,12 +274,12 @@ __device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, v3 ca
 
 // indirect — common.wgsl:80-83.
 template <class C>
-__device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, v3 camera, C& cnt) {
+__device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, C& cnt) {
   v3 delta = ld3(v.position) - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0025f);
   v3 wi = delta * (1.0f / sqrtf(r2));  // inverseSqrt(r2)
   cnt.add(C_BRDF, 1);
-  v3 b = brdf(s, normalize(camera - s.p), wi);
+  v3 b = brdf(s, wi);  // view terms of the frame's camera (surface_view)
   float k1 = fmaxf(dot(s.n, wi), 0.0f);
   float k2 = fmaxf(dot(ld3(v.normal), -wi), 0.0f);
   float den = 4.0f * IRT_PI * r2;

@@ -223,10 +223,9 @@ __device__ __forceinline__ void body_spatial_setup(const LightArgs<P>& a, const 
     ResolveRec r{};
     if (valid) {
       const uint32_t mode = F.flags[2];
-      const v3 cam = ld3(F.camera);
       if (rd.selected_target > 0.0f && rd.id < F.counts[0] && mode != 2u && mode < 3u) {
         const irt_light& l = a.sc.lights[rd.id];
-        v3 value = direct(s, l, cam, cnt);
+        v3 value = direct(s, l, cnt);
         if (luminance(value) > 0.0f) {
           v3 c = value * normalization(rd);
           r.cd[0] = c.x;
@@ -239,7 +238,7 @@ __device__ __forceinline__ void body_spatial_setup(const LightArgs<P>& a, const 
       }
       if (mode != 1u && mode < 3u && vpl_sample_valid(F, ri)) {
         const irt_vpl& smp = selected_vpl(F, a.vpl_ring, ri);
-        v3 value = indirect(s, smp, cam, cnt);
+        v3 value = indirect(s, smp, cnt);
         if (luminance(value) > 0.0f) {
           v3 c = value * normalization(ri);
           r.ci[0] = c.x;

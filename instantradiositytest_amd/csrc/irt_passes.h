@@ -45,6 +45,7 @@ __device__ __forceinline__ Surface load_surface(const LightArgs<P>& a, int32_t x
     s.color = v3{(float)g.cr, (float)g.cg, (float)g.cb};
     s.rough = (float)g.m0;
     s.metal = (float)g.m1;
+    surface_view(s, ld3(a.F.camera));
     return s;
   }
   float2 rm = ld2(a.params, id);
@@ -54,6 +55,7 @@ __device__ __forceinline__ Surface load_surface(const LightArgs<P>& a, int32_t x
   s.color = xyz(ld4(a.color, id));
   s.rough = rm.x;
   s.metal = rm.y;
+  surface_view(s, ld3(a.F.camera));
   return s;
 }
 
@@ -125,11 +127,10 @@ __device__ __forceinline__ irt_reservoir candidates_direct(const LightArgs<P>& a
   irt_reservoir r = empty_reservoir(F.size[2], F.size[3]);
   uint32_t rng = seed(id, 11u, F.size[2]);
   if (valid && F.counts[0] > 0u) {
-    const v3 cam = ld3(F.camera);
     const float pdf = 1.0f / (float)F.counts[0];
     for (uint32_t i = 0; i < F.restir[0]; ++i) {
       uint32_t light_id = min(f2u_sat(rnd(rng) * (float)F.counts[0]), F.counts[0] - 1u);
-      float target = luminance(direct(s, a.sc.lights[light_id], cam, cnt));
+      float target = luminance(direct(s, a.sc.lights[light_id], cnt));
       add(r, light_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
     }
   }
@@ -144,13 +145,12 @@ __device__ __forceinline__ irt_reservoir candidates_indirect(const LightArgs<P>&
   irt_reservoir r = empty_reservoir(F.size[2], F.size[3]);
   uint32_t rng = seed(id, 13u, F.size[2]);
   if (valid) {
-    const v3 cam = ld3(F.camera);
     const float pdf = 1.0f / (float)F.counts[1];
     const uint32_t slots = f2u_sat(F.rr[2]) + 1u;
     const irt_vpl* current = a.vpl_ring + (size_t)(F.size[2] % slots) * F.counts[1];
     for (uint32_t i = 0; i < F.restir[1]; ++i) {
       uint32_t vpl_id = min(f2u_sat(rnd(rng) * (float)F.counts[1]), F.counts[1] - 1u);
-      float target = luminance(indirect(s, current[vpl_id], cam, cnt));
+      float target = luminance(indirect(s, current[vpl_id], cnt));
       add(r, vpl_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
     }
   }
@@ -167,7 +167,7 @@ __device__ __forceinline__ irt_reservoir temporal_direct(const LightArgs<P>& a, 
     const uint32_t hist = f2u_sat(F.rr[2]);
     irt_reservoir prev = cap(a.direct_history[q], hist * F.restir[0]);
     if (prev.generation == F.size[3] && prev.selected_target > 0.0f && prev.id < F.counts[0] && prev.age < hist) {
-      float target = luminance(direct(s, a.sc.lights[prev.id], ld3(F.camera), cnt));
+      float target = luminance(direct(s, a.sc.lights[prev.id], cnt));
       add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
           prev.age + 1u, rng);
     }
@@ -185,7 +185,7 @@ __device__ __forceinline__ irt_reservoir temporal_indirect(const LightArgs<P>& a
     irt_reservoir prev = cap(a.indirect_history[q], f2u_sat(F.rr[2]) * F.restir[1]);
     if (vpl_sample_valid(F, prev)) {
       const irt_vpl& v = selected_vpl(F, a.vpl_ring, prev);
-      float target = luminance(indirect(s, v, ld3(F.camera), cnt));
+      float target = luminance(indirect(s, v, cnt));
       add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
           prev.age + 1u, rng);
     }
@@ -213,7 +213,6 @@ __device__ __forceinline__ irt_reservoir spatial_direct(const LightArgs<P>& a, i
   const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
   uint32_t rng = seed(id, 23u, F.size[2]);
   if (valid) {
-    const v3 cam = ld3(F.camera);
     for (uint32_t i = 0; i < F.restir[2]; ++i) {
       int32_t qx, qy;
       neighbor(F, x, y, rng, &qx, &qy);
@@ -223,7 +222,7 @@ __device__ __forceinline__ irt_reservoir spatial_direct(const LightArgs<P>& a, i
       // The reference indexes lights[prev.id] unchecked; ids written this frame
       // are always < counts.x, and an out-of-range id (injected data) is skipped.
       if (prev.selected_target > 0.0f && prev.id < F.counts[0]) {
-        float target = luminance(direct(s, a.sc.lights[prev.id], cam, cnt));
+        float target = luminance(direct(s, a.sc.lights[prev.id], cnt));
         add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
             prev.age, rng);
       }
@@ -241,7 +240,6 @@ __device__ __forceinline__ irt_reservoir spatial_indirect(const LightArgs<P>& a,
   const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
   uint32_t rng = seed(id, 29u, F.size[2]);
   if (valid) {
-    const v3 cam = ld3(F.camera);
     for (uint32_t i = 0; i < F.restir[2]; ++i) {
       int32_t qx, qy;
       neighbor(F, x, y, rng, &qx, &qy);
@@ -249,7 +247,7 @@ __device__ __forceinline__ irt_reservoir spatial_indirect(const LightArgs<P>& a,
       if (!compatible(a, s, qx, qy)) continue;
       irt_reservoir prev = a.indirect_in[(uint32_t)qy * (uint32_t)W + (uint32_t)qx];
       if (vpl_sample_valid(F, prev)) {
-        float target = luminance(indirect(s, selected_vpl(F, a.vpl_ring, prev), cam, cnt));
+        float target = luminance(indirect(s, selected_vpl(F, a.vpl_ring, prev), cnt));
         add(r, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count, prev.count,
             prev.age, rng);
       }
@@ -268,7 +266,7 @@ __device__ __forceinline__ v3 resolve_direct(const LightArgs<P>& a, const irt_re
     const uint32_t mode = F.flags[2];
     if (d.selected_target > 0.0f && d.id < F.counts[0] && mode != 2u && mode < 3u) {
       const irt_light& l = a.sc.lights[d.id];
-      v3 value = direct(s, l, ld3(F.camera), cnt);
+      v3 value = direct(s, l, cnt);
       if (luminance(value) > 0.0f) {
         float w = normalization(d);
         float vis = visibility(a.sc, F, s.p, s.n, ld3(l.position_strength), v3{0.f, 0.f, 0.f}, cnt);
@@ -294,7 +292,7 @@ __device__ __forceinline__ v3 resolve_indirect(const LightArgs<P>& a, const irt_
   if (valid && F.flags[2] != 1u && F.flags[2] < 3u) {
     if (vpl_sample_valid(F, v)) {
       const irt_vpl& smp = selected_vpl(F, a.vpl_ring, v);
-      v3 value = indirect(s, smp, ld3(F.camera), cnt);
+      v3 value = indirect(s, smp, cnt);
       if (luminance(value) > 0.0f) {
         float w = normalization(v);
         float vis = visibility(a.sc, F, s.p, s.n, ld3(smp.position), ld3(smp.normal), cnt);
