@@ -695,7 +695,16 @@ int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   a.input = (const typename P::R4*)c->temporal;
   a.moments = c->moments[cur];
   a.out = (typename P::R4*)c->scratch[0];
-  LAUNCH_SVGF(c, k_svgf_variance, P, guide != nullptr, a);
+  if (guide) {
+    const dim3 grid((c->W + VAR_TILE - 1) / VAR_TILE, (c->H + VAR_TILE - 1) / VAR_TILE);
+    if (c->counting)
+      hipLaunchKernelGGL((k_svgf_variance_lds<P, true>), grid, dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_svgf_variance_lds<P, false>), grid, dim3(256), 0, c->stream, a);
+    HIP_CHECK(c, hipGetLastError());
+    return IRT_OK;
+  }
+  LAUNCH_SVGF(c, k_svgf_variance, P, false, a);
   return IRT_OK;
 }
 template <class P>

@@ -118,6 +118,19 @@ __global__ __launch_bounds__(256) void k_batch_atrous(const BatchView<P>* __rest
   }
 }
 
+// Batched variance estimate: the 16 x 16 tile table, LDS-staged 7 x 7
+// neighbourhoods; the body is k_svgf_variance_lds's.
+template <class P>
+__global__ __launch_bounds__(256) void k_batch_variance(const BatchView<P>* __restrict__ views,
+                                                       const uint32_t* __restrict__ tiles) {
+  __shared__ uint4 s_g[VAR_SIDE * VAR_SIDE][2];
+  __shared__ float2 s_m[VAR_SIDE * VAR_SIDE];
+  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+  const BatchView<P>& bv = views[e >> 24];
+  Count<false> cnt;
+  variance_tile(bv.sv, (int32_t)((e >> 12) & 0xfffu) * VAR_TILE, (int32_t)(e & 0xfffu) * VAR_TILE, s_g, s_m, cnt);
+}
+
 __global__ __launch_bounds__(64) void k_zero_queues(const MarchView* __restrict__ views, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n * 16u) views[i / 16u].queue[i % 16u] = 0u;
@@ -316,7 +329,12 @@ int batch_svgf(irt_batch* b, int q) {
   bmark(b, st, BK_SVGF_T, 1);
   if (!rc) {
     bmark(b, st, BK_SVGF_V, 0);
-    rc = launch_kind<BK_SVGF_V, P>(b, q, st, 0);
+    // LDS-staged neighbourhoods need the 16 x 16 tiles (table of shape 2)
+    if (b->n_tiles[2]) {
+      hipLaunchKernelGGL((k_batch_variance<P>), dim3(b->n_tiles[2]), dim3(256), 0, st,
+                         (const BatchView<P>*)b->d_views[q], b->d_tiles + b->tile_off[2]);
+      BHIP(b, hipGetLastError());
+    }
     bmark(b, st, BK_SVGF_V, 1);
   }
   bmark(b, st, BK_SVGF_A, 0);

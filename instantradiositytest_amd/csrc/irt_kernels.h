@@ -375,6 +375,56 @@ __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
   cnt.flush(a.counters);
 }
 
+// Variance estimate with the 7 x 7 neighbourhood staged in LDS (guide
+// records present).  A block renders a 16 x 16 tile; when any of its pixels
+// is young enough for the 7 x 7 estimate (age < 4), it stages the guide
+// records and first two moments of the 22 x 22 window around the tile once
+// and every tap reads LDS; otherwise each pixel passes its variance through.
+constexpr int32_t VAR_TILE = 16, VAR_HALO = 3, VAR_SIDE = VAR_TILE + 2 * VAR_HALO;
+// The 16 x 16 tile at (x0, y0) of the variance pass, one pixel per thread of
+// a 256-thread block (every thread of the block calls it).
+template <class P, class C>
+__device__ __forceinline__ void variance_tile(const SvgfArgs<P>& a, int32_t x0, int32_t y0, uint4 (*s_g)[2],
+                                              float2* s_m, C& cnt) {
+  const int32_t W = (int32_t)a.F.size[0], H = (int32_t)a.F.size[1];
+  const int32_t i = (int32_t)threadIdx.x % VAR_TILE, j = (int32_t)threadIdx.x / VAR_TILE;
+  const int32_t x = x0 + i, y = y0 + j;
+  const bool in = x < W && y < H;
+  const uint32_t id = in ? (uint32_t)y * (uint32_t)W + (uint32_t)x : 0u;
+  const bool young = in && a.moments[id].z < 4.0f;
+  if (__syncthreads_or(young)) {
+    for (int32_t e = (int32_t)threadIdx.x; e < VAR_SIDE * VAR_SIDE; e += 256) {
+      const int32_t gx = x0 + e % VAR_SIDE - VAR_HALO, gy = y0 + e / VAR_SIDE - VAR_HALO;
+      if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+        const uint32_t q = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+        const uint4* gp = reinterpret_cast<const uint4*>(a.guide + q);
+        s_g[e][0] = gp[0];
+        s_g[e][1] = gp[1];
+        const float4 mo = a.moments[q];
+        s_m[e] = make_float2(mo.x, mo.y);
+      } else {
+        s_g[e][0] = make_uint4(0u, 0u, 0u, 0u);  // valid = 0: outside the image
+        s_g[e][1] = make_uint4(0u, 0u, 0u, 0u);
+        s_m[e] = make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+    if (in)
+      st_r16(a.out, id, svgf_variance_staged<P, C, true>(a, x, y, s_g, s_m, i + VAR_HALO, j + VAR_HALO, VAR_SIDE, cnt));
+  } else if (in) {
+    const float4 c = ld4(a.input, id);
+    st_r16(a.out, id, make_float4(c.x, c.y, c.z, fminf(c.w, 65000.0f)));
+  }
+}
+template <class P, bool CNT>
+__global__ __launch_bounds__(256) void k_svgf_variance_lds(SvgfArgs<P> a) {
+  __shared__ uint4 s_g[VAR_SIDE * VAR_SIDE][2];
+  __shared__ float2 s_m[VAR_SIDE * VAR_SIDE];
+  Count<CNT> cnt;
+  variance_tile(a, (int32_t)blockIdx.x * VAR_TILE, (int32_t)blockIdx.y * VAR_TILE, s_g, s_m, cnt);
+  cnt.flush(a.counters);
+}
+
 // A-trous iteration with the tap neighbourhood staged in LDS (guide records
 // present).  Block (bx, by, z) renders a 16 x 16 patch of one coset of the
 // stride-s lattice, x = bx*16s + (z % s) + i*s, y = by*16s + (z / s) + j*s:

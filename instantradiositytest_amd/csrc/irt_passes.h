@@ -553,6 +553,45 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
   return make_float4(c.x, c.y, c.z, fminf(sigma, 65000.0f));
 }
 
+// svgf_variance for pixel (x, y) whose 7 x 7 neighbourhood is staged: the
+// guide records (two 16-byte halves; valid = 0 outside the image) and the
+// first two moments of an n-wide patch with the pixel at (li, lj).  Same
+// arithmetic and tap order as svgf_variance with guide records.
+template <class P, class C, bool FAST>
+__device__ __forceinline__ float4 svgf_variance_staged(const SvgfArgs<P>& a, int32_t x, int32_t y,
+                                                       const uint4 (*sg)[2], const float2* sm, int32_t li, int32_t lj,
+                                                       int32_t n, C& cnt) {
+  const uint32_t W = a.F.size[0];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  float4 c = ld4(a.input, id);
+  float age = a.moments[id].z;
+  float sigma = c.w;
+  const int32_t e0 = lj * n + li;
+  const GuidePx self = unpack_guide(sg[e0][0], sg[e0][1]);
+  if (age < 4.0f && self.valid) {
+    float s0 = 0.f, s1 = 0.f, total = 0.f;
+    const Guide g = guide_of(a.F, self);
+    const float inv_tol = FAST ? 1.0f / tap_tolerance(g, 1.0f) : 0.0f;
+    for (int32_t j = -3; j <= 3; ++j) {
+      for (int32_t i = -3; i <= 3; ++i) {
+        cnt.add(C_TAP, 1);
+        const int32_t e = (lj + j) * n + li + i;
+        float w = guide_tap_weight<FAST>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol);
+        if (w <= 0.0f) continue;
+        const float2 mo = sm[e];
+        s0 += mo.x * w;
+        s1 += mo.y * w;
+        total += w;
+      }
+    }
+    float t = fmaxf(total, 0.00001f);
+    s0 /= t;
+    s1 /= t;
+    sigma = sqrtf(fmaxf(s1 - s0 * s0, 0.0f) * 4.0f / fmaxf(age, 1.0f));
+  }
+  return make_float4(c.x, c.y, c.z, fminf(sigma, 65000.0f));
+}
+
 // kernel, svgf_atrous — svgf.wgsl:90-110.
 __device__ __forceinline__ float bspline(int32_t x) { return x == 0 ? 1.0f : ((x == 1 || x == -1) ? 2.0f / 3.0f : 1.0f / 6.0f); }
 
