@@ -684,7 +684,16 @@ int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   a.moments = c->moments[old];
   a.out = (typename P::R4*)c->temporal;
   a.moments_out = c->moments[cur];
-  LAUNCH_SVGF(c, k_svgf_temporal, P, guide != nullptr, a);
+  if (guide) {
+    const dim3 grid((c->W + TMP_TILE - 1) / TMP_TILE, (c->H + TMP_TILE - 1) / TMP_TILE);
+    if (c->counting)
+      hipLaunchKernelGGL((k_svgf_temporal_lds<P, true>), grid, dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_svgf_temporal_lds<P, false>), grid, dim3(256), 0, c->stream, a);
+    HIP_CHECK(c, hipGetLastError());
+    return IRT_OK;
+  }
+  LAUNCH_SVGF(c, k_svgf_temporal, P, false, a);
   return IRT_OK;
 }
 template <class P>

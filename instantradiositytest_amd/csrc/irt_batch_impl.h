@@ -118,6 +118,19 @@ __global__ __launch_bounds__(256) void k_batch_atrous(const BatchView<P>* __rest
   }
 }
 
+// Batched temporal accumulation: the 16 x 16 tile table, LDS-staged clamp
+// windows; the body is k_svgf_temporal_lds's.
+template <class P>
+__global__ __launch_bounds__(256) void k_batch_temporal(const BatchView<P>* __restrict__ views,
+                                                       const uint32_t* __restrict__ tiles) {
+  __shared__ uint4 s_g[TMP_SIDE * TMP_SIDE][2];
+  __shared__ typename P::R4 s_r[TMP_SIDE * TMP_SIDE];
+  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+  const BatchView<P>& bv = views[e >> 24];
+  Count<false> cnt;
+  temporal_tile(bv.st, (int32_t)((e >> 12) & 0xfffu) * TMP_TILE, (int32_t)(e & 0xfffu) * TMP_TILE, s_g, s_r, cnt);
+}
+
 // Batched variance estimate: the 16 x 16 tile table, LDS-staged 7 x 7
 // neighbourhoods; the body is k_svgf_variance_lds's.
 template <class P>
@@ -325,7 +338,12 @@ int batch_svgf(irt_batch* b, int q) {
   hipStream_t st = b->svgf_stream;
   int rc = IRT_OK;
   bmark(b, st, BK_SVGF_T, 0);
-  rc = launch_kind<BK_SVGF_T, P>(b, q, st, 0);
+  // LDS-staged clamp windows need the 16 x 16 tiles (table of shape 2)
+  if (b->n_tiles[2]) {
+    hipLaunchKernelGGL((k_batch_temporal<P>), dim3(b->n_tiles[2]), dim3(256), 0, st,
+                       (const BatchView<P>*)b->d_views[q], b->d_tiles + b->tile_off[2]);
+    BHIP(b, hipGetLastError());
+  }
   bmark(b, st, BK_SVGF_T, 1);
   if (!rc) {
     bmark(b, st, BK_SVGF_V, 0);

@@ -380,6 +380,49 @@ __global__ __launch_bounds__(64) void k_svgf_atrous(SvgfArgs<P> a) {
 // is young enough for the 7 x 7 estimate (age < 4), it stages the guide
 // records and first two moments of the 22 x 22 window around the tile once
 // and every tap reads LDS; otherwise each pixel passes its variance through.
+// Temporal accumulation with the 3 x 3 clamp window staged in LDS (guide
+// records present): the 16 x 16 tile at (x0, y0), one pixel per thread of a
+// 256-thread block (every thread of the block calls it); the guide records
+// and noisy radiance of the 18 x 18 window around the tile are staged once.
+constexpr int32_t TMP_TILE = 16, TMP_HALO = 1, TMP_SIDE = TMP_TILE + 2 * TMP_HALO;
+template <class P, class C>
+__device__ __forceinline__ void temporal_tile(const SvgfArgs<P>& a, int32_t x0, int32_t y0, uint4 (*s_g)[2],
+                                              typename P::R4* s_r, C& cnt) {
+  const int32_t W = (int32_t)a.F.size[0], H = (int32_t)a.F.size[1];
+  for (int32_t e = (int32_t)threadIdx.x; e < TMP_SIDE * TMP_SIDE; e += 256) {
+    const int32_t gx = x0 + e % TMP_SIDE - TMP_HALO, gy = y0 + e / TMP_SIDE - TMP_HALO;
+    if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+      const uint32_t q = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+      const uint4* gp = reinterpret_cast<const uint4*>(a.guide + q);
+      s_g[e][0] = gp[0];
+      s_g[e][1] = gp[1];
+      s_r[e] = a.noisy[q];
+    } else {
+      s_g[e][0] = make_uint4(0u, 0u, 0u, 0u);  // valid = 0: outside the image
+      s_g[e][1] = make_uint4(0u, 0u, 0u, 0u);
+      s_r[e] = typename P::R4{};
+    }
+  }
+  __syncthreads();
+  const int32_t i = (int32_t)threadIdx.x % TMP_TILE, j = (int32_t)threadIdx.x / TMP_TILE;
+  const int32_t x = x0 + i, y = y0 + j;
+  if (x < W && y < H) {
+    const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+    float4 rad, mom;
+    svgf_temporal_staged<P>(a, x, y, s_g, s_r, i + TMP_HALO, j + TMP_HALO, TMP_SIDE, &rad, &mom, cnt);
+    st_r16(a.out, id, rad);
+    a.moments_out[id] = mom;
+  }
+}
+template <class P, bool CNT>
+__global__ __launch_bounds__(256) void k_svgf_temporal_lds(SvgfArgs<P> a) {
+  __shared__ uint4 s_g[TMP_SIDE * TMP_SIDE][2];
+  __shared__ typename P::R4 s_r[TMP_SIDE * TMP_SIDE];
+  Count<CNT> cnt;
+  temporal_tile(a, (int32_t)blockIdx.x * TMP_TILE, (int32_t)blockIdx.y * TMP_TILE, s_g, s_r, cnt);
+  cnt.flush(a.counters);
+}
+
 constexpr int32_t VAR_TILE = 16, VAR_HALO = 3, VAR_SIDE = VAR_TILE + 2 * VAR_HALO;
 // The 16 x 16 tile at (x0, y0) of the variance pass, one pixel per thread of
 // a 256-thread block (every thread of the block calls it).

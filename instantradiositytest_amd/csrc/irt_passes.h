@@ -553,6 +553,102 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
   return make_float4(c.x, c.y, c.z, fminf(sigma, 65000.0f));
 }
 
+// history_ok with the current pixel's position and normal given (its guide
+// record): the same tests in the same order.
+template <class P>
+__device__ __forceinline__ bool history_ok_at(const SvgfArgs<P>& a, v3 pa, v3 na, int32_t qx, int32_t qy) {
+  const irt_frame& F = a.F;
+  if (F.flags[0] == 0u || F.flags[1] != F.size[3] || !inside(a, qx, qy)) return false;
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+  float d = a.old_depth[q];
+  if (d >= 1.0f) return false;
+  v3 pb = world(pixel_u(qx, W), pixel_u(qy, H), d, F.previous_inv_vp);
+  v3 cam = ld3(F.camera);
+  return length(pa - pb) < F.temporal[1] * fmaxf(length(pa - cam), 1.0f) && dot(na, xyz(ld4(a.old_normal, q))) > F.temporal[2];
+}
+
+// svgf_temporal (FAST) for pixel (x, y) whose 3 x 3 clamp window is staged:
+// guide records (two 16-byte halves) and noisy radiance of an n-wide patch
+// with the pixel at (li, lj).  The reprojected history reads the previous
+// frame's buffers.  Same arithmetic and order as svgf_temporal.
+template <class P, class C>
+__device__ __forceinline__ void svgf_temporal_staged(const SvgfArgs<P>& a, int32_t x, int32_t y, const uint4 (*sg)[2],
+                                                     const typename P::R4* sr, int32_t li, int32_t lj, int32_t n,
+                                                     float4* rad_out, float4* mom_out, C& cnt) {
+  const irt_frame& F = a.F;
+  const uint32_t W = F.size[0], H = F.size[1];
+  const uint32_t id = (uint32_t)y * W + (uint32_t)x;
+  const int32_t e0 = lj * n + li;
+  v3 c = xyz(ld4(sr, (uint32_t)e0));
+  float l = luminance(c);
+  const GuidePx self = unpack_guide(sg[e0][0], sg[e0][1]);
+  v3 history = v3{0.f, 0.f, 0.f};
+  float hm0 = 0.f, hm1 = 0.f, length_history = 0.f, total = 0.f;
+  if (self.valid) {
+    const v3 pa = v3{self.px, self.py, self.pz};
+    const v3 na = v3{(float)self.nx, (float)self.ny, (float)self.nz};
+    float2 m = ld2(a.motion, id);
+    float px = (pixel_u(x, W) - m.x) * (float)W - 0.5f;
+    float py = (pixel_u(y, H) - m.y) * (float)H - 0.5f;
+    float flx = floorf(px), fly = floorf(py);
+    int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    float fx = px - flx, fy = py - fly;
+    for (int32_t j = 0; j < 2; ++j) {
+      for (int32_t i = 0; i < 2; ++i) {
+        int32_t qx = bx + i, qy = by + j;
+        if (!history_ok_at(a, pa, na, qx, qy)) continue;
+        const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
+        float4 mo = a.old_moments[q];
+        if (mo.z <= 0.0f) continue;
+        float wx = i == 1 ? fx : 1.0f - fx;
+        float wy = j == 1 ? fy : 1.0f - fy;
+        float w = wx * wy;
+        history = history + xyz(ld4(a.old_radiance, q)) * w;
+        hm0 += mo.x * w;
+        hm1 += mo.y * w;
+        length_history += mo.z * w;
+        total += w;
+      }
+    }
+  }
+  float age = 1.0f;
+  v3 result = c;
+  float m0 = l, m1 = l * l;
+  if (total > 0.001f) {
+    history = history / total;
+    hm0 /= total;
+    hm1 /= total;
+    age = fminf(length_history / total + 1.0f, F.temporal[3]);
+    float lo = 1e20f, hi = 0.0f;
+    const Guide g = guide_of(F, self);
+    const float inv_tol = 1.0f / tap_tolerance(g, 1.0f);
+    for (int32_t j = -1; j <= 1; ++j) {
+      for (int32_t i = -1; i <= 1; ++i) {
+        int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+        int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+        cnt.add(C_TAP, 1);
+        const int32_t e = (lj + qy - y) * n + li + qx - x;
+        if (guide_tap_weight<true>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol) > 0.01f) {
+          float v = luminance(xyz(ld4(sr, (uint32_t)e)));
+          lo = fminf(lo, v);
+          hi = fmaxf(hi, v);
+        }
+      }
+    }
+    float lh = luminance(history);
+    history = history * (clampf(lh, lo * 0.5f, hi * 1.5f + 0.001f) / fmaxf(lh, 0.000001f));
+    float alpha = fmaxf(1.0f - F.temporal[0], 1.0f / age);
+    result = history * (1.0f - alpha) + c * alpha;
+    float beta = fmaxf(0.2f, 1.0f / age);
+    m0 = hm0 * (1.0f - beta) + m0 * beta;
+    m1 = hm1 * (1.0f - beta) + m1 * beta;
+  }
+  float variance = fmaxf(m1 - m0 * m0, 0.0f);
+  *rad_out = make_float4(result.x, result.y, result.z, fminf(sqrtf(variance), 65000.0f));
+  *mom_out = make_float4(m0, m1, age, 0.0f);
+}
+
 // svgf_variance for pixel (x, y) whose 7 x 7 neighbourhood is staged: the
 // guide records (two 16-byte halves; valid = 0 outside the image) and the
 // first two moments of an n-wide patch with the pixel at (li, lj).  Same
