@@ -729,7 +729,8 @@ int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, co
   if (guide) {
     // fused schedule: taps staged in LDS per 16 x 16 patch of a stride-lattice coset
     const uint32_t span = (uint32_t)AT_TILE << i;
-    const dim3 grid((c->W + span - 1) / span, (c->H + span - 1) / span, 1u << (2 * i));
+    const uint32_t regions = ((c->W + span - 1) / span) * ((c->H + span - 1) / span);
+    const dim3 grid(((regions + 7) / 8) * 8 * (1u << (2 * i)));
     if (c->counting)
       hipLaunchKernelGGL((k_svgf_atrous_lds<P, true>), grid, dim3(256), 0, c->stream, a);
     else

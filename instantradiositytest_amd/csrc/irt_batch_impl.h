@@ -77,19 +77,22 @@ __global__ __launch_bounds__(TW* TH) void k_batch(const BatchView<P>* __restrict
 }
 
 // Batched a-trous iteration `iter` with LDS-staged taps: tile entries are
-// 16s x 16s pixel patches (s = 2^iter) of a view, blockIdx.y the lattice
-// coset; the body is k_svgf_atrous_lds's.
+// 16s x 16s pixel patches (s = 2^iter) of a view, placed with their lattice
+// cosets by atrous_slot; the body is k_svgf_atrous_lds's.
 template <class P>
 __global__ __launch_bounds__(256) void k_batch_atrous(const BatchView<P>* __restrict__ views,
-                                                     const uint32_t* __restrict__ tiles, uint32_t iter) {
+                                                     const uint32_t* __restrict__ tiles, uint32_t n_regions,
+                                                     uint32_t iter) {
   __shared__ uint4 s_g[AT_SIDE * AT_SIDE][2];
   __shared__ typename P::R4 s_r[AT_SIDE * AT_SIDE];
-  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[blockIdx.x]);
+  uint32_t region, coset;
+  if (!atrous_slot(blockIdx.x, n_regions, 1u << (2 * iter), &region, &coset)) return;
+  const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[region]);
   const BatchView<P>& bv = views[e >> 24];
   if (iter >= bv.iterations) return;  // this view has finished its a-trous iterations
   const SvgfArgs<P>& a = bv.sa[iter];
   const int32_t W = (int32_t)bv.W, H = (int32_t)bv.H, st = (int32_t)a.stride;
-  const int32_t cx = (int32_t)blockIdx.y % st, cy = (int32_t)blockIdx.y / st;
+  const int32_t cx = (int32_t)coset % st, cy = (int32_t)coset / st;
   const int32_t x0 = (int32_t)((e >> 12) & 0xfffu) * AT_TILE * st + cx;
   const int32_t y0 = (int32_t)(e & 0xfffu) * AT_TILE * st + cy;
   for (int32_t k = (int32_t)threadIdx.x; k < AT_SIDE * AT_SIDE; k += 256) {
@@ -358,8 +361,8 @@ int batch_svgf(irt_batch* b, int q) {
   bmark(b, st, BK_SVGF_A, 0);
   for (uint32_t i = 0; !rc && i < b->max_iterations; ++i) {
     if (b->at_n[i] == 0) continue;
-    hipLaunchKernelGGL((k_batch_atrous<P>), dim3(b->at_n[i], 1u << (2 * i)), dim3(256), 0, st,
-                       (const BatchView<P>*)b->d_views[q], b->d_tiles + b->at_off[i], i);
+    hipLaunchKernelGGL((k_batch_atrous<P>), dim3(((b->at_n[i] + 7) / 8) * 8 * (1u << (2 * i))), dim3(256), 0, st,
+                       (const BatchView<P>*)b->d_views[q], b->d_tiles + b->at_off[i], b->at_n[i], i);
     BHIP(b, hipGetLastError());
   }
   bmark(b, st, BK_SVGF_A, 1);

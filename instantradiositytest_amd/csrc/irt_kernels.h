@@ -469,19 +469,38 @@ __global__ __launch_bounds__(256) void k_svgf_variance_lds(SvgfArgs<P> a) {
 }
 
 // A-trous iteration with the tap neighbourhood staged in LDS (guide records
-// present).  Block (bx, by, z) renders a 16 x 16 patch of one coset of the
-// stride-s lattice, x = bx*16s + (z % s) + i*s, y = by*16s + (z / s) + j*s:
+// present).  A block renders a 16 x 16 patch of one coset (cx, cy) of the
+// stride-s lattice in patch region (bx, by), x = bx*16s + cx + i*s,
+// y = by*16s + cy + j*s (atrous_slot places the block):
 // all 5 x 5 taps of those pixels fall on the same lattice, so the block
 // stages the 20 x 20 lattice points around the patch (guide record + input
 // radiance) once and every tap reads LDS.  Arithmetic as svgf_atrous.
 constexpr int32_t AT_TILE = 16, AT_HALO = 2, AT_SIDE = AT_TILE + 2 * AT_HALO;
+
+// Launch slot L of an a-trous launch over (patch region, lattice coset)
+// pairs.  The dispatcher deals slots to the eight XCDs round-robin, so slot
+// L = 8 (r_hi C + c) + r_lo, with region r = 8 r_hi + r_lo and coset c < C,
+// puts all cosets of a region on one XCD one after another: they read the
+// same cache lines of the guide records and radiance.  Launches have
+// 8 ceil(regions / 8) C slots; false for the padding slots.
+__device__ __forceinline__ bool atrous_slot(uint32_t L, uint32_t regions, uint32_t cosets, uint32_t* region,
+                                            uint32_t* coset) {
+  const uint32_t r_lo = L % 8u, rest = L / 8u;
+  *coset = rest % cosets;
+  *region = (rest / cosets) * 8u + r_lo;
+  return *region < regions;
+}
+
 template <class P, bool CNT>
 __global__ __launch_bounds__(256) void k_svgf_atrous_lds(SvgfArgs<P> a) {
   __shared__ uint4 s_g[AT_SIDE * AT_SIDE][2];
   __shared__ typename P::R4 s_r[AT_SIDE * AT_SIDE];
   const int32_t W = (int32_t)a.F.size[0], H = (int32_t)a.F.size[1], st = (int32_t)a.stride;
-  const int32_t cx = (int32_t)blockIdx.z % st, cy = (int32_t)blockIdx.z / st;
-  const int32_t x0 = (int32_t)blockIdx.x * AT_TILE * st + cx, y0 = (int32_t)blockIdx.y * AT_TILE * st + cy;
+  const uint32_t span = (uint32_t)(AT_TILE * st), nx = ((uint32_t)W + span - 1) / span, ny = ((uint32_t)H + span - 1) / span;
+  uint32_t region, coset;
+  if (!atrous_slot(blockIdx.x, nx * ny, (uint32_t)(st * st), &region, &coset)) return;
+  const int32_t cx = (int32_t)coset % st, cy = (int32_t)coset / st;
+  const int32_t x0 = (int32_t)(region % nx) * AT_TILE * st + cx, y0 = (int32_t)(region / nx) * AT_TILE * st + cy;
   for (int32_t e = (int32_t)threadIdx.x; e < AT_SIDE * AT_SIDE; e += 256) {
     const int32_t gx = x0 + (e % AT_SIDE - AT_HALO) * st, gy = y0 + (e / AT_SIDE - AT_HALO) * st;
     if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
