@@ -282,6 +282,19 @@ def run_serving(streams, rng, seconds: float, warmup_s: float, sessions: int, va
     return d
 
 
+def measured_traffic(config: str, kernel: str):
+    """Bytes per launch of the kernel between L2 and memory, from the two
+    rocprofv3 counter passes summarised in profiles/ (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a copy kernel of known size); None when no
+    summary covers this workload."""
+    path = Path(__file__).resolve().parent / "profiles" / "hbm_traffic.json"
+    if config != "plain-1920x1200" or not path.exists():
+        return None
+    per = json.loads(path.read_text())["per_launch"]
+    rec = per.get("k_" + kernel)
+    return rec["traffic_bytes"] if rec else None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,7 +395,8 @@ def main() -> int:
                 "bound": "mfma", "compute": "FP32 vector ALU (no matrix work on this path; peak is the FP32 "
                                             "vector figure)",
                 "kernel": dominant, "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5),
+                "traffic": measured_traffic(args.config, dominant),
                 "flops_per_launch": flop, "launch_ms": head["kernel_ms"][dominant]}
         line["kernel_tflops"] = {k: round(kf[k] / (ms * 1e-3) / 1e12, 3) for k, ms in per_launch.items() if kf.get(k)}
         line["reference_events_per_frame"] = ref
