@@ -413,11 +413,16 @@ int batch_encode(irt_batch* b) {
     if (!c->grid.cells || c->n_objects > LDS_MAX_OBJECTS || c->n_meshes > LDS_MAX_OBJECTS)
       return b->fail(IRT_ERR_LIMIT, "batched views need scenes of at most 64 objects and meshes");
     if (c->frame.flags[2] >= 3u) return b->fail(IRT_ERR_STATE, "batched views render debug modes < 3");
-    rc = join(c, c->svgf_stream, c->stream);
-    // this step writes the view's other guide and noisy-radiance slots: wait
-    // for the SVGF (batched or not) of the frame before the previous one
-    if (!rc) rc = wait_for_slots(c);
+    // this step writes the view's other guide and noisy-radiance slots and a
+    // G-buffer slot: wait for the SVGF (batched or not) of the frame before
+    // the previous one, their last reader
+    rc = wait_for_slots(c);
     if (rc) return b->fail(rc, std::string("view: ") + c->err);
+    // the step's SVGF follows the view's earlier SVGF work (moments and
+    // feedback ping-pong); its lighting follows the view's ingest.  The
+    // previous step's SVGF is not waited for: it overlaps this step's lighting.
+    BHIP(b, hipEventRecord(b->view_ready, c->svgf_stream));
+    BHIP(b, hipStreamWaitEvent(b->svgf_stream, b->view_ready, 0));
     BHIP(b, hipEventRecord(b->view_ready, c->stream));
     BHIP(b, hipStreamWaitEvent(b->stream, b->view_ready, 0));
   }
