@@ -156,6 +156,7 @@ __global__ __launch_bounds__(64) void k_zero_queues(const MarchView* __restrict_
 
 struct irt_batch {
   std::vector<irt_ctx*> views;
+  std::vector<std::array<uint32_t, 3>> view_shape;  // W, H, SVGF iterations at set_views
   uint32_t precision = 0;
   int device = 0;
   hipStream_t stream = nullptr;       // lighting passes
@@ -586,15 +587,25 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
   b->tiles_used[f] = true;
   b->tflip = 1 - f;
   b->views.assign(views, views + count);
+  b->view_shape.resize(count);
+  for (uint32_t v = 0; v < count; ++v)
+    b->view_shape[v] = {views[v]->W, views[v]->H, views[v]->s.svgf_iterations};
   b->max_iterations = max_it;
   drop_graph(b);
   return IRT_OK;
 }
 
+// True when `views` are the batch's members with the sizes and SVGF
+// iteration counts its tile tables were built for (a destroyed context's
+// address can come back as a new context of another size).
 bool same_views(const irt_batch* b, irt_ctx* const* views, uint32_t count) {
   if (b->views.size() != count) return false;
-  for (uint32_t i = 0; i < count; ++i)
-    if (b->views[i] != views[i]) return false;
+  for (uint32_t i = 0; i < count; ++i) {
+    const irt_ctx* c = views[i];
+    if (b->views[i] != c || b->view_shape[i][0] != c->W || b->view_shape[i][1] != c->H ||
+        b->view_shape[i][2] != c->s.svgf_iterations)
+      return false;
+  }
   return true;
 }
 

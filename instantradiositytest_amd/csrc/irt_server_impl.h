@@ -4,17 +4,23 @@
 // Included at the end of irt_api.hip after the batch implementation.  Each
 // session is one context (reservoirs, SVGF history, previous G-buffer, VPL
 // ring and its own copy of a scene).  Clients submit frame requests whose
-// G-buffer and VPL pool already sit in device memory; every step takes the
-// oldest pending request of each session that has one and renders all of
-// them with one batched frame, so requests that arrive while a step is on
-// the GPU join the next step.  Up to `max_in_flight` steps are queued on the
-// device at once.  A session's frames are rendered in submission order and
-// its results do not depend on which other sessions share its steps.
+// G-buffer and VPL pool already sit in device memory.  Sessions are dealt
+// to `max_in_flight` step groups (the group with the fewest pixels takes a
+// new session), each with its own batch; a step takes the oldest pending
+// request of every session of one group, the groups taking turns, and
+// renders them with one batched frame.  Requests that arrive while steps
+// are on the GPU join their group's next step.  Up to `max_in_flight` steps
+// are queued on the device at once, and because consecutive steps belong to
+// different groups, one group's SVGF overlaps the next group's lighting
+// even when every client waits for its previous frame.  A session's frames
+// are rendered in submission order and its results do not depend on which
+// other sessions share its steps.
 //
 // The frame bookkeeping of Renderer::encode (renderer.rs:1687-1735 packing,
 // 1973-1982 previous_vp / previous_generation / previous_debug /
 // frame_index) is kept per session.
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 
@@ -41,6 +47,8 @@ struct ServerSession {
   uint32_t previous_debug = 0;
   std::deque<ServerRequest> pending;
   uint32_t in_flight = 0;
+  uint32_t group = 0;
+  uint64_t pixels = 0;
 };
 
 struct ServerStep {
@@ -56,7 +64,9 @@ struct irt_server {
   irt_server_config cfg{};
   std::vector<ServerSession> sessions;
   std::vector<irt_ctx*> idle;  // closed sessions' contexts kept for reuse
-  irt_batch* batch = nullptr;
+  std::vector<irt_batch*> batches;      // one per step group (max_in_flight)
+  std::vector<uint64_t> group_pixels;   // pixels of the open sessions of each group
+  uint32_t next_group = 0;              // the group whose turn is next
   std::deque<ServerStep> in_flight;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spare_events;
   std::deque<irt_completion> completed;
@@ -143,6 +153,8 @@ int irt_server_create(const irt_server_config* cfg, irt_server** out) {
   irt_server* s = new irt_server();
   s->cfg = *cfg;
   s->sessions.resize(cfg->max_sessions);
+  s->batches.assign(cfg->max_in_flight, nullptr);
+  s->group_pixels.assign(cfg->max_in_flight, 0);
   *out = s;
   return IRT_OK;
 }
@@ -153,7 +165,8 @@ void irt_server_destroy(irt_server* s) {
     (void)hipEventSynchronize(s->in_flight.front().done);
     retire_front(s);
   }
-  if (s->batch) irt_batch_destroy(s->batch);
+  for (irt_batch* b : s->batches)
+    if (b) irt_batch_destroy(b);
   for (auto& x : s->sessions)
     if (x.ctx) irt_destroy(x.ctx);
   for (irt_ctx* c : s->idle) irt_destroy(c);
@@ -227,6 +240,9 @@ int irt_server_open(irt_server* s, const irt_settings* settings, irt_ctx* scene,
   ss = ServerSession{};
   ss.ctx = c;
   ss.open = true;
+  ss.pixels = (uint64_t)c->W * c->H;
+  ss.group = (uint32_t)(std::min_element(s->group_pixels.begin(), s->group_pixels.end()) - s->group_pixels.begin());
+  s->group_pixels[ss.group] += ss.pixels;
   *session = id;
   return IRT_OK;
 }
@@ -243,6 +259,7 @@ int irt_server_close(irt_server* s, uint32_t session) {
   ServerSession& ss = s->sessions[session];
   ss.open = false;
   ss.pending.clear();
+  s->group_pixels[ss.group] -= ss.pixels;
   // the context stays with the slot until its frames in flight are done
   if (!ss.in_flight) {
     s->idle.push_back(ss.ctx);
@@ -292,14 +309,21 @@ int irt_server_step(irt_server* s) {
       ss.ctx = nullptr;
     }
   if (s->in_flight.size() >= s->cfg.max_in_flight) return 0;
+  // the first group in turn with pending requests
+  const uint32_t groups = (uint32_t)s->batches.size();
   std::vector<irt_ctx*> views;
   std::vector<uint32_t> ids;
-  for (uint32_t i = 0; i < s->sessions.size(); ++i)
-    if (s->sessions[i].open && !s->sessions[i].pending.empty()) {
-      views.push_back(s->sessions[i].ctx);
-      ids.push_back(i);
-    }
+  uint32_t g = 0;
+  for (uint32_t t = 0; t < groups && views.empty(); ++t) {
+    g = (s->next_group + t) % groups;
+    for (uint32_t i = 0; i < s->sessions.size(); ++i)
+      if (s->sessions[i].open && s->sessions[i].group == g && !s->sessions[i].pending.empty()) {
+        views.push_back(s->sessions[i].ctx);
+        ids.push_back(i);
+      }
+  }
   if (views.empty()) return 0;
+  s->next_group = (g + 1) % groups;
   ServerStep st;
   st.seq = s->next_seq++;
   for (size_t k = 0; k < ids.size(); ++k) {
@@ -318,14 +342,15 @@ int irt_server_step(irt_server* s) {
     cm.pixels = (uint64_t)c->W * c->H;
     st.members.push_back(cm);
   }
-  if (!s->batch) {
-    rc = irt_batch_create(views.data(), (uint32_t)views.size(), &s->batch);
+  irt_batch*& batch = s->batches[g];
+  if (!batch) {
+    rc = irt_batch_create(views.data(), (uint32_t)views.size(), &batch);
     if (rc) return s->fail(rc, std::string("batch: ") + irt_host::global_error());
-    irt_batch_use_graph(s->batch, s->cfg.use_graph ? 1 : 0);
-    for (uint32_t k = 0; k < BK_COUNT; ++k) s->batch->shape[k] = s->cfg.tile_shapes[k] <= 2 ? s->cfg.tile_shapes[k] : 0;
+    irt_batch_use_graph(batch, s->cfg.use_graph ? 1 : 0);
+    for (uint32_t k = 0; k < BK_COUNT; ++k) batch->shape[k] = s->cfg.tile_shapes[k] <= 2 ? s->cfg.tile_shapes[k] : 0;
   } else {
-    rc = irt_batch_set_views(s->batch, views.data(), (uint32_t)views.size());
-    if (rc) return s->fail(rc, std::string("batch: ") + s->batch->err);
+    rc = irt_batch_set_views(batch, views.data(), (uint32_t)views.size());
+    if (rc) return s->fail(rc, std::string("batch: ") + batch->err);
   }
   if (s->spare_events.empty()) {
     hipEvent_t a = nullptr, b = nullptr;
@@ -336,11 +361,11 @@ int irt_server_step(irt_server* s) {
   st.start = s->spare_events.back().first;
   st.done = s->spare_events.back().second;
   s->spare_events.pop_back();
-  hipError_t e = hipEventRecord(st.start, s->batch->stream);
+  hipError_t e = hipEventRecord(st.start, batch->stream);
   if (e != hipSuccess) return s->fail(IRT_ERR_DEVICE, "hipEventRecord failed");
-  rc = irt_batch_encode(s->batch);
-  if (rc) return s->fail(rc, std::string("batch: ") + s->batch->err);
-  e = hipEventRecord(st.done, s->batch->svgf_stream);  // a step's last work
+  rc = irt_batch_encode(batch);
+  if (rc) return s->fail(rc, std::string("batch: ") + batch->err);
+  e = hipEventRecord(st.done, batch->svgf_stream);  // a step's last work
   if (e != hipSuccess) return s->fail(IRT_ERR_DEVICE, "hipEventRecord failed");
   st.launch_s = server_clock();
   for (auto& m : st.members) {
