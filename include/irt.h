@@ -260,6 +260,52 @@ int irt_set_gbuffer_device(irt_ctx* ctx, const float* depth, const uint16_t* nor
 int irt_set_vpls(irt_ctx* ctx, const irt_vpl* vpls, uint32_t count);
 int irt_set_vpls_device(irt_ctx* ctx, const irt_vpl* vpls, uint32_t count);
 
+/* ------------------------------------------------ VPL pool generation
+ * The producer of the VPL pool (vpl.wgsl:2-47).  The reference traces its
+ * light paths with hardware ray queries against a TLAS of triangle BLASes
+ * (renderer.rs:1061-1106, 1786-1797); gfx950 has no ray-tracing units, so
+ * the library builds a bounding-volume hierarchy per mesh on the host and
+ * walks it in software.  The instance level is the hierarchy of
+ * irt_set_objects: its leaf boxes must contain each object's triangles (the
+ * exported SDF boxes do, they pad the vertex bounds). */
+
+/* Vertex, 64 B (common.wgsl:13; vertex stride 64 at renderer.rs:1096). */
+typedef struct irt_vertex {
+  float position[4];
+  float normal[4];
+  float tangent[4]; /* xyz, w = handedness */
+  float uv[4];
+} irt_vertex;
+
+/* GpuMaterial, 64 B (renderer.rs:31-36, filled at renderer.rs:951-981). */
+typedef struct irt_material {
+  float color[4];      /* base colour rgba */
+  float params[4];     /* roughness, metallic, normal strength, 1 if the base colour texture is sRGB */
+  int32_t textures[4]; /* texture layer of base colour, roughness, metallic, normal map; -1 = none */
+  uint32_t channels[4];/* texel channel read for roughness (y) and metallic (z) */
+} irt_material;
+
+/* Triangle geometry and materials (group 0 bindings 3, 5, 6 of
+ * renderer.rs:1115-1204).  Each mesh of irt_set_meshes finds its vertices
+ * and its mesh-local indices through irt_mesh.geometry (vertex offset and
+ * count, index offset and count); objects name their material in info[1].
+ * Builds the per-mesh hierarchies (the BLAS build of renderer.rs:1061-1106)
+ * and uploads everything; blocking.  Only untextured materials are
+ * accepted (textures[] all -1): the material textures and their sampler
+ * (renderer.rs:951-1030) are not part of this library.  Call after
+ * irt_set_meshes; a later irt_set_meshes drops the geometry. */
+int irt_set_geometry(irt_ctx* ctx, const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
+                     uint32_t index_count, const irt_material* materials, uint32_t material_count);
+
+/* The VPL pass (vpl.wgsl:2-47, dispatched at renderer.rs:1863-1872):
+ * frame.counts.y light paths of up to frame.counts.z surface hits each,
+ * seeded from the frame index; path i writes pool entry i.  The pool goes
+ * to archive slot frame_index % (restir_max_history + 1), as irt_set_vpls
+ * does (the copy of renderer.rs:1873-1884); it replaces irt_set_vpls for the
+ * frame.  Uses the frame's objects, hierarchy and lights.  Enqueued on
+ * irt_stream; call after irt_set_frame and irt_set_geometry. */
+int irt_generate_vpls(irt_ctx* ctx);
+
 /* The six reservoir passes and the two resolves (renderer.rs:1889-1928). */
 int irt_lighting(irt_ctx* ctx);
 /* Svgf::encode (svgf.rs:140-261): temporal, variance, `iterations` a-trous,
@@ -313,7 +359,8 @@ int irt_enable_counters(irt_ctx* ctx, int enable);
 int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
 
 /* Device-to-device copy of src's scene (SDF atlas, meshes, objects,
- * hierarchy, lights, candidate grid) into dst; both contexts must use the
+ * hierarchy, lights, candidate grid, and the triangle geometry of
+ * irt_set_geometry when src has it) into dst; both contexts must use the
  * same precision profile.  Synchronises both contexts' streams.  The
  * reference uploads one scene per Renderer (renderer.rs:713-817); this lets
  * a server give each session its own copy of a prepared scene without a
@@ -477,6 +524,37 @@ int irt_host_pack_frame(const irt_settings* settings, const float vp[16], const 
                         uint32_t light_count, uint32_t object_count, irt_frame* out);
 /* 4x4 column-major inverse (double internally, rounded to float). Returns 0 if singular. */
 int irt_host_inverse(const float m[16], float out[16]);
+
+/* Node of a mesh's triangle hierarchy, 32 B, in preorder.  count == 0: an
+ * inner node whose left child is the next node, link = the node after its
+ * subtree (escape); count > 0: a leaf holding triangles link ..
+ * link + count - 1, whose escape is the next node.  Boxes are the bounds of
+ * their triangles, widened by 4e-5 of their largest coordinate magnitude so
+ * that rounding in the slab test never culls a triangle that is hit. */
+typedef struct irt_tri_node {
+  float lo[3];
+  uint32_t link;
+  float hi[3];
+  uint32_t count;
+} irt_tri_node;
+/* Triangle of a hierarchy leaf, 48 B: its corner positions; prim = index of
+ * the triangle in its mesh (index-buffer order). */
+typedef struct irt_tri {
+  float a[3];
+  uint32_t prim;
+  float b[3];
+  uint32_t pad0;
+  float c[3];
+  uint32_t pad1;
+} irt_tri;
+/* The per-mesh triangle hierarchy irt_set_geometry builds (software form of
+ * the BLAS build, renderer.rs:1061-1106): binned SAH over triangle
+ * centroids, leaves of at most 4 triangles.  indices are mesh-local
+ * (index_count a multiple of 3, each < vertex_count).  `nodes` must hold
+ * max(2*index_count/3 - 1, 1) records and `tris` index_count/3.  Returns the
+ * node count (0 for no triangles) or a negative irt_status. */
+int irt_host_mesh_hierarchy(const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
+                            uint32_t index_count, irt_tri_node* nodes, irt_tri* tris);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,8 @@ def irt() -> ctypes.CDLL:
         "irt_set_gbuffer_device": (c_int, [P, P, P, P, P, P]),
         "irt_set_vpls": (c_int, [P, P, u32]),
         "irt_set_vpls_device": (c_int, [P, P, u32]),
+        "irt_set_geometry": (c_int, [P, P, u32, P, u32, P, u32]),
+        "irt_generate_vpls": (c_int, [P]),
         "irt_lighting": (c_int, [P]),
         "irt_svgf": (c_int, [P, u32, u32]),
         "irt_end_frame": (c_int, [P]),
@@ -109,6 +111,7 @@ def irt() -> ctypes.CDLL:
         "irt_host_atlas_grid": (None, [u32, u32, P]),
         "irt_host_pack_frame": (c_int, [POINTER(Settings), P, P, P, u32, u32, u32, u32, u32, u32, u32, P]),
         "irt_host_inverse": (c_int, [P, P]),
+        "irt_host_mesh_hierarchy": (c_int, [P, u32, P, u32, P, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -137,6 +140,8 @@ def scenegen() -> ctypes.CDLL:
         "sg_camera_matrix": (None, [P, c_float, P]),
         "sg_render_gbuffer": (c_int, [P, P, P, u32, u32, P, P, P, P, P]),
         "sg_vpls": (c_int, [P, c_uint64, u32, u32, c_float, P]),
+        "sg_geometry_info": (None, [P, P]),
+        "sg_geometry": (None, [P, P, P, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)

@@ -13,6 +13,7 @@
 
 #include "irt_host.h"
 #include "irt_kernels.h"
+#include "irt_vpl.h"
 
 using namespace irt;
 
@@ -89,6 +90,21 @@ struct irt_ctx {
   bool have_frame = false;
   std::vector<irt_mesh> host_meshes;     // host copies for launch-time validation
   uint32_t max_mesh_ref = 0;             // largest mesh index referenced by an object
+  uint32_t max_material_ref = 0;         // largest material index referenced by an object
+  // triangle geometry of the VPL pass (irt_set_geometry)
+  irt_vertex* vertices = nullptr;
+  uint32_t n_vertices = 0, cap_vertices = 0;
+  uint32_t* indices = nullptr;
+  uint32_t n_indices = 0, cap_indices = 0;
+  irt_material* materials = nullptr;
+  uint32_t n_materials = 0, cap_materials = 0;
+  irt_tri_node* tri_nodes = nullptr;
+  uint32_t n_tri_nodes = 0, cap_tri_nodes = 0;
+  irt_tri* tris = nullptr;
+  uint32_t n_tris = 0, cap_tris = 0;
+  uint2* mesh_nodes = nullptr;  // per mesh: [first, end) of its triangle hierarchy
+  uint32_t cap_mesh_nodes = 0;
+  bool have_geometry = false;
   // G-buffer slots (internal formats).  The current and previous frames'
   // G-buffers are slots gb_cur / gb_prev (the same slot right after a frame
   // ends, as when the reference copies the G-buffer); a write into the
@@ -1032,6 +1048,7 @@ int irt_set_meshes(irt_ctx* c, const irt_mesh* meshes, uint32_t count) {
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
   c->n_meshes = count;
   c->host_meshes.assign(meshes, meshes + count);
+  c->have_geometry = false;  // the triangle hierarchies follow the mesh records
   c->mesh_k_dirty = true;
   c->grid_dirty = true;
   return IRT_OK;
@@ -1042,11 +1059,12 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!objects || !nodes || count == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "no objects");
   if (node_count != 2 * count - 1) return c->fail(IRT_ERR_INVALID_ARGUMENT, "node count must be 2*objects-1");
-  uint32_t max_ref = 0;
+  uint32_t max_ref = 0, max_material = 0;
   for (uint32_t i = 0; i < count; ++i) {
     if (c->n_meshes && objects[i].info[0] >= c->n_meshes)
       return c->fail(IRT_ERR_INVALID_ARGUMENT, "object references a missing mesh");
     max_ref = std::max(max_ref, objects[i].info[0]);
+    max_material = std::max(max_material, objects[i].info[1]);
   }
   for (uint32_t i = 0; i < node_count; ++i) {
     const irt_node& nd = nodes[i];
@@ -1064,6 +1082,7 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   c->n_objects = count;
   c->n_nodes = node_count;
   c->max_mesh_ref = max_ref;
+  c->max_material_ref = max_material;
   // Object world boxes (the hierarchy's leaves) and the grid over the root box.
   std::vector<float4> boxes(2 * (size_t)count, make_float4(0.f, 0.f, 0.f, 0.f));
   for (uint32_t i = 0; i < node_count; ++i)
@@ -1160,6 +1179,33 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
   dst->n_lights = src->n_lights;
   dst->host_meshes = src->host_meshes;
   dst->max_mesh_ref = src->max_mesh_ref;
+  dst->max_material_ref = src->max_material_ref;
+  dst->have_geometry = false;
+  if (src->have_geometry) {
+    rc = ensure_array(dst, &dst->vertices, &dst->cap_vertices, src->n_vertices);
+    if (!rc) rc = ensure_array(dst, &dst->indices, &dst->cap_indices, src->n_indices);
+    if (!rc) rc = ensure_array(dst, &dst->materials, &dst->cap_materials, src->n_materials);
+    if (!rc) rc = ensure_array(dst, &dst->tri_nodes, &dst->cap_tri_nodes, src->n_tri_nodes);
+    if (!rc) rc = ensure_array(dst, &dst->tris, &dst->cap_tris, src->n_tris);
+    if (!rc) rc = ensure_array(dst, &dst->mesh_nodes, &dst->cap_mesh_nodes, src->n_meshes);
+    if (rc) return rc;
+    const std::pair<void*, std::pair<const void*, size_t>> parts[] = {
+        {dst->vertices, {src->vertices, sizeof(irt_vertex) * src->n_vertices}},
+        {dst->indices, {src->indices, sizeof(uint32_t) * src->n_indices}},
+        {dst->materials, {src->materials, sizeof(irt_material) * src->n_materials}},
+        {dst->tri_nodes, {src->tri_nodes, sizeof(irt_tri_node) * src->n_tri_nodes}},
+        {dst->tris, {src->tris, sizeof(irt_tri) * src->n_tris}},
+        {dst->mesh_nodes, {src->mesh_nodes, sizeof(uint2) * src->n_meshes}}};
+    for (const auto& p : parts)
+      if (p.second.second)
+        HIP_CHECK(dst, hipMemcpyAsync(p.first, p.second.first, p.second.second, hipMemcpyDeviceToDevice, dst->stream));
+    dst->n_vertices = src->n_vertices;
+    dst->n_indices = src->n_indices;
+    dst->n_materials = src->n_materials;
+    dst->n_tri_nodes = src->n_tri_nodes;
+    dst->n_tris = src->n_tris;
+    dst->have_geometry = true;
+  }
   dst->grid = src->grid;
   dst->grid.cells = nullptr;
   dst->grid.boxes = dst->obj_boxes;
@@ -1317,6 +1363,88 @@ static int set_vpls_impl(irt_ctx* c, const irt_vpl* v, uint32_t count, hipMemcpy
 int irt_set_vpls(irt_ctx* c, const irt_vpl* v, uint32_t count) { return set_vpls_impl(c, v, count, hipMemcpyHostToDevice); }
 int irt_set_vpls_device(irt_ctx* c, const irt_vpl* v, uint32_t count) {
   return set_vpls_impl(c, v, count, hipMemcpyDeviceToDevice);
+}
+
+int irt_set_geometry(irt_ctx* c, const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
+                     uint32_t index_count, const irt_material* materials, uint32_t material_count) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if ((vertex_count && !vertices) || (index_count && !indices) || !materials || material_count == 0)
+    return c->fail(IRT_ERR_INVALID_ARGUMENT, "null geometry array or no materials");
+  if (c->host_meshes.empty()) return c->fail(IRT_ERR_STATE, "set the meshes before their geometry");
+  for (uint32_t i = 0; i < material_count; ++i)
+    for (int k = 0; k < 4; ++k)
+      if (materials[i].textures[k] >= 0)
+        return c->fail(IRT_ERR_INVALID_ARGUMENT, "textured materials are not supported (textures must be -1)");
+  std::vector<irt_tri_node> all_nodes, nodes;
+  std::vector<irt_tri> all_tris, tris;
+  std::vector<uint2> ranges(c->n_meshes);
+  for (uint32_t m = 0; m < c->n_meshes; ++m) {
+    const uint32_t* g = c->host_meshes[m].geometry;
+    if ((uint64_t)g[0] + g[1] > vertex_count || (uint64_t)g[2] + g[3] > index_count)
+      return c->fail(IRT_ERR_INVALID_ARGUMENT, "mesh geometry lies outside the vertex or index arrays");
+    std::string msg;
+    const int rc = irt_host::mesh_hierarchy(vertices + g[0], g[1], indices + g[2], g[3], nodes, tris, &msg);
+    if (rc) return c->fail(rc, msg);
+    const uint32_t node_base = (uint32_t)all_nodes.size(), tri_base = (uint32_t)all_tris.size();
+    for (irt_tri_node nd : nodes) {
+      nd.link += nd.count ? tri_base : node_base;
+      all_nodes.push_back(nd);
+    }
+    all_tris.insert(all_tris.end(), tris.begin(), tris.end());
+    ranges[m] = make_uint2(node_base, (uint32_t)all_nodes.size());
+  }
+  int rc = ensure_array(c, &c->vertices, &c->cap_vertices, vertex_count);
+  if (!rc) rc = ensure_array(c, &c->indices, &c->cap_indices, index_count);
+  if (!rc) rc = ensure_array(c, &c->materials, &c->cap_materials, material_count);
+  if (!rc) rc = ensure_array(c, &c->tri_nodes, &c->cap_tri_nodes, (uint32_t)all_nodes.size());
+  if (!rc) rc = ensure_array(c, &c->tris, &c->cap_tris, (uint32_t)all_tris.size());
+  if (!rc) rc = ensure_array(c, &c->mesh_nodes, &c->cap_mesh_nodes, c->n_meshes);
+  if (rc) return rc;
+  if (vertex_count)
+    HIP_CHECK(c, hipMemcpyAsync(c->vertices, vertices, sizeof(irt_vertex) * vertex_count, hipMemcpyHostToDevice,
+                                c->stream));
+  if (index_count)
+    HIP_CHECK(c, hipMemcpyAsync(c->indices, indices, sizeof(uint32_t) * index_count, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->materials, materials, sizeof(irt_material) * material_count, hipMemcpyHostToDevice,
+                              c->stream));
+  if (!all_nodes.empty())
+    HIP_CHECK(c, hipMemcpyAsync(c->tri_nodes, all_nodes.data(), sizeof(irt_tri_node) * all_nodes.size(),
+                                hipMemcpyHostToDevice, c->stream));
+  if (!all_tris.empty())
+    HIP_CHECK(c, hipMemcpyAsync(c->tris, all_tris.data(), sizeof(irt_tri) * all_tris.size(), hipMemcpyHostToDevice,
+                                c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->mesh_nodes, ranges.data(), sizeof(uint2) * ranges.size(), hipMemcpyHostToDevice,
+                              c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->n_vertices = vertex_count;
+  c->n_indices = index_count;
+  c->n_materials = material_count;
+  c->n_tri_nodes = (uint32_t)all_nodes.size();
+  c->n_tris = (uint32_t)all_tris.size();
+  c->have_geometry = true;
+  return IRT_OK;
+}
+
+int irt_generate_vpls(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "set the frame before generating its VPL pool");
+  if (!c->have_geometry) return c->fail(IRT_ERR_STATE, "no triangle geometry (irt_set_geometry)");
+  if (!c->n_objects || !c->n_nodes) return c->fail(IRT_ERR_STATE, "no objects");
+  const irt_frame& F = c->frame;
+  if (F.counts[1] != c->V) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame VPL count differs from settings.vpl_count");
+  if (F.counts[0] > c->n_lights) return c->fail(IRT_ERR_INVALID_ARGUMENT, "frame light count exceeds the lights set");
+  if (F.counts[2] < 1 || F.counts[2] > 32) return c->fail(IRT_ERR_INVALID_ARGUMENT, "invalid VPL bounce count");
+  if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "object references a missing mesh");
+  if (c->max_material_ref >= c->n_materials) return c->fail(IRT_ERR_STATE, "object references a missing material");
+  const uint32_t slot = F.size[2] % c->slots;
+  TriScene sc{c->objects, c->nodes,    c->meshes,    c->mesh_nodes, c->tri_nodes, c->tris,
+              c->vertices, c->indices, c->materials, c->lights,     c->n_nodes};
+  timer_begin(c, "vpl_generate");
+  hipLaunchKernelGGL(k_generate_vpls, dim3((c->V + 63) / 64), dim3(64), 0, c->stream, sc, F,
+                     c->ring + (size_t)slot * c->V);
+  HIP_CHECK(c, hipGetLastError());
+  timer_end(c);
+  return IRT_OK;
 }
 
 int irt_lighting(irt_ctx* c) {

@@ -217,10 +217,10 @@ __device__ __forceinline__ float pow64(float x) {
 }
 
 // brdf — common.wgsl:67-74.
-// The camera-dependent per-pixel terms of brdf for s (wo = normalize(camera
-// - p), n.v, alpha^2, the view Smith term, 1 - metal and F0).
-__device__ __forceinline__ void surface_view(Surface& s, v3 camera) {
-  s.wo = normalize(camera - s.p);
+// The wo-dependent terms of brdf for s (wo, n.v, alpha^2, the view Smith
+// term, 1 - metal and F0); surface_view() takes wo = normalize(camera - p).
+__device__ __forceinline__ void surface_toward(Surface& s, v3 wo) {
+  s.wo = wo;
   s.nv = fmaxf(dot(s.n, s.wo), 0.00001f);
   const float a = s.rough * s.rough;
   s.a2 = a * a;
@@ -229,6 +229,7 @@ __device__ __forceinline__ void surface_view(Surface& s, v3 camera) {
   s.f0 = v3{0.04f * s.im + s.color.x * s.metal, 0.04f * s.im + s.color.y * s.metal,
             0.04f * s.im + s.color.z * s.metal};
 }
+__device__ __forceinline__ void surface_view(Surface& s, v3 camera) { surface_toward(s, normalize(camera - s.p)); }
 
 // brdf toward wi for a surface whose view terms surface_view() filled.
 __device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {

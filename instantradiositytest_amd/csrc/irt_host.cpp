@@ -1,7 +1,8 @@
 // Host-side precompute of the path (the CPU work renderer.rs does before it
 // uploads the scene): settings validation, box certification, Lipschitz
 // bound of a sampled field, the object BVH with escape links, SDF atlas
-// placement, binary16 conversion for the reference texture formats.
+// placement, binary16 conversion for the reference texture formats, and the
+// per-mesh triangle hierarchies the VPL pass walks.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -347,6 +348,174 @@ float f16_to_f32(uint16_t h) {
   return f;
 }
 
+// ------------------------------------------------ mesh triangle hierarchy
+// The software counterpart of a triangle BLAS (renderer.rs:1061-1106): a
+// binary hierarchy over a mesh's triangles in preorder with escape links,
+// walked without a stack by the VPL kernel.
+namespace {
+struct TriRef {
+  float lo[3], hi[3], c[3];
+  uint32_t prim;
+};
+constexpr uint32_t TRI_LEAF = 4;  // triangles per leaf at most
+constexpr int TRI_BINS = 16;
+
+float half_area(const float lo[3], const float hi[3]) {
+  const float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+  return e0 * e1 + e1 * e2 + e2 * e0;
+}
+
+// Binned SAH split on the widest centroid axis (median split when the
+// centroids coincide, the best bin leaves one side empty, or the tree is
+// deep); leaves of at most TRI_LEAF triangles.  Returns the node's index.
+uint32_t build_tris(TriRef* t, size_t n, int depth, std::vector<irt_tri_node>& nodes, std::vector<uint32_t>& order) {
+  const uint32_t index = (uint32_t)nodes.size();
+  nodes.push_back(irt_tri_node{});
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], t[i].lo[a]);
+      hi[a] = std::max(hi[a], t[i].hi[a]);
+      clo[a] = std::min(clo[a], t[i].c[a]);
+      chi[a] = std::max(chi[a], t[i].c[a]);
+    }
+  uint32_t link = 0, count = 0;
+  if (n <= TRI_LEAF) {
+    link = (uint32_t)order.size();
+    count = (uint32_t)n;
+    for (size_t i = 0; i < n; ++i) order.push_back(t[i].prim);
+  } else {
+    const float ce[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+    const int axis = (ce[0] >= ce[1] && ce[0] >= ce[2]) ? 0 : (ce[1] >= ce[2] ? 1 : 2);
+    size_t mid = 0;
+    if (ce[axis] > 0.f && depth < 48) {
+      auto bin_of = [&](const TriRef& r) {
+        int b = (int)((r.c[axis] - clo[axis]) / ce[axis] * (float)TRI_BINS);
+        return std::min(std::max(b, 0), TRI_BINS - 1);
+      };
+      size_t cnt[TRI_BINS] = {};
+      float blo[TRI_BINS][3], bhi[TRI_BINS][3];
+      for (int b = 0; b < TRI_BINS; ++b)
+        for (int a = 0; a < 3; ++a) {
+          blo[b][a] = INFINITY;
+          bhi[b][a] = -INFINITY;
+        }
+      for (size_t i = 0; i < n; ++i) {
+        const int b = bin_of(t[i]);
+        ++cnt[b];
+        for (int a = 0; a < 3; ++a) {
+          blo[b][a] = std::min(blo[b][a], t[i].lo[a]);
+          bhi[b][a] = std::max(bhi[b][a], t[i].hi[a]);
+        }
+      }
+      // cost of splitting after bin s: area(left) * n_left + area(right) * n_right
+      float best = INFINITY;
+      int best_s = -1;
+      for (int s = 0; s < TRI_BINS - 1; ++s) {
+        float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        size_t nl = 0, nr = 0;
+        for (int b = 0; b < TRI_BINS; ++b) {
+          if (!cnt[b]) continue;
+          float* L = b <= s ? llo : rlo;
+          float* Hh = b <= s ? lhi : rhi;
+          for (int a = 0; a < 3; ++a) {
+            L[a] = std::min(L[a], blo[b][a]);
+            Hh[a] = std::max(Hh[a], bhi[b][a]);
+          }
+          (b <= s ? nl : nr) += cnt[b];
+        }
+        if (!nl || !nr) continue;
+        const float cost = half_area(llo, lhi) * (float)nl + half_area(rlo, rhi) * (float)nr;
+        if (cost < best) {
+          best = cost;
+          best_s = s;
+        }
+      }
+      if (best_s >= 0)
+        mid = (size_t)(std::partition(t, t + n, [&](const TriRef& r) { return bin_of(r) <= best_s; }) - t);
+    }
+    if (mid == 0 || mid == n) {
+      mid = n / 2;
+      std::nth_element(t, t + mid, t + n, [axis](const TriRef& p, const TriRef& q) {
+        return p.c[axis] < q.c[axis] || (p.c[axis] == q.c[axis] && p.prim < q.prim);
+      });
+    }
+    build_tris(t, mid, depth + 1, nodes, order);
+    build_tris(t + mid, n - mid, depth + 1, nodes, order);
+    link = (uint32_t)nodes.size();  // escape: the node after this subtree
+  }
+  irt_tri_node& nd = nodes[index];
+  float mag = 0.f;
+  for (int a = 0; a < 3; ++a) mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+  const float pad = mag * 4e-5f;
+  for (int a = 0; a < 3; ++a) {
+    nd.lo[a] = lo[a] - pad;
+    nd.hi[a] = hi[a] + pad;
+  }
+  nd.link = link;
+  nd.count = count;
+  return index;
+}
+}  // namespace
+
+int mesh_hierarchy(const irt_vertex* v, uint32_t nv, const uint32_t* idx, uint32_t ni,
+                   std::vector<irt_tri_node>& nodes, std::vector<irt_tri>& tris, std::string* msg) {
+  nodes.clear();
+  tris.clear();
+  if (ni % 3u) {
+    *msg = "mesh index count is not a multiple of 3";
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  const uint32_t nt = ni / 3u;
+  std::vector<TriRef> refs(nt);
+  for (uint32_t p = 0; p < nt; ++p) {
+    TriRef& r = refs[p];
+    r.prim = p;
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = INFINITY;
+      r.hi[a] = -INFINITY;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t i = idx[3 * p + k];
+      if (i >= nv) {
+        *msg = "mesh index refers past the mesh's vertices";
+        return IRT_ERR_INVALID_ARGUMENT;
+      }
+      for (int a = 0; a < 3; ++a) {
+        r.lo[a] = std::min(r.lo[a], v[i].position[a]);
+        r.hi[a] = std::max(r.hi[a], v[i].position[a]);
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(r.lo[a]) || !std::isfinite(r.hi[a])) {
+        *msg = "vertex position is not finite";
+        return IRT_ERR_INVALID_ARGUMENT;
+      }
+      r.c[a] = 0.5f * (r.lo[a] + r.hi[a]);
+    }
+  }
+  if (nt == 0) return IRT_OK;
+  std::vector<uint32_t> order;
+  order.reserve(nt);
+  nodes.reserve(2 * (size_t)nt);
+  build_tris(refs.data(), refs.size(), 0, nodes, order);
+  tris.resize(nt);
+  for (uint32_t k = 0; k < nt; ++k) {
+    const uint32_t p = order[k];
+    irt_tri& t = tris[k];
+    std::memset(&t, 0, sizeof(t));
+    t.prim = p;
+    for (int a = 0; a < 3; ++a) {
+      t.a[a] = v[idx[3 * p]].position[a];
+      t.b[a] = v[idx[3 * p + 1]].position[a];
+      t.c[a] = v[idx[3 * p + 2]].position[a];
+    }
+  }
+  return IRT_OK;
+}
+
 }  // namespace irt_host
 
 // ------------------------------------------------------------- C ABI part
@@ -406,6 +575,25 @@ int irt_host_pack_frame(const irt_settings* s, const float vp[16], const float p
 int irt_host_inverse(const float m[16], float out[16]) {
   if (!m || !out) return 0;
   return irt_host::inverse4(m, out) ? 1 : 0;
+}
+
+int irt_host_mesh_hierarchy(const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
+                            uint32_t index_count, irt_tri_node* nodes, irt_tri* tris) {
+  if ((index_count && (!vertices || !indices || !nodes || !tris))) {
+    irt_host::set_global_error("null argument");
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  std::vector<irt_tri_node> n;
+  std::vector<irt_tri> t;
+  std::string msg;
+  int rc = irt_host::mesh_hierarchy(vertices, vertex_count, indices, index_count, n, t, &msg);
+  if (rc) {
+    irt_host::set_global_error(msg);
+    return rc;
+  }
+  std::copy(n.begin(), n.end(), nodes);
+  std::copy(t.begin(), t.end(), tris);
+  return (int)n.size();
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 #include <array>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/irt.h"
 
@@ -25,5 +26,9 @@ int pack_frame(const irt_settings& s, const float* vp, const float* pvp, const f
 uint16_t f32_to_f16(float f);
 float f16_to_f32(uint16_t h);
 uint16_t f32_to_f16_down(float f);
+// Triangle hierarchy of one mesh (mesh-local indices); node and triangle
+// links are local to the returned arrays.
+int mesh_hierarchy(const irt_vertex* v, uint32_t nv, const uint32_t* idx, uint32_t ni,
+                   std::vector<irt_tri_node>& nodes, std::vector<irt_tri>& tris, std::string* msg);
 
 }  // namespace irt_host

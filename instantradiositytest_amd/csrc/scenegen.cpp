@@ -495,6 +495,79 @@ void sg_records(const sg_scene* s, irt_mesh* meshes, irt_object* objects, irt_no
   }
 }
 
+// Triangle geometry in the reference's vertex and material records (the
+// vertex buffer and materials of renderer.rs:951-981, 1061-1106):
+// info = vertex count, index count, material count.
+void sg_geometry_info(const sg_scene* s, uint32_t info[3]) {
+  const Scene* sc = reinterpret_cast<const Scene*>(s);
+  uint32_t nv = 0, ni = 0;
+  for (const irt_mesh& m : sc->meshes) {
+    nv += m.geometry[1];
+    ni += m.geometry[3];
+  }
+  info[0] = nv;
+  info[1] = ni;
+  info[2] = (uint32_t)sc->objects.size();
+}
+
+// Vertices carry the shape's analytic normal (boxes: the face normal of the
+// one triangle each vertex belongs to), a unit tangent orthogonal to it and
+// a (u, v) parameter; indices are mesh-local.  Material i is object i's
+// untextured albedo / roughness / metalness (objects use material index i).
+void sg_geometry(const sg_scene* s, irt_vertex* vertices, uint32_t* indices, irt_material* materials) {
+  const Scene* sc = reinterpret_cast<const Scene*>(s);
+  for (size_t m = 0; m < sc->shapes.size(); ++m) {
+    const sg_shape& shape = sc->shapes[m];
+    const irt_mesh& g = sc->meshes[m];
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    tessellate(shape, pos, idx);
+    const uint32_t nv = (uint32_t)(pos.size() / 3);
+    std::vector<d3> normal(nv);
+    if (shape.kind == SG_BOX) {
+      for (size_t t = 0; t + 2 < idx.size(); t += 3) {
+        d3 c = {0, 0, 0};
+        for (int k = 0; k < 3; ++k)
+          c = c + d3{pos[3 * idx[t + k]], pos[3 * idx[t + k] + 1], pos[3 * idx[t + k] + 2]} * (1.0 / 3.0);
+        const d3 n = normal_local(shape, c);
+        for (int k = 0; k < 3; ++k) normal[idx[t + k]] = n;
+      }
+    } else {
+      for (uint32_t v = 0; v < nv; ++v) normal[v] = normal_local(shape, d3{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]});
+    }
+    for (uint32_t v = 0; v < nv; ++v) {
+      irt_vertex& out = vertices[g.geometry[0] + v];
+      std::memset(&out, 0, sizeof(out));
+      const d3 n = normal[v];
+      const d3 axis = std::fabs(n.y) < 0.9 ? d3{0, 1, 0} : d3{1, 0, 0};
+      const d3 t = nrm(cross(axis, n));
+      for (int a = 0; a < 3; ++a) out.position[a] = pos[3 * v + a];
+      out.position[3] = 1.0f;
+      out.normal[0] = (float)n.x;
+      out.normal[1] = (float)n.y;
+      out.normal[2] = (float)n.z;
+      out.tangent[0] = (float)t.x;
+      out.tangent[1] = (float)t.y;
+      out.tangent[2] = (float)t.z;
+      out.tangent[3] = 1.0f;
+      out.uv[0] = shape.kind == SG_BOX ? 0.0f : (float)((v % 32u) / 32.0);
+      out.uv[1] = shape.kind == SG_BOX ? 0.0f : (float)((v / 32u) / 16.0);
+    }
+    std::copy(idx.begin(), idx.end(), indices + g.geometry[2]);
+  }
+  for (size_t i = 0; i < sc->objects.size(); ++i) {
+    irt_material& mat = materials[i];
+    std::memset(&mat, 0, sizeof(mat));
+    const sg_object& o = sc->objects[i];
+    for (int a = 0; a < 3; ++a) mat.color[a] = o.albedo[a];
+    mat.color[3] = 1.0f;
+    mat.params[0] = o.rough;
+    mat.params[1] = o.metal;
+    mat.params[2] = 1.0f;
+    for (int k = 0; k < 4; ++k) mat.textures[k] = -1;
+  }
+}
+
 void sg_atlas(const sg_scene* s, float* out) {
   const Scene* sc = reinterpret_cast<const Scene*>(s);
   std::copy(sc->atlas.begin(), sc->atlas.end(), out);

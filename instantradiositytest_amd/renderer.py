@@ -116,6 +116,17 @@ class Renderer:
         lights = scene.lights if len(scene.lights) else np.zeros(0, dtype=scene.lights.dtype)
         check(L.irt_set_lights(ctx, ptr(lights) if len(lights) else None, len(lights)), ctx)
 
+    def set_geometry(self, vertices: np.ndarray, indices: np.ndarray, materials: np.ndarray):
+        """Triangle geometry and materials of the VPL pass (irt_set_geometry)."""
+        v = np.ascontiguousarray(vertices)
+        i = np.ascontiguousarray(indices, dtype=np.uint32)
+        m = np.ascontiguousarray(materials)
+        check(self.L.irt_set_geometry(self.ctx, ptr(v), len(v), ptr(i), len(i), ptr(m), len(m)), self.ctx)
+
+    def generate_vpls(self):
+        """The VPL pass of the frame set last (irt_generate_vpls, vpl.wgsl:2-47)."""
+        check(self.L.irt_generate_vpls(self.ctx), self.ctx)
+
     def schedule(self, name: str):
         check(self.L.irt_set_schedule(self.ctx, SCHEDULES[name]), self.ctx)
         self.schedule_name = name
@@ -133,7 +144,10 @@ class Renderer:
         check(L.irt_set_frame(ctx, ptr(f)), ctx)
         check(L.irt_set_gbuffer(ctx, ptr(inputs.depth), ptr(inputs.normal), ptr(inputs.color), ptr(inputs.params),
                                 ptr(inputs.motion)), ctx)
-        check(L.irt_set_vpls(ctx, ptr(inputs.vpls), len(inputs.vpls)), ctx)
+        if inputs.vpls is None:
+            self.generate_vpls()
+        else:
+            check(L.irt_set_vpls(ctx, ptr(inputs.vpls), len(inputs.vpls)), ctx)
         self.last_frame = f
         self._pending = (vp, scene.generation)
         return f

@@ -21,8 +21,8 @@ from dataclasses import dataclass, field, replace
 import numpy as np
 
 from . import _lib
-from .types import (CAMERA_DT, LIGHT_DT, MESH_DT, NODE_DT, OBJECT_DT, SG_BOX, SG_SPHERE, SG_TORUS, SGOBJECT_DT,
-                    SHAPE_DT, VPL_DT, make_settings, ptr)
+from .types import (CAMERA_DT, LIGHT_DT, MATERIAL_DT, MESH_DT, NODE_DT, OBJECT_DT, SG_BOX, SG_SPHERE, SG_TORUS,
+                    SGOBJECT_DT, SHAPE_DT, VERTEX_DT, VPL_DT, make_settings, ptr)
 
 ROOM_HALF = (2.0, 1.5, 2.5)  # x half-width, half-height, z half-depth of the 4x3x5 m room
 ROOM_CENTER_Y = 1.5
@@ -299,6 +299,17 @@ class Scene:
         self._sg.sg_render_gbuffer(self.handle, ptr(cam), ptr(pcam), width, height, ptr(depth), ptr(normal),
                                    ptr(color), ptr(params), ptr(motion))
         return depth, normal, color, params, motion
+
+    def geometry(self):
+        """Triangle geometry in the reference's records: (vertices VERTEX_DT, mesh-local
+        indices uint32, materials MATERIAL_DT); object i uses material i."""
+        info = np.zeros(3, dtype=np.uint32)
+        self._sg.sg_geometry_info(self.handle, ptr(info))
+        vertices = np.zeros(int(info[0]), dtype=VERTEX_DT)
+        indices = np.zeros(int(info[1]), dtype=np.uint32)
+        materials = np.zeros(int(info[2]), dtype=MATERIAL_DT)
+        self._sg.sg_geometry(self.handle, ptr(vertices), ptr(indices), ptr(materials))
+        return vertices, indices, materials
 
     def vpl_pool(self, seed: int, paths: int, bounces: int, flux_scale: float = 1.0) -> np.ndarray:
         out = np.zeros(paths * bounces, dtype=VPL_DT)

@@ -32,7 +32,17 @@ RESERVOIR_DT = np.dtype([
     ("sum", "<f4"), ("selected_target", "<f4"), ("pdf", "<f4"), ("generation", "<u4"),
 ])
 
+VERTEX_DT = np.dtype([("position", F4x4), ("normal", F4x4), ("tangent", F4x4), ("uv", F4x4)])
+MATERIAL_DT = np.dtype([("color", F4x4), ("params", F4x4), ("textures", ("<i4", (4,))), ("channels", U4x4)])
+TRI_NODE_DT = np.dtype([("lo", "<f4", (3,)), ("link", "<u4"), ("hi", "<f4", (3,)), ("count", "<u4")])
+TRI_DT = np.dtype([("a", "<f4", (3,)), ("prim", "<u4"), ("b", "<f4", (3,)), ("pad0", "<u4"),
+                   ("c", "<f4", (3,)), ("pad1", "<u4")])
+
 assert FRAME_DT.itemsize == 384
+assert VERTEX_DT.itemsize == 64
+assert MATERIAL_DT.itemsize == 64
+assert TRI_NODE_DT.itemsize == 32
+assert TRI_DT.itemsize == 48
 assert OBJECT_DT.itemsize == 208
 assert MESH_DT.itemsize == 128
 assert LIGHT_DT.itemsize == 48

@@ -96,6 +96,13 @@ def lib() -> ctypes.CDLL:
     L.oracle_f32_to_f16.argtypes = [c_float]
     L.oracle_f16_to_f32.restype = c_float
     L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
+    L.oracle_generate_vpls.restype = c_int
+    L.oracle_generate_vpls.argtypes = [P, P, P, P, c_int]
+    L.oracle_trace.restype = c_int
+    L.oracle_trace.argtypes = [P, P, P, P, POINTER(c_float), POINTER(c_float), POINTER(c_float),
+                               POINTER(c_uint32), POINTER(c_uint32)]
+    L.oracle_sample_bsdf.restype = None
+    L.oracle_sample_bsdf.argtypes = [P, P, c_float, c_float, P, POINTER(c_uint32), P, P]
     _L = L
     return L
 
@@ -128,6 +135,59 @@ class SceneRecords:
     @staticmethod
     def from_scene(scene) -> "SceneRecords":
         return SceneRecords(scene.objects, scene.meshes, scene.nodes, scene.lights, scene.atlas())
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("vertices", c_void_p), ("indices", c_void_p), ("materials", c_void_p), ("n_vertices", c_uint32),
+                ("n_indices", c_uint32), ("n_materials", c_uint32)]
+
+
+class GeometryRecords:
+    """Triangle geometry of the VPL pass held for the oracle: vertex records
+    (64 B: position, normal, tangent, uv), mesh-local uint32 indices and
+    material records (64 B)."""
+
+    def __init__(self, vertices, indices, materials):
+        self.vertices = np.ascontiguousarray(vertices)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.materials = np.ascontiguousarray(materials)
+        assert self.vertices.itemsize == 64 and self.materials.itemsize == 64
+        self.c = Geometry(_p(self.vertices), _p(self.indices), _p(self.materials), len(self.vertices),
+                          len(self.indices), len(self.materials))
+
+
+def generate_vpls(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRecords, threads: int = 0) -> np.ndarray:
+    """vpl.wgsl:2-47 for the frame: float32 [counts.y, 12] records (position, normal, flux)."""
+    f = np.ascontiguousarray(frame)
+    n = int(np.frombuffer(f.tobytes(), np.uint32)[73])  # counts.y (counts at byte 288)
+    out = np.zeros((n, 12), dtype=np.float32)
+    rc = lib().oracle_generate_vpls(_p(f), ctypes.byref(scene.c), ctypes.byref(geometry.c), _p(out), threads)
+    if rc != 0:
+        raise ValueError("oracle_generate_vpls: bad arguments")
+    return out
+
+
+def trace(scene: SceneRecords, geometry: GeometryRecords, origin, direction):
+    """Closest hit of one world-space ray: (hit, t, u, v, object, prim)."""
+    o = np.ascontiguousarray(origin, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    t, u, v = c_float(), c_float(), c_float()
+    obj, prim = c_uint32(), c_uint32()
+    hit = lib().oracle_trace(ctypes.byref(scene.c), ctypes.byref(geometry.c), _p(o), _p(d), ctypes.byref(t),
+                             ctypes.byref(u), ctypes.byref(v), ctypes.byref(obj), ctypes.byref(prim))
+    return bool(hit), t.value, u.value, v.value, obj.value, prim.value
+
+
+def sample_bsdf(n, color, rough: float, metal: float, wo, state: int):
+    """sample_bsdf (common.wgsl:86-97): (direction, weight, next PCG state)."""
+    nn = np.ascontiguousarray(n, np.float32)
+    cc = np.ascontiguousarray(color, np.float32)
+    w = np.ascontiguousarray(wo, np.float32)
+    st = c_uint32(state & 0xFFFFFFFF)
+    direction = np.zeros(3, np.float32)
+    weight = np.zeros(3, np.float32)
+    lib().oracle_sample_bsdf(_p(nn), _p(cc), rough, metal, _p(w), ctypes.byref(st), _p(direction), _p(weight))
+    return direction, weight, st.value
 
 
 class GBufferPlanes:

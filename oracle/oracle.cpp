@@ -885,6 +885,201 @@ int add_impl(oracle_reservoir* r, uint32_t id, uint32_t source, double target, d
   return taken;
 }
 
+// ------------------------------------------------------------ VPL pass
+// vpl.wgsl:2-47 in FP32.  The ray query's closest hit is found by testing
+// every triangle of every object (no hierarchy): triangles both-sided, t in
+// [0.0005, 10000] (vpl.wgsl:18), Moller-Trumbore intersection, equal
+// distances to the lower (object, triangle) pair.
+using Vf = Vec<float>;
+
+Vf cross3(Vf a, Vf b) { return V<float>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// column-major 4x4 times (x, y, z, w), row by row in column order
+Vf xform(const float* m, float x, float y, float z, float w) {
+  return V<float>(m[0] * x + m[4] * y + m[8] * z + m[12] * w, m[1] * x + m[5] * y + m[9] * z + m[13] * w,
+                  m[2] * x + m[6] * y + m[10] * z + m[14] * w);
+}
+
+struct Hit {
+  float t, u, v;
+  uint32_t object, prim;
+};
+
+bool closer(float t, uint32_t object, uint32_t prim, const Hit& h) {
+  return t < h.t || (t == h.t && (object < h.object || (object == h.object && prim < h.prim)));
+}
+
+void ray_triangle(Vf o, Vf d, Vf a, Vf b, Vf c, uint32_t object, uint32_t prim, Hit* h) {
+  const Vf e1 = sub3(b, a), e2 = sub3(c, a);
+  const Vf p = cross3(d, e2);
+  const float det = dot3(e1, p);
+  if (det == 0.0f) return;
+  const float inv = 1.0f / det;
+  const Vf s = sub3(o, a);
+  const float u = dot3(s, p) * inv;
+  if (!(u >= 0.0f && u <= 1.0f)) return;
+  const Vf q = cross3(s, e1);
+  const float v = dot3(d, q) * inv;
+  if (!(v >= 0.0f && u + v <= 1.0f)) return;
+  const float t = dot3(e2, q) * inv;
+  if (!(t >= 0.0005f) || !closer(t, object, prim, *h)) return;
+  *h = Hit{t, u, v, object, prim};
+}
+
+Hit trace_all(const oracle_scene* sc, const oracle_geometry* g, Vf o, Vf d) {
+  Hit h{10000.0f, 0.0f, 0.0f, 0xffffffffu, 0xffffffffu};
+  for (uint32_t i = 0; i < sc->n_objects; ++i) {
+    const irt_object& ob = sc->objects[i];
+    const irt_mesh& me = sc->meshes[ob.info[0]];
+    const Vf lo = xform(ob.inverse, o.x, o.y, o.z, 1.0f);
+    const Vf ld = xform(ob.inverse, d.x, d.y, d.z, 0.0f);
+    const irt_vertex* vx = g->vertices + me.geometry[0];
+    const uint32_t* ix = g->indices + me.geometry[2];
+    for (uint32_t p = 0; p < me.geometry[3] / 3u; ++p)
+      ray_triangle(lo, ld, F3<float>(vx[ix[3 * p]].position), F3<float>(vx[ix[3 * p + 1]].position),
+                   F3<float>(vx[ix[3 * p + 2]].position), i, p, &h);
+  }
+  return h;
+}
+
+// basis — common.wgsl:84, applied to v.
+Vf basis_mul(Vf n, Vf v) {
+  const Vf axis = std::fabs(n.z) > 0.9f ? V<float>(1, 0, 0) : V<float>(0, 0, 1);
+  const Vf t = normalize3(cross3(axis, n));
+  const Vf b = cross3(n, t);
+  return V<float>(t.x * v.x + b.x * v.y + n.x * v.z, t.y * v.x + b.y * v.y + n.y * v.z,
+                  t.z * v.x + b.z * v.y + n.z * v.z);
+}
+
+const float PI32 = 3.14159265359f;
+
+// cosine_direction — common.wgsl:85.
+Vf cosine_direction(Vf n, uint32_t* rng) {
+  const float u = random_f32(rng);
+  const float phi = 2.0f * PI32 * random_f32(rng);
+  return basis_mul(n, V<float>(std::sqrt(u) * std::cos(phi), std::sqrt(u) * std::sin(phi), std::sqrt(1.0f - u)));
+}
+
+// sample_bsdf — common.wgsl:86-97.  pow(rough, 4) as (rough^2)^2.
+void sample_bsdf(Oracle<float>& o, const Surface<float>& s, Vf wo, uint32_t* rng, Vf* dir, Vf* weight) {
+  const float spec_probability = 0.25f * (1.0f - s.metal) + 0.9f * s.metal;
+  const float ra = s.rough * s.rough;
+  const float a2 = ra * ra;
+  Vf wi;
+  if (random_f32(rng) < spec_probability) {
+    const float u = random_f32(rng);
+    const float cos_theta = std::sqrt((1.0f - u) / (1.0f + (a2 - 1.0f) * u));
+    const float phi = 2.0f * PI32 * random_f32(rng);
+    const float st = std::sqrt(1.0f - cos_theta * cos_theta);
+    const Vf h = basis_mul(s.n, V<float>(st * std::cos(phi), st * std::sin(phi), cos_theta));
+    const Vf e = scale3(wo, -1.0f);
+    wi = sub3(e, scale3(h, 2.0f * dot3(h, e)));  // reflect(-wo, h)
+  } else {
+    wi = cosine_direction(s.n, rng);
+  }
+  *dir = wi;
+  const float nl = maxr(dot3(s.n, wi), 0.0f);
+  if (nl <= 0.0f) {
+    *weight = V<float>(0, 0, 0);
+    return;
+  }
+  const Vf h = normalize3(add3(wo, wi));
+  const float nh = maxr(dot3(s.n, h), 0.0f);
+  const float q = nh * nh * (a2 - 1.0f) + 1.0f;
+  const float pdf = (nl / PI32) * (1.0f - spec_probability) +
+                    (a2 * nh / (PI32 * q * q * maxr(4.0f * std::fabs(dot3(wo, h)), 0.00001f))) * spec_probability;
+  const Vf b = o.brdf(s, wo, wi);
+  const float den = maxr(pdf, 0.00001f);
+  *weight = V<float>(b.x * nl / den, b.y * nl / den, b.z * nl / den);
+}
+
+float clamp32(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
+
+// generate — vpl.wgsl:2-47, path `id`.
+irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geometry* g, uint32_t id) {
+  const irt_frame& F = o.F;
+  uint32_t rng = seed_of(id, 7u, F.size[2]);  // vpl.wgsl:4
+  irt_vpl v;
+  std::memset(&v, 0, sizeof(v));
+  const uint32_t L = F.counts[0];
+  if (L == 0u) return v;  // vpl.wgsl:5
+  const uint32_t light_id = std::min(to_u32(random_f32(&rng) * (float)L), L - 1u);  // vpl.wgsl:6
+  const irt_light& l = sc->lights[light_id];
+  Vf origin = F3<float>(l.position_strength), direction;
+  float omega = 4.0f * PI32;
+  if (l.color_type[3] > 0.5f) {  // vpl.wgsl:8-10
+    omega = 2.0f * PI32 * (1.0f - std::cos(l.direction_angle[3] * 0.5f));
+    const float r = random_f32(&rng);
+    const float z = std::cos(l.direction_angle[3] * 0.5f) * (1.0f - r) + 1.0f * r;
+    const float phi = 2.0f * PI32 * random_f32(&rng);
+    direction = basis_mul(normalize3(F3<float>(l.direction_angle)),
+                          V<float>(std::sqrt(1.0f - z * z) * std::cos(phi), std::sqrt(1.0f - z * z) * std::sin(phi), z));
+  } else {  // vpl.wgsl:11
+    const float z = 2.0f * random_f32(&rng) - 1.0f;
+    const float phi = 2.0f * PI32 * random_f32(&rng);
+    direction = V<float>(std::sqrt(1.0f - z * z) * std::cos(phi), std::sqrt(1.0f - z * z) * std::sin(phi), z);
+  }
+  const float Lf = (float)L;  // vpl.wgsl:14
+  const Vf source_flux = V<float>(l.color_type[0] * l.position_strength[3] * omega * Lf,
+                                  l.color_type[1] * l.position_strength[3] * omega * Lf,
+                                  l.color_type[2] * l.position_strength[3] * omega * Lf);
+  Vf throughput = V<float>(1, 1, 1);
+  const float Vn = (float)F.counts[1];
+  for (uint32_t bounce = 0; bounce < F.counts[2]; ++bounce) {  // vpl.wgsl:16-45
+    const Hit hit = trace_all(sc, g, origin, direction);
+    if (hit.object == 0xffffffffu) {
+      v.flux[0] = v.flux[1] = v.flux[2] = v.flux[3] = 0.0f;
+      break;
+    }
+    const irt_object& ob = sc->objects[hit.object];
+    const irt_mesh& me = sc->meshes[ob.info[0]];
+    const uint32_t base = me.geometry[2] + hit.prim * 3u;  // vpl.wgsl:23-28
+    const irt_vertex& a = g->vertices[me.geometry[0] + g->indices[base]];
+    const irt_vertex& b = g->vertices[me.geometry[0] + g->indices[base + 1u]];
+    const irt_vertex& c = g->vertices[me.geometry[0] + g->indices[base + 2u]];
+    const float bx = 1.0f - hit.u - hit.v, by = hit.u, bz = hit.v;
+    const Vf ln = V<float>(a.normal[0] * bx + b.normal[0] * by + c.normal[0] * bz,
+                           a.normal[1] * bx + b.normal[1] * by + c.normal[1] * bz,
+                           a.normal[2] * bx + b.normal[2] * by + c.normal[2] * bz);
+    const Vf n = normalize3(xform(ob.model, ln.x, ln.y, ln.z, 0.0f));
+    const irt_material& m = g->materials[ob.info[1]];  // material_at, common.wgsl:50-66 (untextured)
+    Surface<float> s;
+    s.color = F3<float>(m.color);
+    s.rough = clamp32(m.params[0], 0.045f, 1.0f);
+    s.metal = clamp32(m.params[1], 0.0f, 1.0f);
+    s.n = normalize3(n);
+    s.p = add3(origin, scale3(direction, hit.t));  // vpl.wgsl:30
+    const Vf wo = scale3(direction, -1.0f);
+    if (dot3(s.n, wo) <= 0.0f) {  // vpl.wgsl:32
+      v.flux[0] = v.flux[1] = v.flux[2] = v.flux[3] = 0.0f;
+      break;
+    }
+    v.position[0] = s.p.x;
+    v.position[1] = s.p.y;
+    v.position[2] = s.p.z;
+    v.position[3] = 1.0f;
+    v.normal[0] = s.n.x;
+    v.normal[1] = s.n.y;
+    v.normal[2] = s.n.z;
+    v.normal[3] = 0.0f;
+    Vf sdir, sweight;
+    sample_bsdf(o, s, wo, &rng, &sdir, &sweight);  // vpl.wgsl:36-38
+    const Vf reflected = mul3(throughput, sweight);
+    v.flux[0] = source_flux.x * reflected.x / Vn;
+    v.flux[1] = source_flux.y * reflected.y / Vn;
+    v.flux[2] = source_flux.z * reflected.z / Vn;
+    v.flux[3] = 0.0f;
+    if (bounce + 1u >= F.counts[2]) break;  // vpl.wgsl:39-44
+    const float survive = clamp32(std::fmax(reflected.x, std::fmax(reflected.y, reflected.z)), F.rr[0], F.rr[1]);
+    if (random_f32(&rng) > survive) break;
+    if (o.luminance(sweight) <= 0.0f) break;
+    throughput = div3(reflected, survive);
+    origin = add3(s.p, scale3(s.n, 0.001f));
+    direction = sdir;
+  }
+  return v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -985,5 +1180,50 @@ double oracle_shadow_step(int bits, double result, double h, double previous, do
 
 uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }
 float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+
+int oracle_generate_vpls(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
+                         irt_vpl* out, int threads) {
+  if (!frame || !scene || !geometry || !out) return -1;
+  if (frame->counts[0] > scene->n_lights) return -1;
+  Oracle<float> o = make<float>(frame, scene);
+  const int n = (int)frame->counts[1];
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  for (int i = 0; i < n; ++i) {
+    Oracle<float> local = o;
+    out[i] = generate_one(local, scene, geometry, (uint32_t)i);
+  }
+  return 0;
+}
+
+int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, const float origin[3],
+                 const float direction[3], float* t, float* u, float* v, uint32_t* object, uint32_t* prim) {
+  const Hit h = trace_all(scene, geometry, F3<float>(origin), F3<float>(direction));
+  *t = h.t;
+  *u = h.u;
+  *v = h.v;
+  *object = h.object;
+  *prim = h.prim;
+  return h.object != 0xffffffffu;
+}
+
+void oracle_sample_bsdf(const float n[3], const float color[3], float rough, float metal, const float wo[3],
+                        uint32_t* rng, float direction[3], float weight[3]) {
+  Oracle<float> o = make<float>(&dummy_frame(), nullptr);
+  Surface<float> s;
+  s.p = V<float>(0, 0, 0);
+  s.n = F3<float>(n);
+  s.color = F3<float>(color);
+  s.rough = rough;
+  s.metal = metal;
+  Vf d, w;
+  sample_bsdf(o, s, F3<float>(wo), rng, &d, &w);
+  direction[0] = d.x;
+  direction[1] = d.y;
+  direction[2] = d.z;
+  weight[0] = w.x;
+  weight[1] = w.y;
+  weight[2] = w.z;
+}
 
 }  // extern "C"

@@ -119,6 +119,26 @@ double oracle_shadow_step(int bits, double result, double h, double previous, do
 uint16_t oracle_f32_to_f16(float f);
 float oracle_f16_to_f32(uint16_t h);
 
+/* Triangle geometry of the VPL pass (vpl.wgsl bindings, common.wgsl:11-13). */
+typedef struct oracle_geometry {
+  const irt_vertex* vertices;
+  const uint32_t* indices; /* mesh-local, located by irt_mesh.geometry */
+  const irt_material* materials;
+  uint32_t n_vertices, n_indices, n_materials;
+} oracle_geometry;
+
+/* vpl.wgsl:2-47 (FP32): frame->counts.y paths into out[0..counts.y).  The
+ * closest hit is found by testing every triangle of every object in
+ * scene->objects.  Returns 0, or -1 for bad arguments. */
+int oracle_generate_vpls(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
+                         irt_vpl* out, int threads);
+/* Closest hit of one ray (world space); returns 1 on a hit. */
+int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, const float origin[3],
+                 const float direction[3], float* t, float* u, float* v, uint32_t* object, uint32_t* prim);
+/* sample_bsdf (common.wgsl:86-97) for an untextured surface. */
+void oracle_sample_bsdf(const float n[3], const float color[3], float rough, float metal, const float wo[3],
+                        uint32_t* rng, float direction[3], float weight[3]);
+
 #ifdef __cplusplus
 }
 #endif

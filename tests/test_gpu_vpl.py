@@ -1,0 +1,192 @@
+"""The VPL pass on the GPU (irt_generate_vpls) against the oracle.
+
+Parity: the oracle tests every triangle of every object; the kernel walks
+the instance hierarchy and the per-mesh triangle hierarchies.  Both use the
+same intersection arithmetic and the same tie rule, so a record differs only
+where the device's sin/cos (one ulp from the host's) moves a direction
+across a triangle edge or a roulette threshold.  Bar: every record field
+within 1e-4 (positions, normals) or 1e-4 relative (flux, denominator
+max(|oracle|, 1e-3)) on >= 99.5 % of the records.  The reference's own
+invariant test (gpu_tests.rs:356-407) runs through the device path as it
+is written there.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from instantradiositytest_amd import _lib
+from instantradiositytest_amd.renderer import Renderer, pack_frame
+from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
+from instantradiositytest_amd.types import ptr
+from vpl_scenes import build, closed_sphere, device_pool, light, model_matrix, vpl_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def records_match(got: np.ndarray, ref: np.ndarray) -> np.ndarray:
+    geo = np.all(np.abs(got[:, :8] - ref[:, :8]) <= 1e-4, axis=1)
+    flux = np.all(np.abs(got[:, 8:] - ref[:, 8:]) <= 1e-4 * np.maximum(np.abs(ref[:, 8:]), 1e-3), axis=1)
+    return geo & flux
+
+
+def room_renderer(name, precision, **settings):
+    cfg = CONFIGS[name].scaled(64, 32, 4)
+    cs = ConfigScene(cfg)
+    r = Renderer(cs.scene, cfg.settings(**settings), precision=precision, schedule="fused")
+    r.set_geometry(*cs.scene.geometry())
+    return cs, r
+
+
+def generated_pool(r: Renderer, frame_index: int) -> np.ndarray:
+    V = r.settings.vpl_count
+    ring = r.read("vpl_archive")
+    slot = frame_index % (r.settings.restir_max_history + 1)
+    return ring[slot * V:(slot + 1) * V].view(np.float32).reshape(V, 12).copy()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("vpls,bounces", [(300, 3), (2048, 5)])
+def test_generated_pool_matches_oracle(require_gpu, precision, vpls, bounces):
+    print(f"[vpl] room {precision} V={vpls} bounces={bounces}", flush=True)
+    cs, r = room_renderer("plain-640x400", precision, vpl_count=vpls, vpl_max_bounces=bounces)
+    try:
+        sr = oracle.SceneRecords.from_scene(cs.scene)
+        g = oracle.GeometryRecords(*cs.scene.geometry())
+        for fi in range(3):
+            inp = cs.frame_inputs(fi)
+            inp.vpls = None  # the pool comes from irt_generate_vpls
+            f = r.begin_frame(cs.scene, inp)
+            r.sync()
+            got = generated_pool(r, fi)
+            ref = oracle.generate_vpls(f, sr, g)
+            ok = records_match(got, ref)
+            exact = np.all(got == ref, axis=1)
+            print(f"[vpl] frame {fi}: {ok.mean():.4f} within bar, {exact.mean():.4f} bit-exact", flush=True)
+            assert ok.mean() >= 0.995
+            assert np.any(ref[:, 8:11] > 0)
+            r.lighting()
+            r.svgf()
+            r.end_frame()
+            r.finish_frame()
+    finally:
+        r.close()
+
+
+def test_budget_and_power_normalisation_on_device(require_gpu):
+    """gpu_tests.rs:356-407 through irt_generate_vpls."""
+    def pool(vpl_count, strength_scale):
+        cfg = CONFIGS["plain-640x400"].scaled(16, 16, 1)
+        cs = ConfigScene(cfg)
+        if strength_scale != 1:
+            lights = cs.scene.lights.copy()
+            lights["position_strength"][:, 3] *= strength_scale
+            cs.scene.set_lights(lights)
+        r = Renderer(cs.scene, cfg.settings(vpl_count=vpl_count), precision="fp32", schedule="fused")
+        try:
+            r.set_geometry(*cs.scene.geometry())
+            inp = cs.frame_inputs(0)
+            inp.vpls = None
+            r.begin_frame(cs.scene, inp)
+            r.sync()
+            return generated_pool(r, 0)
+        finally:
+            r.close()
+
+    V = CONFIGS["plain-640x400"].settings().vpl_count
+    baseline = pool(V, 1)
+    doubled_budget = pool(2 * V, 1)
+    doubled_power = pool(V, 2)
+    assert np.array_equal(baseline[:, :8], doubled_budget[:V, :8]), "path budget must preserve emission and roulette"
+    assert np.all(np.abs(baseline[:, 8:11] - 2 * doubled_budget[:V, 8:11]) < 1e-6)
+    assert np.array_equal(baseline[:, :8], doubled_power[:, :8]), "source power must preserve the path distribution"
+    assert np.all(np.abs(2 * baseline[:, 8:11] - doubled_power[:, 8:11]) < 1e-6 * np.maximum(1, doubled_power[:, 8:11]))
+    assert np.any(baseline[:, 8:11] > 0)
+
+
+def test_hit_through_a_transformed_instance_on_device(require_gpu):
+    tri_p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    normals = np.array([[0, 0, 1.0], [0.70710678, 0, 0.70710678], [0, 0.70710678, 0.70710678]])
+    model = model_matrix(scale=2.0, axis=(1, 1, 0), angle=0.7, translate=(0.3, -0.4, 1.5))
+    world_hit = (model @ np.array([0.6, 0.2, 0.0, 1.0]))[:3]
+    world_n = model[:3, :3] @ np.array([0, 0, 1.0])
+    world_n /= np.linalg.norm(world_n)
+    source = world_hit + 1.5 * world_n
+    sc = build([(tri_p, normals, [0, 1, 2])], [(0, model, 0)], [((0.8, 0.6, 0.4), 0.5, 0.0)],
+               light(source, axis=-world_n, angle=2e-5))
+    got = device_pool(sc, 64, 1)
+    ref = oracle.generate_vpls(vpl_frame(64, 1, 1, 1), sc.oracle_scene(),
+                               sc.oracle_geometry())
+    assert np.all(got[:, 3] == 1)
+    assert np.abs(got[:, :3] - world_hit).max() < 1e-4
+    assert records_match(got, ref).all()
+
+
+@pytest.mark.parametrize("kind", ["point", "spot"])
+def test_closed_sphere_pool_matches_oracle(require_gpu, kind):
+    lt = light((0.1, 0.2, -0.1), strength=2.0) if kind == "point" else \
+        light((0.1, 0.2, -0.1), strength=2.0, axis=(0.3, 1.0, 0.2), angle=0.9)
+    sc = closed_sphere(lt, rough=0.5, metal=0.4)
+    got = device_pool(sc, 4096, 4, frame_index=3)
+    ref = oracle.generate_vpls(vpl_frame(4096, 4, 1, 1, 3), sc.oracle_scene(),
+                               sc.oracle_geometry())
+    ok = records_match(got, ref)
+    print(f"[vpl] sphere {kind}: {ok.mean():.4f} within bar", flush=True)
+    assert ok.mean() >= 0.995
+
+
+def test_frame_with_generated_pool_equals_frame_with_uploaded_pool(require_gpu):
+    """A frame lit from irt_generate_vpls equals the same frame lit from that
+    pool uploaded with irt_set_vpls (the pool is the only input that changes)."""
+    cfg = CONFIGS["plain-640x400"].scaled(64, 32, 2)
+    outs = []
+    pool = None
+    for mode in ("generate", "upload"):
+        cs = ConfigScene(cfg)
+        r = Renderer(cs.scene, cfg.settings(), precision="fp32", schedule="fused")
+        try:
+            r.set_geometry(*cs.scene.geometry())
+            inp = cs.frame_inputs(0)
+            if mode == "generate":
+                inp.vpls = None
+            else:
+                inp.vpls = pool
+            r.begin_frame(cs.scene, inp)
+            if mode == "generate":
+                r.sync()
+                pool = r.read("vpl_archive")[:r.settings.vpl_count].copy()
+            r.lighting()
+            r.svgf()
+            r.end_frame()
+            r.finish_frame()
+            r.sync()
+            outs.append((r.read("lighting").copy(), r.read("history").copy()))
+        finally:
+            r.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_vpl_entry_points_reject_bad_state(require_gpu):
+    cfg = CONFIGS["plain-640x400"].scaled(16, 16, 1)
+    cs = ConfigScene(cfg)
+    r = Renderer(cs.scene, cfg.settings(), precision="fp32", schedule="fused")
+    L = _lib.irt()
+    try:
+        inp = cs.frame_inputs(0)
+        r.begin_frame(cs.scene, inp)
+        assert L.irt_generate_vpls(r.ctx) == -4  # IRT_ERR_STATE: no geometry yet
+        v, i, m = cs.scene.geometry()
+        textured = m.copy()
+        textured["textures"][0, 0] = 0
+        rc = L.irt_set_geometry(r.ctx, ptr(v), len(v), ptr(i), len(i), ptr(textured), len(textured))
+        assert rc == -1 and b"textured" in L.irt_last_error(r.ctx)
+        short = i[: len(i) - 3]
+        rc = L.irt_set_geometry(r.ctx, ptr(v), len(v), ptr(short), len(short), ptr(m), len(m))
+        assert rc == -1
+        r.set_geometry(v, i, m)
+        assert L.irt_generate_vpls(r.ctx) == 0
+        r.sync()
+    finally:
+        r.close()
