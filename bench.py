@@ -10,12 +10,16 @@ VPL pool are generated up front and staged in HBM; the timed region starts
 with them resident and covers G-buffer ingest + lighting + SVGF + history
 per frame.
 
-Legs (one JSON line carries all of them):
+Legs (one JSON line carries all of them; vpl_pass is reported beside the
+headline, not inside its frame, because the path's input VPL pool is staged
+like the G-buffer):
   headline   fused kernels, mixed storage (16-bit G-buffer / radiance / SDF
              samples, FP32 reservoirs, moments and march), single view
   plain      one kernel per reference pass, FP32 storage: the plain FP32
              path the headline is compared with
   fused_fp32 fused kernels, FP32 storage
+  vpls       the VPL pass (vpl.wgsl) on the same scene's triangles: launch
+             time, rays/s, and the oracle's rate for the same pool
   batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
              launch per pass, autotuned tiles, HIP graph replay
   serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
@@ -249,6 +253,50 @@ def run_batched(frames: int, warmup: int, steps: int, graph: bool = True) -> dic
     return out
 
 
+def run_vpls(cs, settings, host_frames, steps: int, threads: int, cpu: bool) -> dict:
+    """The VPL pass (irt_generate_vpls) on the scene's triangles: mean launch
+    time over `steps` launches of frame 0's pool, rays traced (from the
+    oracle's run of the same pool), and the oracle's time for that pool."""
+    import oracle
+    from instantradiositytest_amd._lib import check
+    from instantradiositytest_amd.renderer import Renderer, pack_frame
+    from instantradiositytest_amd.types import ptr
+    r = Renderer(cs.scene, settings, precision="mixed", schedule="fused")
+    try:
+        vertices, indices, materials = cs.scene.geometry()
+        r.set_geometry(vertices, indices, materials)
+        vp, cam = host_frames[0]
+        f = pack_frame(settings, vp, vp, cam, 0, cs.scene.generation, 0, False, 0, len(cs.scene.lights),
+                       len(cs.scene.objects))
+        check(r.L.irt_set_frame(r.ctx, ptr(f)), r.ctx)
+        for _ in range(3):
+            r.generate_vpls()
+        r.sync()
+        r.enable_timing(True)
+        for _ in range(steps):
+            r.generate_vpls()
+        r.sync()
+        ms = [t for name, t in pass_time_entries(r) if name == "vpl_generate"]
+        r.enable_timing(False)
+        t0 = time.perf_counter()
+        _, rays = oracle.generate_vpls(f, oracle.SceneRecords.from_scene(cs.scene),
+                                       oracle.GeometryRecords(vertices, indices, materials), threads=threads,
+                                       with_rays=True)
+        cpu_s = time.perf_counter() - t0
+    finally:
+        r.close()
+    launch_ms = float(np.mean(ms))
+    out = {"launch_ms": round(launch_ms, 4), "launches": len(ms), "paths": settings.vpl_count,
+           "max_bounces": settings.vpl_max_bounces, "rays": int(rays), "triangles": int(len(indices) // 3),
+           "objects": len(cs.scene.objects), "mrays_s": round(rays / launch_ms / 1e3, 2)}
+    if cpu:
+        out["cpu_baseline"] = {"value": round(rays / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"the oracle's VPL pass (every ray tested against every "
+                                                         f"triangle) for the same pool, {cpu_s:.2f} s"}
+    log(f"[bench] vpls: {out}")
+    return out
+
+
 def stage_serving_streams(variants: int, frames: int, seed: int):
     from instantradiositytest_amd.serving import SERVING_MIX, SceneStream, session_config
     rng = np.random.default_rng(seed)
@@ -309,7 +357,7 @@ def main() -> int:
     ap.add_argument("--serve-sweep", type=lambda v: [int(x) for x in v.split(",") if x], default=[4, 8, 16],
                     help="resident-session caps of the short serving runs after the main one")
     ap.add_argument("--sweep-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of legs: headline,plain,fused_fp32,batched,serving")
+    ap.add_argument("--only", default="", help="comma list of legs: headline,plain,fused_fp32,vpls,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     args = ap.parse_args()
@@ -317,7 +365,8 @@ def main() -> int:
         print(f"bench.py measures one GPU: the path renders each view and each session on one GPU with no "
               f"cross-GPU work; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
-    legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "batched", "serving"}
+    legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "vpls", "batched",
+                                                        "serving"}
 
     from instantradiositytest_amd import hip
     from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
@@ -403,6 +452,11 @@ def main() -> int:
     if head_r is not None:
         head_r.close()
     del staged
+    if "vpls" in legs:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        line["vpl_pass"] = run_vpls(cs, settings, host_frames, 50, threads, not args.no_cpu_baseline)
+        if head:
+            line["vpl_pass"]["share_of_headline_frame"] = round(line["vpl_pass"]["launch_ms"] / head["ms_per_step"], 4)
 
     if "batched" in legs:
         line["batched"] = run_batched(12, 4, args.batch_steps)

@@ -1437,11 +1437,31 @@ int irt_generate_vpls(irt_ctx* c) {
   if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "object references a missing mesh");
   if (c->max_material_ref >= c->n_materials) return c->fail(IRT_ERR_STATE, "object references a missing material");
   const uint32_t slot = F.size[2] % c->slots;
-  TriScene sc{c->objects, c->nodes,    c->meshes,    c->mesh_nodes, c->tri_nodes, c->tris,
-              c->vertices, c->indices, c->materials, c->lights,     c->n_nodes};
+  TriScene sc{c->objects, c->meshes,  c->mesh_nodes, c->tri_nodes, c->tris,
+              c->vertices, c->indices, c->materials,  c->lights};
+  // lanes per path: enough for one object per lane up to a wave
+  uint32_t group = 8;
+  while (group < 64 && group < c->n_objects) group <<= 1;
+  const uint32_t blocks = (uint32_t)(((uint64_t)c->V * group + 255) / 256);
+  irt_vpl* pool = c->ring + (size_t)slot * c->V;
   timer_begin(c, "vpl_generate");
-  hipLaunchKernelGGL(k_generate_vpls, dim3((c->V + 63) / 64), dim3(64), 0, c->stream, sc, F,
-                     c->ring + (size_t)slot * c->V);
+  switch (group) {
+    case 8:
+      hipLaunchKernelGGL(k_generate_vpls<8>, dim3(blocks), dim3(256), 0, c->stream, sc, F, c->obj_boxes, c->n_objects,
+                         pool);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_generate_vpls<16>, dim3(blocks), dim3(256), 0, c->stream, sc, F, c->obj_boxes,
+                         c->n_objects, pool);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_generate_vpls<32>, dim3(blocks), dim3(256), 0, c->stream, sc, F, c->obj_boxes,
+                         c->n_objects, pool);
+      break;
+    default:
+      hipLaunchKernelGGL(k_generate_vpls<64>, dim3(blocks), dim3(256), 0, c->stream, sc, F, c->obj_boxes,
+                         c->n_objects, pool);
+  }
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);
   return IRT_OK;

@@ -2,12 +2,12 @@
 //
 // The reference traces each light path with hardware ray queries against a
 // TLAS of per-mesh triangle BLASes (renderer.rs:1061-1106, 1786-1797).  Here
-// the instance level is the object hierarchy the lighting passes already
-// hold (preorder, escape links, leaf boxes that contain each object's
-// triangles) and the mesh level is a triangle hierarchy of the same stackless
-// form built on the host (irt_host::mesh_hierarchy).  A ray is moved into an
-// object's space by its inverse model matrix, as a ray query moves it into
-// instance space, so the hit distance is the world-space ray parameter.
+// the instance level is the objects' world boxes (the leaf boxes of the
+// object hierarchy, which contain each object's triangles), tested side by
+// side by the lanes of a group, and the mesh level is a stackless triangle
+// hierarchy built on the host (irt_host::mesh_hierarchy).  A ray is moved
+// into an object's space by its inverse model matrix, as a ray query moves it
+// into instance space, so the hit distance is the world-space ray parameter.
 //
 // Closest hit: triangles are both-sided (opaque geometry, no cull flags),
 // t in [0.0005, 10000] (the RayDesc of vpl.wgsl:18), Moller-Trumbore
@@ -16,7 +16,8 @@
 // so the answer does not depend on the walk order: it equals a scan over
 // every triangle of every object.
 //
-// One lane per light path; the path's bounces are sequential.
+// A group of lanes per light path: the path's bounces are sequential, its
+// rays' object tests run side by side across the group.
 #pragma once
 
 #include "irt_device.h"
@@ -25,7 +26,6 @@ namespace irt {
 
 struct TriScene {
   const irt_object* objects;
-  const irt_node* nodes;  // instance level
   const irt_mesh* meshes;
   const uint2* mesh_nodes;  // per mesh: [first, end) of its triangle hierarchy
   const irt_tri_node* tri_nodes;
@@ -34,7 +34,6 @@ struct TriScene {
   const uint32_t* indices;
   const irt_material* materials;
   const irt_light* lights;
-  uint32_t n_nodes;
 };
 
 struct RayHit {
@@ -87,40 +86,52 @@ __device__ __forceinline__ void ray_triangle(v3 o, v3 d, const irt_tri& tr, uint
   h = RayHit{t, u, v, object, tr.prim};
 }
 
-// rayQueryInitialize/Proceed/GetCommittedIntersection of vpl.wgsl:17-20.
-// Returns the closest hit (object == 0xffffffff: none).
-__device__ __forceinline__ RayHit trace_closest(const TriScene& sc, v3 o, v3 d) {
+// Closest hit of the ray in object `obj`'s triangles, folded into h: the
+// ray moves into object space by the inverse model matrix (a ray query's
+// instance transform) and walks the mesh's hierarchy without a stack.
+__device__ __forceinline__ void trace_object(const TriScene& sc, uint32_t obj, v3 o, v3 d, RayHit& h) {
+  const irt_object& ob = sc.objects[obj];
+  const float4 lo4 = mat_mul(ob.inverse, o.x, o.y, o.z, 1.0f);
+  const float4 ld4 = mat_mul(ob.inverse, d.x, d.y, d.z, 0.0f);
+  const v3 lo = v3{lo4.x, lo4.y, lo4.z}, ld = v3{ld4.x, ld4.y, ld4.z};
+  const v3 linv = v3{slab_rcp(ld.x), slab_rcp(ld.y), slab_rcp(ld.z)};
+  const uint2 range = sc.mesh_nodes[ob.info[0]];
+  uint32_t j = range.x;
+  while (j < range.y) {
+    const irt_tri_node& tn = sc.tri_nodes[j];
+    if (!ray_box(lo, linv, tn.lo, tn.hi, RAY_TMIN, h.t)) {
+      j = tn.count ? j + 1u : tn.link;
+      continue;
+    }
+    for (uint32_t k = 0; k < tn.count; ++k) ray_triangle(lo, ld, sc.tris[tn.link + k], obj, h);
+    ++j;
+  }
+}
+
+// rayQueryInitialize/Proceed/GetCommittedIntersection of vpl.wgsl:17-20 for
+// a group of G lanes that share one ray: lane k takes objects k, k+G, ...
+// (each one's box is its leaf box in the object hierarchy), walks their
+// triangles, and the group keeps the closest hit by the tie rule above, so
+// every lane returns the same hit (object == 0xffffffff: none).
+template <int G>
+__device__ __forceinline__ RayHit trace_group(const TriScene& sc, const float4* boxes, uint32_t n_objects, v3 o,
+                                              v3 d, uint32_t lane) {
   RayHit h{RAY_TMAX, 0.0f, 0.0f, 0xffffffffu, 0xffffffffu};
   const v3 inv = v3{slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z)};
-  uint32_t i = 0;
-  while (i < sc.n_nodes) {
-    const irt_node& nd = sc.nodes[i];
-    if (!ray_box(o, inv, nd.lo, nd.hi, RAY_TMIN, h.t)) {
-      i = nd.info[3];
-      continue;
-    }
-    if (nd.info[2] != 1u) {
-      ++i;  // left child follows in preorder
-      continue;
-    }
-    const uint32_t obj = nd.info[0];
-    const irt_object& ob = sc.objects[obj];
-    const float4 lo4 = mat_mul(ob.inverse, o.x, o.y, o.z, 1.0f);
-    const float4 ld4 = mat_mul(ob.inverse, d.x, d.y, d.z, 0.0f);
-    const v3 lo = v3{lo4.x, lo4.y, lo4.z}, ld = v3{ld4.x, ld4.y, ld4.z};
-    const v3 linv = v3{slab_rcp(ld.x), slab_rcp(ld.y), slab_rcp(ld.z)};
-    const uint2 range = sc.mesh_nodes[ob.info[0]];
-    uint32_t j = range.x;
-    while (j < range.y) {
-      const irt_tri_node& tn = sc.tri_nodes[j];
-      if (!ray_box(lo, linv, tn.lo, tn.hi, RAY_TMIN, h.t)) {
-        j = tn.count ? j + 1u : tn.link;
-        continue;
-      }
-      for (uint32_t k = 0; k < tn.count; ++k) ray_triangle(lo, ld, sc.tris[tn.link + k], obj, h);
-      ++j;
-    }
-    i = nd.info[3];
+  for (uint32_t obj = lane; obj < n_objects; obj += G) {
+    const float4 blo = boxes[2 * obj], bhi = boxes[2 * obj + 1];
+    const float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
+    if (ray_box(o, inv, lo, hi, RAY_TMIN, h.t)) trace_object(sc, obj, o, d, h);
+  }
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    RayHit q;
+    q.t = __shfl_xor(h.t, off, G);
+    q.u = __shfl_xor(h.u, off, G);
+    q.v = __shfl_xor(h.v, off, G);
+    q.object = (uint32_t)__shfl_xor((int)h.object, off, G);
+    q.prim = (uint32_t)__shfl_xor((int)h.prim, off, G);
+    if (closer(q.t, q.object, q.prim, h)) h = q;
   }
   return h;
 }
@@ -181,9 +192,15 @@ __device__ __forceinline__ Scatter sample_bsdf(const Surface& s, uint32_t& rng) 
   return Scatter{wi, v3{b.x * nl / den, b.y * nl / den, b.z * nl / den}};
 }
 
-// generate — vpl.wgsl:2-47.  One lane per path id < frame.counts.y.
-__global__ __launch_bounds__(64) void k_generate_vpls(TriScene sc, irt_frame F, irt_vpl* __restrict__ pool) {
-  const uint32_t id = blockIdx.x * 64u + threadIdx.x;
+// generate — vpl.wgsl:2-47.  G lanes per path id < frame.counts.y: every
+// lane of a group evaluates the path's (identical) sampling and shading,
+// the group shares each ray's object tests, and the group's first lane
+// writes the record.  Whole groups leave together.
+template <int G>
+__global__ __launch_bounds__(256) void k_generate_vpls(TriScene sc, irt_frame F, const float4* __restrict__ boxes,
+                                                       uint32_t n_objects, irt_vpl* __restrict__ pool) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t id = t / G, lane = t % G;
   const uint32_t V = F.counts[1];
   if (id >= V) return;
   uint32_t rng = seed(id, 7u, F.size[2]);
@@ -191,7 +208,7 @@ __global__ __launch_bounds__(64) void k_generate_vpls(TriScene sc, irt_frame F, 
   for (int k = 0; k < 4; ++k) v.position[k] = v.normal[k] = v.flux[k] = 0.0f;
   const uint32_t L = F.counts[0];
   if (L == 0u) {
-    pool[id] = v;
+    if (lane == 0) pool[id] = v;
     return;
   }
   const uint32_t light_id = min(f2u_sat(rnd(rng) * (float)L), L - 1u);
@@ -225,7 +242,7 @@ __global__ __launch_bounds__(64) void k_generate_vpls(TriScene sc, irt_frame F, 
   v3 throughput = v3{1.0f, 1.0f, 1.0f};
   const float Vf = (float)V;
   for (uint32_t bounce = 0; bounce < F.counts[2]; ++bounce) {
-    const RayHit hit = trace_closest(sc, origin, direction);
+    const RayHit hit = trace_group<G>(sc, boxes, n_objects, origin, direction, lane);
     if (hit.object == 0xffffffffu) {
       v.flux[0] = v.flux[1] = v.flux[2] = v.flux[3] = 0.0f;
       break;
@@ -278,7 +295,7 @@ __global__ __launch_bounds__(64) void k_generate_vpls(TriScene sc, irt_frame F, 
     origin = s.p + s.n * 0.001f;
     direction = sc_.direction;
   }
-  pool[id] = v;
+  if (lane == 0) pool[id] = v;
 }
 
 }  // namespace irt

@@ -97,7 +97,7 @@ def lib() -> ctypes.CDLL:
     L.oracle_f16_to_f32.restype = c_float
     L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
     L.oracle_generate_vpls.restype = c_int
-    L.oracle_generate_vpls.argtypes = [P, P, P, P, c_int]
+    L.oracle_generate_vpls.argtypes = [P, P, P, P, c_int, POINTER(c_uint64)]
     L.oracle_trace.restype = c_int
     L.oracle_trace.argtypes = [P, P, P, P, POINTER(c_float), POINTER(c_float), POINTER(c_float),
                                POINTER(c_uint32), POINTER(c_uint32)]
@@ -156,15 +156,19 @@ class GeometryRecords:
                           len(self.indices), len(self.materials))
 
 
-def generate_vpls(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRecords, threads: int = 0) -> np.ndarray:
-    """vpl.wgsl:2-47 for the frame: float32 [counts.y, 12] records (position, normal, flux)."""
+def generate_vpls(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRecords, threads: int = 0,
+                  with_rays: bool = False):
+    """vpl.wgsl:2-47 for the frame: float32 [counts.y, 12] records (position,
+    normal, flux), and the number of rays traced when `with_rays` is set."""
     f = np.ascontiguousarray(frame)
     n = int(np.frombuffer(f.tobytes(), np.uint32)[73])  # counts.y (counts at byte 288)
     out = np.zeros((n, 12), dtype=np.float32)
-    rc = lib().oracle_generate_vpls(_p(f), ctypes.byref(scene.c), ctypes.byref(geometry.c), _p(out), threads)
+    rays = c_uint64()
+    rc = lib().oracle_generate_vpls(_p(f), ctypes.byref(scene.c), ctypes.byref(geometry.c), _p(out), threads,
+                                    ctypes.byref(rays))
     if rc != 0:
         raise ValueError("oracle_generate_vpls: bad arguments")
-    return out
+    return (out, rays.value) if with_rays else out
 
 
 def trace(scene: SceneRecords, geometry: GeometryRecords, origin, direction):

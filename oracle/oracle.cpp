@@ -996,7 +996,8 @@ void sample_bsdf(Oracle<float>& o, const Surface<float>& s, Vf wo, uint32_t* rng
 float clamp32(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
 
 // generate — vpl.wgsl:2-47, path `id`.
-irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geometry* g, uint32_t id) {
+irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geometry* g, uint32_t id,
+                     uint64_t* rays) {
   const irt_frame& F = o.F;
   uint32_t rng = seed_of(id, 7u, F.size[2]);  // vpl.wgsl:4
   irt_vpl v;
@@ -1027,6 +1028,7 @@ irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geom
   const float Vn = (float)F.counts[1];
   for (uint32_t bounce = 0; bounce < F.counts[2]; ++bounce) {  // vpl.wgsl:16-45
     const Hit hit = trace_all(sc, g, origin, direction);
+    ++*rays;
     if (hit.object == 0xffffffffu) {
       v.flux[0] = v.flux[1] = v.flux[2] = v.flux[3] = 0.0f;
       break;
@@ -1182,17 +1184,21 @@ uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }
 float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
 
 int oracle_generate_vpls(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
-                         irt_vpl* out, int threads) {
+                         irt_vpl* out, int threads, uint64_t* rays) {
   if (!frame || !scene || !geometry || !out) return -1;
   if (frame->counts[0] > scene->n_lights) return -1;
   Oracle<float> o = make<float>(frame, scene);
   const int n = (int)frame->counts[1];
   if (threads <= 0) threads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : total)
   for (int i = 0; i < n; ++i) {
     Oracle<float> local = o;
-    out[i] = generate_one(local, scene, geometry, (uint32_t)i);
+    uint64_t traced = 0;
+    out[i] = generate_one(local, scene, geometry, (uint32_t)i, &traced);
+    total += traced;
   }
+  if (rays) *rays = total;
   return 0;
 }
 

@@ -129,9 +129,10 @@ typedef struct oracle_geometry {
 
 /* vpl.wgsl:2-47 (FP32): frame->counts.y paths into out[0..counts.y).  The
  * closest hit is found by testing every triangle of every object in
- * scene->objects.  Returns 0, or -1 for bad arguments. */
+ * scene->objects.  *rays (may be null) receives the number of rays traced.
+ * Returns 0, or -1 for bad arguments. */
 int oracle_generate_vpls(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
-                         irt_vpl* out, int threads);
+                         irt_vpl* out, int threads, uint64_t* rays);
 /* Closest hit of one ray (world space); returns 1 on a hit. */
 int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, const float origin[3],
                  const float direction[3], float* t, float* u, float* v, uint32_t* object, uint32_t* prim);
