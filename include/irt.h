@@ -306,6 +306,17 @@ int irt_set_geometry(irt_ctx* ctx, const irt_vertex* vertices, uint32_t vertex_c
  * irt_stream; call after irt_set_frame and irt_set_geometry. */
 int irt_generate_vpls(irt_ctx* ctx);
 
+/* The G-buffer pass (gbuffer.wgsl:1-12; the depth pre-pass and G-buffer
+ * pass of renderer.rs:1801-1862 with the pipelines of renderer.rs:1370-1415:
+ * back faces culled, counter-clockwise front faces, depth Less then Equal,
+ * targets cleared to 0 and depth to 1): rasterises the frame's objects
+ * (model and previous matrices of irt_set_objects, triangles of
+ * irt_set_geometry, frame.vp and frame.previous_vp) into the current
+ * G-buffer, in place of irt_set_gbuffer.  Values are rounded as the
+ * reference's Depth32Float / Rgba16F / Rg16F targets hold them.  Untextured
+ * materials.  Enqueued on irt_stream; call after irt_set_frame. */
+int irt_render_gbuffer(irt_ctx* ctx);
+
 /* The six reservoir passes and the two resolves (renderer.rs:1889-1928). */
 int irt_lighting(irt_ctx* ctx);
 /* Svgf::encode (svgf.rs:140-261): temporal, variance, `iterations` a-trous,

@@ -14,6 +14,9 @@
 #include "irt_host.h"
 #include "irt_kernels.h"
 #include "irt_vpl.h"
+#include "irt_raster.h"
+
+#include <hipcub/device/device_scan.hpp>
 
 using namespace irt;
 
@@ -105,6 +108,23 @@ struct irt_ctx {
   uint2* mesh_nodes = nullptr;  // per mesh: [first, end) of its triangle hierarchy
   uint32_t cap_mesh_nodes = 0;
   bool have_geometry = false;
+  // G-buffer pass (irt_render_gbuffer)
+  std::vector<uint32_t> host_obj_mesh;   // mesh index of each object
+  std::vector<uint32_t> host_tri_base;   // instance-triangle prefix sums the device holds
+  uint32_t* obj_tri_base = nullptr;
+  uint32_t cap_obj_tri_base = 0;
+  GbObject* gb_objs = nullptr;
+  uint32_t cap_gb_objs = 0;
+  GbTri* gb_tris = nullptr;
+  uint32_t cap_gb_tris = 0;
+  uint32_t* gb_tiles = nullptr;
+  uint32_t cap_gb_tiles = 0;
+  uint32_t* gb_scan = nullptr;
+  uint32_t cap_gb_scan = 0;
+  uint8_t* gb_scan_tmp = nullptr;
+  uint32_t cap_gb_scan_tmp = 0;
+  unsigned long long* gb_vis = nullptr;
+  uint32_t cap_gb_vis = 0;
   // G-buffer slots (internal formats).  The current and previous frames'
   // G-buffers are slots gb_cur / gb_prev (the same slot right after a frame
   // ends, as when the reference copies the G-buffer); a write into the
@@ -1083,6 +1103,8 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   c->n_nodes = node_count;
   c->max_mesh_ref = max_ref;
   c->max_material_ref = max_material;
+  c->host_obj_mesh.resize(count);
+  for (uint32_t i = 0; i < count; ++i) c->host_obj_mesh[i] = objects[i].info[0];
   // Object world boxes (the hierarchy's leaves) and the grid over the root box.
   std::vector<float4> boxes(2 * (size_t)count, make_float4(0.f, 0.f, 0.f, 0.f));
   for (uint32_t i = 0; i < node_count; ++i)
@@ -1180,6 +1202,7 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
   dst->host_meshes = src->host_meshes;
   dst->max_mesh_ref = src->max_mesh_ref;
   dst->max_material_ref = src->max_material_ref;
+  dst->host_obj_mesh = src->host_obj_mesh;
   dst->have_geometry = false;
   if (src->have_geometry) {
     rc = ensure_array(dst, &dst->vertices, &dst->cap_vertices, src->n_vertices);
@@ -1462,6 +1485,68 @@ int irt_generate_vpls(irt_ctx* c) {
       hipLaunchKernelGGL(k_generate_vpls<64>, dim3(blocks), dim3(256), 0, c->stream, sc, F, c->obj_boxes,
                          c->n_objects, pool);
   }
+  HIP_CHECK(c, hipGetLastError());
+  timer_end(c);
+  return IRT_OK;
+}
+
+int irt_render_gbuffer(irt_ctx* c) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!c->have_frame) return c->fail(IRT_ERR_STATE, "set the frame before rendering its G-buffer");
+  if (!c->have_geometry) return c->fail(IRT_ERR_STATE, "no triangle geometry (irt_set_geometry)");
+  if (!c->n_objects) return c->fail(IRT_ERR_STATE, "no objects");
+  if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "object references a missing mesh");
+  if (c->max_material_ref >= c->n_materials) return c->fail(IRT_ERR_STATE, "object references a missing material");
+  // instance triangles: objects in order, each its mesh's triangles in index order (the draw order)
+  std::vector<uint32_t> base(c->n_objects + 1, 0);
+  for (uint32_t o = 0; o < c->n_objects; ++o) {
+    const uint64_t next = (uint64_t)base[o] + c->host_meshes[c->host_obj_mesh[o]].geometry[3] / 3u;
+    if (next >= 0xffffffffull) return c->fail(IRT_ERR_LIMIT, "too many instance triangles");
+    base[o + 1] = (uint32_t)next;
+  }
+  const uint32_t n = base[c->n_objects];
+  int rc = ensure_array(c, &c->obj_tri_base, &c->cap_obj_tri_base, c->n_objects + 1);
+  if (!rc) rc = ensure_array(c, &c->gb_objs, &c->cap_gb_objs, c->n_objects);
+  if (!rc) rc = ensure_array(c, &c->gb_tris, &c->cap_gb_tris, n);
+  if (!rc) rc = ensure_array(c, &c->gb_tiles, &c->cap_gb_tiles, n);
+  if (!rc) rc = ensure_array(c, &c->gb_scan, &c->cap_gb_scan, n);
+  if (!rc) rc = ensure_array(c, &c->gb_vis, &c->cap_gb_vis, (uint32_t)c->npx);
+  if (rc) return rc;
+  size_t tmp_bytes = 0;
+  HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, c->gb_tiles, c->gb_scan, (int)std::max(n, 1u),
+                                                c->stream));
+  rc = ensure_array(c, &c->gb_scan_tmp, &c->cap_gb_scan_tmp, (uint32_t)tmp_bytes);
+  if (rc) return rc;
+  if (base != c->host_tri_base || c->cap_obj_tri_base == 0) {
+    HIP_CHECK(c, hipMemcpyAsync(c->obj_tri_base, base.data(), sizeof(uint32_t) * base.size(), hipMemcpyHostToDevice,
+                                c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->host_tri_base = base;
+  }
+  rc = make_current_writable(c, false);
+  if (rc) return rc;
+  c->guide_dirty = true;
+  const uint32_t W = c->W, H = c->H, npx = (uint32_t)c->npx;
+  GbScene sc{c->objects, c->meshes, c->vertices, c->indices, c->materials, c->obj_tri_base, c->n_objects};
+  timer_begin(c, "gbuffer");
+  hipLaunchKernelGGL(k_gb_objects, dim3((c->n_objects + 63) / 64), dim3(64), 0, c->stream, c->objects, c->n_objects,
+                     c->frame, c->gb_objs);
+  hipLaunchKernelGGL(k_gb_clear, dim3((npx + 255) / 256), dim3(256), 0, c->stream, c->gb_vis, npx);
+  if (n) {
+    hipLaunchKernelGGL(k_gb_setup, dim3((n + 255) / 256), dim3(256), 0, c->stream, sc, c->gb_objs, n, W, H,
+                       c->gb_tris, c->gb_tiles);
+    HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(c->gb_scan_tmp, tmp_bytes, c->gb_tiles, c->gb_scan, (int)n,
+                                                  c->stream));
+    hipLaunchKernelGGL(k_gb_raster, dim3(c->march_blocks), dim3(256), 0, c->stream, c->gb_tris, c->gb_scan, n, W, H,
+                       c->gb_vis);
+  }
+  const dim3 tiles((W + GB_TILE - 1) / GB_TILE, (H + GB_TILE - 1) / GB_TILE);
+  if (mixed(c))
+    hipLaunchKernelGGL(k_gb_resolve<ProfileMixed>, tiles, dim3(256), 0, c->stream, sc, c->gb_objs, c->gb_vis, W, H,
+                       c->depth, (h4*)c->normal, (h4*)c->color, (h2*)c->params, (h2*)c->motion);
+  else
+    hipLaunchKernelGGL(k_gb_resolve<ProfileF32>, tiles, dim3(256), 0, c->stream, sc, c->gb_objs, c->gb_vis, W, H,
+                       c->depth, (float4*)c->normal, (float4*)c->color, (float2*)c->params, (float2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);
   return IRT_OK;

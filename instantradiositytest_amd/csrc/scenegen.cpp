@@ -205,6 +205,19 @@ void tessellate(const sg_shape& s, std::vector<float>& pos, std::vector<uint32_t
     }
 }
 
+// Winds every triangle counter-clockwise seen from outside (the front face of
+// the raster pass, renderer.rs:1384-1386): its geometric normal points along
+// the shape's outward normal at the triangle's centroid.
+void wind_outward(const sg_shape& s, const std::vector<float>& pos, std::vector<uint32_t>& idx) {
+  for (size_t t = 0; t + 2 < idx.size(); t += 3) {
+    d3 v[3];
+    for (int k = 0; k < 3; ++k) v[k] = d3{pos[3 * idx[t + k]], pos[3 * idx[t + k] + 1], pos[3 * idx[t + k] + 2]};
+    const d3 g = cross(v[1] - v[0], v[2] - v[0]);
+    const d3 c = (v[0] + v[1] + v[2]) * (1.0 / 3.0);
+    if (dot(g, normal_local(s, c)) < 0) std::swap(idx[t + 1], idx[t + 2]);
+  }
+}
+
 // Ray / shape intersection in local space; returns t > tmin or -1.
 double intersect_local(const sg_shape& s, d3 o, d3 d, double tmin) {
   // Slab test against the local AABB first (also the box answer).
@@ -368,6 +381,7 @@ sg_scene* sg_create(const sg_shape* shapes, uint32_t n_shapes, const sg_object* 
     std::vector<float> pos;
     std::vector<uint32_t> idx;
     tessellate(s, pos, idx);
+    wind_outward(s, pos, idx);
     const uint32_t nv = (uint32_t)(pos.size() / 3);
     float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t v = 0; v < nv; ++v)
@@ -522,6 +536,7 @@ void sg_geometry(const sg_scene* s, irt_vertex* vertices, uint32_t* indices, irt
     std::vector<float> pos;
     std::vector<uint32_t> idx;
     tessellate(shape, pos, idx);
+    wind_outward(shape, pos, idx);
     const uint32_t nv = (uint32_t)(pos.size() / 3);
     std::vector<d3> normal(nv);
     if (shape.kind == SG_BOX) {

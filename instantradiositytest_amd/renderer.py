@@ -123,6 +123,10 @@ class Renderer:
         m = np.ascontiguousarray(materials)
         check(self.L.irt_set_geometry(self.ctx, ptr(v), len(v), ptr(i), len(i), ptr(m), len(m)), self.ctx)
 
+    def render_gbuffer(self):
+        """The G-buffer pass of the frame set last (irt_render_gbuffer, gbuffer.wgsl)."""
+        check(self.L.irt_render_gbuffer(self.ctx), self.ctx)
+
     def generate_vpls(self):
         """The VPL pass of the frame set last (irt_generate_vpls, vpl.wgsl:2-47)."""
         check(self.L.irt_generate_vpls(self.ctx), self.ctx)
@@ -133,7 +137,9 @@ class Renderer:
 
     # -- frame ------------------------------------------------------------
     def begin_frame(self, scene: Scene, inputs: FrameInputs) -> np.ndarray:
-        """Upload the frame's records and inputs; returns the packed Frame."""
+        """Upload the frame's records and inputs; returns the packed Frame.
+        inputs.depth None: the G-buffer pass renders the G-buffer (set_geometry
+        first); inputs.vpls None: the VPL pass generates the pool."""
         L, ctx = self.L, self.ctx
         self.upload_scene(scene)
         vp = np.asarray(inputs.vp, np.float32)
@@ -142,8 +148,11 @@ class Renderer:
         f = pack_frame(self.settings, vp, pvp, inputs.camera, self.frame_index, scene.generation,
                        self.previous_generation, history, self.debug_mode, len(scene.lights), len(scene.objects))
         check(L.irt_set_frame(ctx, ptr(f)), ctx)
-        check(L.irt_set_gbuffer(ctx, ptr(inputs.depth), ptr(inputs.normal), ptr(inputs.color), ptr(inputs.params),
-                                ptr(inputs.motion)), ctx)
+        if inputs.depth is None:
+            self.render_gbuffer()
+        else:
+            check(L.irt_set_gbuffer(ctx, ptr(inputs.depth), ptr(inputs.normal), ptr(inputs.color),
+                                    ptr(inputs.params), ptr(inputs.motion)), ctx)
         if inputs.vpls is None:
             self.generate_vpls()
         else:

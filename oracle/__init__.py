@@ -101,6 +101,8 @@ def lib() -> ctypes.CDLL:
     L.oracle_trace.restype = c_int
     L.oracle_trace.argtypes = [P, P, P, P, POINTER(c_float), POINTER(c_float), POINTER(c_float),
                                POINTER(c_uint32), POINTER(c_uint32)]
+    L.oracle_render_gbuffer.restype = c_int
+    L.oracle_render_gbuffer.argtypes = [P, P, P, P, P, P, P, P]
     L.oracle_sample_bsdf.restype = None
     L.oracle_sample_bsdf.argtypes = [P, P, c_float, c_float, P, POINTER(c_uint32), P, P]
     _L = L
@@ -169,6 +171,24 @@ def generate_vpls(frame: np.ndarray, scene: SceneRecords, geometry: GeometryReco
     if rc != 0:
         raise ValueError("oracle_generate_vpls: bad arguments")
     return (out, rays.value) if with_rays else out
+
+
+def render_gbuffer(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRecords):
+    """gbuffer.wgsl for the frame: (depth float32 [H, W], normal/color uint16
+    [H, W, 4], params/motion uint16 [H, W, 2]), binary16 bit patterns."""
+    f = np.ascontiguousarray(frame)
+    words = np.frombuffer(f.tobytes(), np.uint32)
+    W, H = int(words[68]), int(words[69])  # size.xy (size at byte 272)
+    depth = np.zeros((H, W), np.float32)
+    normal = np.zeros((H, W, 4), np.uint16)
+    color = np.zeros((H, W, 4), np.uint16)
+    params = np.zeros((H, W, 2), np.uint16)
+    motion = np.zeros((H, W, 2), np.uint16)
+    rc = lib().oracle_render_gbuffer(_p(f), ctypes.byref(scene.c), ctypes.byref(geometry.c), _p(depth), _p(normal),
+                                     _p(color), _p(params), _p(motion))
+    if rc != 0:
+        raise ValueError("oracle_render_gbuffer: bad arguments")
+    return depth, normal, color, params, motion
 
 
 def trace(scene: SceneRecords, geometry: GeometryRecords, origin, direction):

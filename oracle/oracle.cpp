@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace {
 
@@ -1082,6 +1083,73 @@ irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geom
   return v;
 }
 
+// ------------------------------------------------------------ G-buffer pass
+// gbuffer.wgsl:1-12 with the raster state of renderer.rs:1370-1415 (back
+// faces culled, counter-clockwise front, depth Less then Equal): every
+// instance triangle's covered pixels by 2-D homogeneous edge functions,
+// the least (depth, ~draw index) per pixel, then the fragment stage.
+void mat4_mul(const float* a, const float* b, float* out) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r)
+      out[c * 4 + r] = a[r] * b[c * 4] + a[4 + r] * b[c * 4 + 1] + a[8 + r] * b[c * 4 + 2] + a[12 + r] * b[c * 4 + 3];
+}
+struct Clip {
+  float x, y, z, w;
+};
+Clip clip_of(const float* m, const float* p) {
+  return Clip{m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12] * p[3], m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13] * p[3],
+              m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14] * p[3],
+              m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15] * p[3]};
+}
+float edge_planes(const Clip c[3], float e[9]) {
+  const Vf V0 = V<float>(c[0].x, c[0].y, c[0].w), V1 = V<float>(c[1].x, c[1].y, c[1].w),
+           V2 = V<float>(c[2].x, c[2].y, c[2].w);
+  const Vf a = cross3(V1, V2), b = cross3(V2, V0), d = cross3(V0, V1);
+  e[0] = a.x; e[1] = a.y; e[2] = a.z;
+  e[3] = b.x; e[4] = b.y; e[5] = b.z;
+  e[6] = d.x; e[7] = d.y; e[8] = d.z;
+  return V0.x * a.x + V0.y * a.y + V0.z * a.z;
+}
+float plane_at(const float* e, float px, float py) { return e[0] * px + e[1] * py + e[2]; }
+float ndc_x(uint32_t x, uint32_t W) { return ((float)x + 0.5f) / (float)W * 2.0f - 1.0f; }
+float ndc_y(uint32_t y, uint32_t H) { return 1.0f - ((float)y + 0.5f) / (float)H * 2.0f; }
+
+// Pixel bounds of the part of the triangle in front of the near plane
+// (false: nothing to draw).
+bool tri_bounds(const Clip c[3], uint32_t W, uint32_t H, uint32_t* x0, uint32_t* x1, uint32_t* y0, uint32_t* y1) {
+  if (c[0].z >= c[0].w && c[1].z >= c[1].w && c[2].z >= c[2].w) return false;
+  float lx = INFINITY, hx = -INFINITY, ly = INFINITY, hy = -INFINITY;
+  bool any = false, whole = false;
+  for (int k = 0; k < 3; ++k) {
+    const Clip a = c[k], b = c[(k + 1) % 3];
+    Clip q[2];
+    int nq = 0;
+    if (a.z >= 0.0f) q[nq++] = a;
+    if ((a.z >= 0.0f) != (b.z >= 0.0f)) {
+      const float f = a.z / (a.z - b.z);
+      q[nq++] = Clip{a.x + (b.x - a.x) * f, a.y + (b.y - a.y) * f, 0.0f, a.w + (b.w - a.w) * f};
+    }
+    for (int j = 0; j < nq; ++j) {
+      any = true;
+      if (!(q[j].w > 0.0f)) {
+        whole = true;
+        continue;
+      }
+      const float sx = (q[j].x / q[j].w + 1.0f) * 0.5f * (float)W - 0.5f;
+      const float sy = (1.0f - q[j].y / q[j].w) * 0.5f * (float)H - 0.5f;
+      lx = std::fmin(lx, sx); hx = std::fmax(hx, sx); ly = std::fmin(ly, sy); hy = std::fmax(hy, sy);
+    }
+  }
+  float fx0 = 0.0f, fx1 = (float)W - 1.0f, fy0 = 0.0f, fy1 = (float)H - 1.0f;
+  if (!whole) {
+    fx0 = std::fmax(fx0, std::floor(lx) - 1.0f); fx1 = std::fmin(fx1, std::ceil(hx) + 1.0f);
+    fy0 = std::fmax(fy0, std::floor(ly) - 1.0f); fy1 = std::fmin(fy1, std::ceil(hy) + 1.0f);
+  }
+  if (!any || !(fx0 <= fx1 && fy0 <= fy1)) return false;
+  *x0 = (uint32_t)fx0; *x1 = (uint32_t)fx1; *y0 = (uint32_t)fy0; *y1 = (uint32_t)fy1;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1230,6 +1298,107 @@ void oracle_sample_bsdf(const float n[3], const float color[3], float rough, flo
   weight[0] = w.x;
   weight[1] = w.y;
   weight[2] = w.z;
+}
+
+int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* g, float* depth,
+                          uint16_t* normal, uint16_t* color, uint16_t* params, uint16_t* motion) {
+  if (!frame || !scene || !g || !depth || !normal || !color || !params || !motion) return -1;
+  const uint32_t W = frame->size[0], H = frame->size[1], n_obj = scene->n_objects;
+  std::vector<float> mvp(16 * (size_t)n_obj), pmvp(16 * (size_t)n_obj);
+  std::vector<uint32_t> base(n_obj + 1, 0);
+  for (uint32_t o = 0; o < n_obj; ++o) {
+    mat4_mul(frame->vp, scene->objects[o].model, &mvp[16 * o]);
+    mat4_mul(frame->previous_vp, scene->objects[o].previous, &pmvp[16 * o]);
+    base[o + 1] = base[o] + scene->meshes[scene->objects[o].info[0]].geometry[3] / 3u;
+  }
+  auto corners = [&](uint32_t o, uint32_t prim, uint32_t vi[3], Clip c[3]) {
+    const irt_mesh& me = scene->meshes[scene->objects[o].info[0]];
+    for (int k = 0; k < 3; ++k) {
+      vi[k] = me.geometry[0] + g->indices[me.geometry[2] + 3u * prim + (uint32_t)k];
+      c[k] = clip_of(&mvp[16 * o], g->vertices[vi[k]].position);
+    }
+  };
+  const uint64_t clear = (uint64_t)0x3f800000u << 32 | 0xffffffffu;
+  std::vector<uint64_t> vis((size_t)W * H, clear);
+  for (uint32_t o = 0; o < n_obj; ++o)
+    for (uint32_t prim = 0; prim + base[o] < base[o + 1]; ++prim) {
+      const uint32_t t = base[o] + prim;
+      uint32_t vi[3];
+      Clip c[3];
+      corners(o, prim, vi, c);
+      float e[9], z[3];
+      const float det = edge_planes(c, e);
+      if (!(det > 0.0f)) continue;
+      for (int k = 0; k < 3; ++k) z[k] = (c[0].z * e[k] + c[1].z * e[3 + k] + c[2].z * e[6 + k]) / det;
+      uint32_t x0, x1, y0, y1;
+      if (!tri_bounds(c, W, H, &x0, &x1, &y0, &y1)) continue;
+      for (uint32_t y = y0; y <= y1; ++y)
+        for (uint32_t x = x0; x <= x1; ++x) {
+          const float px = ndc_x(x, W), py = ndc_y(y, H);
+          const float e0 = plane_at(e, px, py), e1 = plane_at(e + 3, px, py), e2 = plane_at(e + 6, px, py);
+          if (!(e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f && e0 + e1 + e2 > 0.0f)) continue;
+          const float zz = plane_at(z, px, py) + 0.0f;
+          if (!(zz >= 0.0f && zz < 1.0f)) continue;
+          uint32_t zb;
+          std::memcpy(&zb, &zz, 4);
+          const uint64_t key = (uint64_t)zb << 32 | (uint64_t)(~t);
+          uint64_t& slot = vis[(size_t)y * W + x];
+          if (key < slot) slot = key;
+        }
+    }
+#pragma omp parallel for schedule(static)
+  for (int yy = 0; yy < (int)H; ++yy)
+    for (uint32_t x = 0; x < W; ++x) {
+      const uint32_t y = (uint32_t)yy;
+      const size_t id = (size_t)y * W + x;
+      const uint64_t key = vis[id];
+      uint16_t* np = normal + 4 * id;
+      uint16_t* cp = color + 4 * id;
+      if (key == clear) {
+        depth[id] = 1.0f;
+        for (int k = 0; k < 4; ++k) np[k] = cp[k] = 0;
+        params[2 * id] = params[2 * id + 1] = motion[2 * id] = motion[2 * id + 1] = 0;
+        continue;
+      }
+      const uint32_t t = ~(uint32_t)(key & 0xffffffffu);
+      uint32_t o = 0;
+      while (base[o + 1] <= t) ++o;
+      uint32_t vi[3];
+      Clip c[3];
+      corners(o, t - base[o], vi, c);
+      float e[9];
+      edge_planes(c, e);
+      const float px = ndc_x(x, W), py = ndc_y(y, H);
+      const float E0 = plane_at(e, px, py), E1 = plane_at(e + 3, px, py), E2 = plane_at(e + 6, px, py);
+      const float sum = E0 + E1 + E2;
+      const float l0 = E0 / sum, l1 = E1 / sum, l2 = E2 / sum;
+      const irt_object& ob = scene->objects[o];
+      Vf n[3];
+      Clip pc[3];
+      for (int k = 0; k < 3; ++k) {
+        const irt_vertex& vx = g->vertices[vi[k]];
+        n[k] = normalize3(xform(ob.model, vx.normal[0], vx.normal[1], vx.normal[2], 0.0f));
+        pc[k] = clip_of(&pmvp[16 * o], vx.position);
+      }
+      const Vf ni = V<float>(n[0].x * l0 + n[1].x * l1 + n[2].x * l2, n[0].y * l0 + n[1].y * l1 + n[2].y * l2,
+                             n[0].z * l0 + n[1].z * l1 + n[2].z * l2);
+      const float cx = c[0].x * l0 + c[1].x * l1 + c[2].x * l2, cy = c[0].y * l0 + c[1].y * l1 + c[2].y * l2;
+      const float cw = c[0].w * l0 + c[1].w * l1 + c[2].w * l2;
+      const float qx = pc[0].x * l0 + pc[1].x * l1 + pc[2].x * l2, qy = pc[0].y * l0 + pc[1].y * l1 + pc[2].y * l2;
+      const float qw = pc[0].w * l0 + pc[1].w * l1 + pc[2].w * l2;
+      const irt_material& m = g->materials[ob.info[1]];
+      const Vf sn = normalize3(ni);
+      uint32_t zb = (uint32_t)(key >> 32);
+      std::memcpy(&depth[id], &zb, 4);
+      np[0] = f32_to_f16(sn.x); np[1] = f32_to_f16(sn.y); np[2] = f32_to_f16(sn.z); np[3] = f32_to_f16(1.0f);
+      cp[0] = f32_to_f16(m.color[0]); cp[1] = f32_to_f16(m.color[1]); cp[2] = f32_to_f16(m.color[2]);
+      cp[3] = f32_to_f16(1.0f);
+      params[2 * id] = f32_to_f16(clamp32(m.params[0], 0.045f, 1.0f));
+      params[2 * id + 1] = f32_to_f16(clamp32(m.params[1], 0.0f, 1.0f));
+      motion[2 * id] = f32_to_f16((cx / cw - qx / qw) * 0.5f);
+      motion[2 * id + 1] = f32_to_f16((cy / cw - qy / qw) * -0.5f);
+    }
+  return 0;
 }
 
 }  // extern "C"

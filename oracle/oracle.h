@@ -136,6 +136,12 @@ int oracle_generate_vpls(const irt_frame* frame, const oracle_scene* scene, cons
 /* Closest hit of one ray (world space); returns 1 on a hit. */
 int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, const float origin[3],
                  const float direction[3], float* t, float* u, float* v, uint32_t* object, uint32_t* prim);
+/* The G-buffer pass (gbuffer.wgsl:1-12, raster state renderer.rs:1370-1415)
+ * for frame->size[0] x frame->size[1] pixels, in the reference formats:
+ * depth float[W*H], normal/color binary16[4*W*H], params/motion
+ * binary16[2*W*H].  Untextured materials.  Returns 0, or -1. */
+int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
+                          float* depth, uint16_t* normal, uint16_t* color, uint16_t* params, uint16_t* motion);
 /* sample_bsdf (common.wgsl:86-97) for an untextured surface. */
 void oracle_sample_bsdf(const float n[3], const float color[3], float rough, float metal, const float wo[3],
                         uint32_t* rng, float direction[3], float weight[3]);

@@ -1,0 +1,137 @@
+"""The G-buffer pass on the GPU (irt_render_gbuffer) against the oracle.
+
+The kernels and the oracle evaluate the same edge functions, depth planes,
+keys and fragment arithmetic with one rounding per operation, so every plane
+must be identical bit for bit (both storage profiles hold the reference's
+binary16 targets without loss).  A frame lit from the rendered G-buffer and
+the generated VPL pool equals the same frame lit from those inputs uploaded.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from instantradiositytest_amd import _lib
+from instantradiositytest_amd.renderer import Renderer
+from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
+from test_oracle_raster import box_room, frame_of
+from instantradiositytest_amd.scenes import look_at_camera
+
+pytestmark = pytest.mark.gpu
+
+PLANES = ("depth", "normal", "color", "params", "motion")
+
+
+def device_planes(r):
+    return tuple(r.read(name).copy() for name in PLANES)
+
+
+def compare(got, ref, label):
+    for name, a, b in zip(PLANES, got, ref):
+        a = np.ascontiguousarray(a).reshape(b.shape)
+        diff = np.any((a != b).reshape(b.shape[0], b.shape[1], -1), axis=2)
+        print(f"[gbuffer] {label} {name}: {diff.mean():.6f} of pixels differ", flush=True)
+        assert not diff.any(), f"{label}: {name} differs at {int(diff.sum())} pixels"
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_box_room_matches_oracle(require_gpu, precision):
+    W, H = 160, 100
+    scene, O = box_room()
+    moved = O.copy()
+    moved[6]["position"] = (0.45, 0.3, -0.55)
+    scene.set_objects(moved)
+    prev_cam = look_at_camera((1.2, 1.1, 2.0), (0.0, 0.6, -1.0))
+    cam = look_at_camera((1.1, 1.15, 2.05), (0.0, 0.6, -1.0))
+    cfg = CONFIGS["plain-640x400"].scaled(W, H, 1)
+    r = Renderer(scene, cfg.settings(), precision=precision, schedule="fused")
+    try:
+        geo = scene.geometry()
+        r.set_geometry(*geo)
+        f = frame_of(scene, cam, prev_cam, W, H, r.settings)
+        assert r.L.irt_set_frame(r.ctx, f.ctypes.data) == 0
+        r.render_gbuffer()
+        r.sync()
+        got = device_planes(r)
+        ref = oracle.render_gbuffer(f, oracle.SceneRecords.from_scene(scene), oracle.GeometryRecords(*geo))
+        compare(got, ref, f"box room {precision}")
+    finally:
+        r.close()
+
+
+@pytest.mark.parametrize("size", [(160, 100), (640, 400)])
+def test_config_room_matches_oracle(require_gpu, size):
+    W, H = size
+    cfg = CONFIGS["plain-640x400"].scaled(W, H, 3)
+    cs = ConfigScene(cfg)
+    r = Renderer(cs.scene, cfg.settings(), precision="mixed", schedule="fused")
+    try:
+        geo = cs.scene.geometry()
+        r.set_geometry(*geo)
+        sr = oracle.SceneRecords.from_scene(cs.scene)
+        g = oracle.GeometryRecords(*geo)
+        for fi in range(2):
+            inp = cs.frame_inputs(fi)
+            inp.depth = None
+            f = r.begin_frame(cs.scene, inp)
+            r.sync()
+            compare(device_planes(r), oracle.render_gbuffer(f, sr, g), f"room {W}x{H} frame {fi}")
+            r.lighting()
+            r.svgf()
+            r.end_frame()
+            r.finish_frame()
+    finally:
+        r.close()
+
+
+def test_frame_from_rendered_inputs_equals_frame_from_uploaded_inputs(require_gpu):
+    cfg = CONFIGS["plain-640x400"].scaled(128, 80, 3)
+    results = []
+    staged = []
+    for mode in ("render", "upload"):
+        cs = ConfigScene(cfg)
+        r = Renderer(cs.scene, cfg.settings(), precision="mixed", schedule="fused")
+        try:
+            r.set_geometry(*cs.scene.geometry())
+            outs = []
+            for fi in range(3):
+                inp = cs.frame_inputs(fi)
+                if mode == "render":
+                    inp.depth = inp.normal = inp.color = inp.params = inp.motion = None
+                    inp.vpls = None
+                else:
+                    inp.depth, inp.normal, inp.color, inp.params, inp.motion, inp.vpls = staged[fi]
+                r.begin_frame(cs.scene, inp)
+                if mode == "render":
+                    r.sync()
+                    V = r.settings.vpl_count
+                    slot = fi % (r.settings.restir_max_history + 1)
+                    pool = r.read("vpl_archive")[slot * V:(slot + 1) * V].copy()
+                    planes = device_planes(r)
+                    staged.append((planes[0], planes[1], planes[2], planes[3], planes[4], pool))
+                r.lighting()
+                r.svgf()
+                r.end_frame()
+                r.finish_frame()
+                r.sync()
+                outs.append((r.read("lighting").copy(), r.read("history").copy()))
+            results.append(outs)
+        finally:
+            r.close()
+    for (la, ha), (lb, hb) in zip(*results):
+        assert np.array_equal(la, lb) and np.array_equal(ha, hb)
+
+
+def test_render_gbuffer_needs_geometry(require_gpu):
+    cfg = CONFIGS["plain-640x400"].scaled(32, 16, 1)
+    cs = ConfigScene(cfg)
+    r = Renderer(cs.scene, cfg.settings(), precision="fp32", schedule="fused")
+    try:
+        r.begin_frame(cs.scene, cs.frame_inputs(0))
+        assert _lib.irt().irt_render_gbuffer(r.ctx) == -4
+        r.set_geometry(*cs.scene.geometry())
+        assert _lib.irt().irt_render_gbuffer(r.ctx) == 0
+        r.sync()
+    finally:
+        r.close()
