@@ -10,9 +10,9 @@ VPL pool are generated up front and staged in HBM; the timed region starts
 with them resident and covers G-buffer ingest + lighting + SVGF + history
 per frame.
 
-Legs (one JSON line carries all of them; vpl_pass is reported beside the
-headline, not inside its frame, because the path's input VPL pool is staged
-like the G-buffer):
+Legs (one JSON line carries all of them; the G-buffer and VPL passes are
+reported beside the headline and together with it in full_frame, because the
+headline frame takes its G-buffer and VPL pool as staged inputs):
   headline   fused kernels, mixed storage (16-bit G-buffer / radiance / SDF
              samples, FP32 reservoirs, moments and march), single view
   plain      one kernel per reference pass, FP32 storage: the plain FP32
@@ -20,6 +20,10 @@ like the G-buffer):
   fused_fp32 fused kernels, FP32 storage
   vpls       the VPL pass (vpl.wgsl) on the same scene's triangles: launch
              time, rays/s, and the oracle's rate for the same pool
+  gbuffer    the G-buffer pass (gbuffer.wgsl raster) of the same scene:
+             launch time and the oracle's time for the same frame
+  full       the whole reference frame per view on the GPU: G-buffer pass,
+             VPL pass, lighting, SVGF, history (nothing staged)
   batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
              launch per pass, autotuned tiles, HIP graph replay
   serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
@@ -297,6 +301,92 @@ def run_vpls(cs, settings, host_frames, steps: int, threads: int, cpu: bool) -> 
     return out
 
 
+def run_gbuffer(cs, settings, host_frames, steps: int, threads: int, cpu: bool) -> dict:
+    """The G-buffer pass (irt_render_gbuffer) of frame 0 at the workload's size:
+    mean time over `steps` launches, and the oracle's time for the same frame."""
+    import oracle
+    from instantradiositytest_amd._lib import check
+    from instantradiositytest_amd.renderer import Renderer, pack_frame
+    from instantradiositytest_amd.types import ptr
+    r = Renderer(cs.scene, settings, precision="mixed", schedule="fused")
+    try:
+        geo = cs.scene.geometry()
+        r.set_geometry(*geo)
+        vp, cam = host_frames[1]
+        pvp, _ = host_frames[0]
+        f = pack_frame(settings, vp, pvp, cam, 1, cs.scene.generation, cs.scene.generation, True, 0,
+                       len(cs.scene.lights), len(cs.scene.objects))
+        check(r.L.irt_set_frame(r.ctx, ptr(f)), r.ctx)
+        for _ in range(3):
+            r.render_gbuffer()
+        r.sync()
+        r.enable_timing(True)
+        for _ in range(steps):
+            r.render_gbuffer()
+        r.sync()
+        ms = [t for name, t in pass_time_entries(r) if name == "gbuffer"]
+        r.enable_timing(False)
+        cpu_s = None
+        if cpu:
+            t0 = time.perf_counter()
+            oracle.render_gbuffer(f, oracle.SceneRecords.from_scene(cs.scene), oracle.GeometryRecords(*geo))
+            cpu_s = time.perf_counter() - t0
+    finally:
+        r.close()
+    W, H = settings.render_width, settings.render_height
+    objs = cs.scene.objects
+    inst = int(sum(cs.scene.meshes[o["info"][0]]["geometry"][3] // 3 for o in objs))
+    launch_ms = float(np.mean(ms))
+    out = {"launch_ms": round(launch_ms, 4), "launches": len(ms), "width": W, "height": H,
+           "instance_triangles": inst, "mpix_s": round(W * H / launch_ms / 1e3, 1)}
+    if cpu_s is not None:
+        out["cpu_baseline"] = {"value": round(W * H / cpu_s / 1e6, 3), "unit": "Mpix/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's G-buffer pass (raster loop on one core, resolve on "
+                                         f"{threads} threads) for the same frame, {cpu_s:.2f} s"}
+    log(f"[bench] gbuffer: {out}")
+    return out
+
+
+def run_full_frame(cs, settings, host_frames, warmup: int, steps: int) -> dict:
+    """The whole reference frame on the GPU for one view: G-buffer pass, VPL
+    pass, lighting, SVGF and history (Renderer::encode's passes), fused
+    kernels, mixed storage; the host uploads only the objects, lights and
+    the Frame record each frame."""
+    from instantradiositytest_amd._lib import check
+    from instantradiositytest_amd.renderer import Renderer
+    from instantradiositytest_amd.scenes import FrameInputs
+    r = Renderer(cs.scene, settings, precision="mixed", schedule="fused")
+    try:
+        r.set_geometry(*cs.scene.geometry())
+
+        def submit(fi: int):
+            vp, cam = host_frames[fi]
+            r.begin_frame(cs.scene, FrameInputs(None, None, None, None, None, None, vp, cam))
+            check(r.L.irt_encode(r.ctx), r.ctx)
+            r.finish_frame()
+
+        for fi in range(warmup):
+            submit(fi)
+        r.sync()
+        r.enable_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            submit(warmup + k)
+        r.sync()
+        wall = time.perf_counter() - t0
+        per = {}
+        for name, ms in pass_time_entries(r):
+            per.setdefault(name, []).append(ms)
+        r.enable_timing(False)
+    finally:
+        r.close()
+    W, H = settings.render_width, settings.render_height
+    out = {"mpix_s": round(W * H * steps / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / steps, 4),
+           "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in per.items()}}
+    log(f"[bench] full frame: {out}")
+    return out
+
+
 def stage_serving_streams(variants: int, frames: int, seed: int):
     from instantradiositytest_amd.serving import SERVING_MIX, SceneStream, session_config
     rng = np.random.default_rng(seed)
@@ -357,7 +447,8 @@ def main() -> int:
     ap.add_argument("--serve-sweep", type=lambda v: [int(x) for x in v.split(",") if x], default=[4, 8, 16],
                     help="resident-session caps of the short serving runs after the main one")
     ap.add_argument("--sweep-seconds", type=float, default=10.0)
-    ap.add_argument("--only", default="", help="comma list of legs: headline,plain,fused_fp32,vpls,batched,serving")
+    ap.add_argument("--only", default="",
+                    help="comma list of legs: headline,plain,fused_fp32,vpls,gbuffer,full,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     args = ap.parse_args()
@@ -365,8 +456,8 @@ def main() -> int:
         print(f"bench.py measures one GPU: the path renders each view and each session on one GPU with no "
               f"cross-GPU work; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
-    legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "vpls", "batched",
-                                                        "serving"}
+    legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "vpls", "gbuffer", "full",
+                                                        "batched", "serving"}
 
     from instantradiositytest_amd import hip
     from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
@@ -452,11 +543,18 @@ def main() -> int:
     if head_r is not None:
         head_r.close()
     del staged
+    threads = min(16, len(os.sched_getaffinity(0)))
     if "vpls" in legs:
-        threads = min(16, len(os.sched_getaffinity(0)))
         line["vpl_pass"] = run_vpls(cs, settings, host_frames, 50, threads, not args.no_cpu_baseline)
         if head:
             line["vpl_pass"]["share_of_headline_frame"] = round(line["vpl_pass"]["launch_ms"] / head["ms_per_step"], 4)
+    if "gbuffer" in legs:
+        line["gbuffer_pass"] = run_gbuffer(cs, settings, host_frames, 50, threads, not args.no_cpu_baseline)
+        if head:
+            line["gbuffer_pass"]["share_of_headline_frame"] = round(
+                line["gbuffer_pass"]["launch_ms"] / head["ms_per_step"], 4)
+    if "full" in legs:
+        line["full_frame"] = run_full_frame(cs, settings, host_frames, args.warmup, args.steps)
 
     if "batched" in legs:
         line["batched"] = run_batched(12, 4, args.batch_steps)

@@ -117,6 +117,8 @@ struct irt_ctx {
   uint32_t cap_gb_objs = 0;
   GbTri* gb_tris = nullptr;
   uint32_t cap_gb_tris = 0;
+  GbAttr* gb_attrs = nullptr;
+  uint32_t cap_gb_attrs = 0;
   uint32_t* gb_tiles = nullptr;
   uint32_t cap_gb_tiles = 0;
   uint32_t* gb_scan = nullptr;
@@ -125,6 +127,7 @@ struct irt_ctx {
   uint32_t cap_gb_scan_tmp = 0;
   unsigned long long* gb_vis = nullptr;
   uint32_t cap_gb_vis = 0;
+  bool gb_vis_clear = false;  // gb_vis holds the clear key (each resolve leaves it so)
   // G-buffer slots (internal formats).  The current and previous frames'
   // G-buffers are slots gb_cur / gb_prev (the same slot right after a frame
   // ends, as when the reference copies the G-buffer); a write into the
@@ -1508,10 +1511,13 @@ int irt_render_gbuffer(irt_ctx* c) {
   int rc = ensure_array(c, &c->obj_tri_base, &c->cap_obj_tri_base, c->n_objects + 1);
   if (!rc) rc = ensure_array(c, &c->gb_objs, &c->cap_gb_objs, c->n_objects);
   if (!rc) rc = ensure_array(c, &c->gb_tris, &c->cap_gb_tris, n);
+  if (!rc) rc = ensure_array(c, &c->gb_attrs, &c->cap_gb_attrs, n);
   if (!rc) rc = ensure_array(c, &c->gb_tiles, &c->cap_gb_tiles, n);
   if (!rc) rc = ensure_array(c, &c->gb_scan, &c->cap_gb_scan, n);
+  unsigned long long* vis_before = c->gb_vis;
   if (!rc) rc = ensure_array(c, &c->gb_vis, &c->cap_gb_vis, (uint32_t)c->npx);
   if (rc) return rc;
+  if (c->gb_vis != vis_before) c->gb_vis_clear = false;
   size_t tmp_bytes = 0;
   HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, c->gb_tiles, c->gb_scan, (int)std::max(n, 1u),
                                                 c->stream));
@@ -1531,10 +1537,13 @@ int irt_render_gbuffer(irt_ctx* c) {
   timer_begin(c, "gbuffer");
   hipLaunchKernelGGL(k_gb_objects, dim3((c->n_objects + 63) / 64), dim3(64), 0, c->stream, c->objects, c->n_objects,
                      c->frame, c->gb_objs);
-  hipLaunchKernelGGL(k_gb_clear, dim3((npx + 255) / 256), dim3(256), 0, c->stream, c->gb_vis, npx);
+  if (!c->gb_vis_clear) {
+    hipLaunchKernelGGL(k_gb_clear, dim3((npx + 255) / 256), dim3(256), 0, c->stream, c->gb_vis, npx);
+    c->gb_vis_clear = true;
+  }
   if (n) {
     hipLaunchKernelGGL(k_gb_setup, dim3((n + 255) / 256), dim3(256), 0, c->stream, sc, c->gb_objs, n, W, H,
-                       c->gb_tris, c->gb_tiles);
+                       c->gb_tris, c->gb_attrs, c->gb_tiles);
     HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(c->gb_scan_tmp, tmp_bytes, c->gb_tiles, c->gb_scan, (int)n,
                                                   c->stream));
     hipLaunchKernelGGL(k_gb_raster, dim3(c->march_blocks), dim3(256), 0, c->stream, c->gb_tris, c->gb_scan, n, W, H,
@@ -1542,11 +1551,13 @@ int irt_render_gbuffer(irt_ctx* c) {
   }
   const dim3 tiles((W + GB_TILE - 1) / GB_TILE, (H + GB_TILE - 1) / GB_TILE);
   if (mixed(c))
-    hipLaunchKernelGGL(k_gb_resolve<ProfileMixed>, tiles, dim3(256), 0, c->stream, sc, c->gb_objs, c->gb_vis, W, H,
-                       c->depth, (h4*)c->normal, (h4*)c->color, (h2*)c->params, (h2*)c->motion);
+    hipLaunchKernelGGL(k_gb_resolve<ProfileMixed>, tiles, dim3(256), 0, c->stream, c->gb_tris, c->gb_attrs,
+                       c->materials, c->gb_vis, W, H, c->depth, (h4*)c->normal, (h4*)c->color, (h2*)c->params,
+                       (h2*)c->motion);
   else
-    hipLaunchKernelGGL(k_gb_resolve<ProfileF32>, tiles, dim3(256), 0, c->stream, sc, c->gb_objs, c->gb_vis, W, H,
-                       c->depth, (float4*)c->normal, (float4*)c->color, (float2*)c->params, (float2*)c->motion);
+    hipLaunchKernelGGL(k_gb_resolve<ProfileF32>, tiles, dim3(256), 0, c->stream, c->gb_tris, c->gb_attrs,
+                       c->materials, c->gb_vis, W, H, c->depth, (float4*)c->normal, (float4*)c->color,
+                       (float2*)c->params, (float2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);
   return IRT_OK;

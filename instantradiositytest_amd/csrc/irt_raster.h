@@ -24,11 +24,13 @@
 // Equal-test G-buffer pass leaves in the targets.
 //
 // Work: k_gb_setup — one lane per instance triangle: corners, culling, the
-// edge/depth planes and the 16x16 pixel tiles of its screen bounds;
+// edge/depth planes, the 16x16 pixel tiles of its screen bounds and the
+// vertex-stage outputs of the drawn ones;
 // an inclusive scan of the tile counts; k_gb_raster — persistent
 // workgroups, one 16x16 tile of one triangle per step, one pixel per lane;
 // k_gb_resolve — one lane per pixel: the winner's attributes, interpolated
-// and shaded as the fragment shader does, written in the reference formats.
+// and shaded as the fragment shader does, written in the reference formats;
+// it leaves the visibility buffer cleared for the next frame.
 #pragma once
 
 #include "irt_device.h"
@@ -50,6 +52,16 @@ struct __align__(16) GbTri {
   float e[9];      // edge functions: E_i(p) = e[3i] px + e[3i+1] py + e[3i+2]
   float z[3];      // z_ndc(p) = z[0] px + z[1] py + z[2]
   uint32_t bx0, by0, bw, pad;  // first tile and tiles per row of the screen bounds
+};
+
+// Per drawn triangle, written by k_gb_setup: the vertex-stage outputs the
+// resolve interpolates (gbuffer.wgsl:2-6): world normals, current and
+// previous clip x, y, w of the three corners, and the material.
+struct __align__(16) GbAttr {
+  float n[9];
+  float cur[9];
+  float prev[9];
+  uint32_t material;
 };
 
 struct GbScene {
@@ -121,7 +133,7 @@ __device__ __forceinline__ float plane_at(const float* e, float px, float py) { 
 
 __global__ __launch_bounds__(256) void k_gb_setup(GbScene sc, const GbObject* __restrict__ objs, uint32_t n_tris,
                                                   uint32_t W, uint32_t H, GbTri* __restrict__ tris,
-                                                  uint32_t* __restrict__ tiles) {
+                                                  GbAttr* __restrict__ attrs, uint32_t* __restrict__ tiles) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tris) return;
   uint32_t obj, vi[3];
@@ -175,6 +187,20 @@ __global__ __launch_bounds__(256) void k_gb_setup(GbScene sc, const GbObject* __
   if (!count) g.bx0 = g.by0 = g.bw = 0;
   tris[t] = g;
   tiles[t] = count;
+  if (!count) return;
+  const irt_object& ob = sc.objects[obj];
+  GbAttr a;
+  for (int k = 0; k < 3; ++k) {
+    const irt_vertex& vx = sc.vertices[vi[k]];
+    const float4 wn = mat_mul(ob.model, vx.normal[0], vx.normal[1], vx.normal[2], 0.0f);
+    const v3 nk = normalize(v3{wn.x, wn.y, wn.z});
+    const float4 pk = mat_mul(objs[obj].pmvp, vx.position[0], vx.position[1], vx.position[2], vx.position[3]);
+    a.n[3 * k] = nk.x; a.n[3 * k + 1] = nk.y; a.n[3 * k + 2] = nk.z;
+    a.cur[3 * k] = c[k].x; a.cur[3 * k + 1] = c[k].y; a.cur[3 * k + 2] = c[k].w;
+    a.prev[3 * k] = pk.x; a.prev[3 * k + 1] = pk.y; a.prev[3 * k + 2] = pk.w;
+  }
+  a.material = ob.info[1];
+  attrs[t] = a;
 }
 
 // Persistent: workgroup steps over (triangle, tile) items; `scan` is the
@@ -219,14 +245,15 @@ __device__ __forceinline__ void st_rg16(float2* p, uint32_t i, float a, float b)
 }
 __device__ __forceinline__ void st_rg16(h2* p, uint32_t i, float a, float b) { p[i] = h2{(_Float16)a, (_Float16)b}; }
 
-// Fragment stage (gbuffer.wgsl:7-11) of each pixel's winning triangle; the
-// vertex stage outputs (gbuffer.wgsl:2-6) are recomputed for its corners and
-// interpolated with the perspective-correct barycentrics at the pixel centre.
+// Fragment stage (gbuffer.wgsl:7-11) of each pixel's winning triangle: its
+// vertex-stage outputs (k_gb_setup) interpolated with the perspective-correct
+// barycentrics at the pixel centre.
 // Values are rounded as the reference's targets hold them (Depth32Float,
 // Rgba16F, Rg16F) and stored in the profile's G-buffer formats.
 template <class P>
-__global__ __launch_bounds__(256) void k_gb_resolve(GbScene sc, const GbObject* __restrict__ objs,
-                                                    const unsigned long long* __restrict__ vis, uint32_t W,
+__global__ __launch_bounds__(256) void k_gb_resolve(const GbTri* __restrict__ tris, const GbAttr* __restrict__ attrs,
+                                                    const irt_material* __restrict__ materials,
+                                                    unsigned long long* __restrict__ vis, uint32_t W,
                                                     uint32_t H, float* __restrict__ depth,
                                                     typename P::N4* __restrict__ normal,
                                                     typename P::N4* __restrict__ color,
@@ -237,6 +264,7 @@ __global__ __launch_bounds__(256) void k_gb_resolve(GbScene sc, const GbObject* 
   if (x >= W || y >= H) return;
   const uint32_t id = y * W + x;
   const unsigned long long key = vis[id];
+  vis[id] = GB_CLEAR_KEY;  // cleared for the next frame's raster
   if (key == GB_CLEAR_KEY) {
     depth[id] = 1.0f;
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -247,43 +275,28 @@ __global__ __launch_bounds__(256) void k_gb_resolve(GbScene sc, const GbObject* 
     return;
   }
   const uint32_t t = ~(uint32_t)(key & 0xffffffffu);
-  uint32_t obj, vi[3];
-  float4 c[3];
-  tri_corners(sc, objs, t, obj, vi, c);
-  float e[9];
-  edge_planes(c, e);
+  const GbTri& g = tris[t];
+  const GbAttr& a = attrs[t];
   const float px = ndc_x(x, W), py = ndc_y(y, H);
-  const float E0 = plane_at(e, px, py), E1 = plane_at(e + 3, px, py), E2 = plane_at(e + 6, px, py);
+  const float E0 = plane_at(g.e, px, py), E1 = plane_at(g.e + 3, px, py), E2 = plane_at(g.e + 6, px, py);
   const float sum = E0 + E1 + E2;
   const float l0 = E0 / sum, l1 = E1 / sum, l2 = E2 / sum;
-  const irt_object& ob = sc.objects[obj];
-  // vertex outputs: world normal, current and previous clip positions
-  v3 n[3];
-  float4 pc[3];
-  for (int k = 0; k < 3; ++k) {
-    const irt_vertex& vx = sc.vertices[vi[k]];
-    const float4 wn = mat_mul(ob.model, vx.normal[0], vx.normal[1], vx.normal[2], 0.0f);
-    n[k] = normalize(v3{wn.x, wn.y, wn.z});
-    pc[k] = mat_mul(objs[obj].pmvp, vx.position[0], vx.position[1], vx.position[2], vx.position[3]);
-  }
-  const v3 ni = v3{n[0].x * l0 + n[1].x * l1 + n[2].x * l2, n[0].y * l0 + n[1].y * l1 + n[2].y * l2,
-                   n[0].z * l0 + n[1].z * l1 + n[2].z * l2};
-  const float4 cur = make_float4(c[0].x * l0 + c[1].x * l1 + c[2].x * l2, c[0].y * l0 + c[1].y * l1 + c[2].y * l2,
-                                 0.0f, c[0].w * l0 + c[1].w * l1 + c[2].w * l2);
-  const float4 prev = make_float4(pc[0].x * l0 + pc[1].x * l1 + pc[2].x * l2,
-                                  pc[0].y * l0 + pc[1].y * l1 + pc[2].y * l2, 0.0f,
-                                  pc[0].w * l0 + pc[1].w * l1 + pc[2].w * l2);
+  const v3 ni = v3{a.n[0] * l0 + a.n[3] * l1 + a.n[6] * l2, a.n[1] * l0 + a.n[4] * l1 + a.n[7] * l2,
+                   a.n[2] * l0 + a.n[5] * l1 + a.n[8] * l2};
+  const float cx = a.cur[0] * l0 + a.cur[3] * l1 + a.cur[6] * l2, cy = a.cur[1] * l0 + a.cur[4] * l1 + a.cur[7] * l2;
+  const float cw = a.cur[2] * l0 + a.cur[5] * l1 + a.cur[8] * l2;
+  const float qx = a.prev[0] * l0 + a.prev[3] * l1 + a.prev[6] * l2;
+  const float qy = a.prev[1] * l0 + a.prev[4] * l1 + a.prev[7] * l2;
+  const float qw = a.prev[2] * l0 + a.prev[5] * l1 + a.prev[8] * l2;
   // material_at (common.wgsl:50-66), untextured
-  const irt_material& m = sc.materials[ob.info[1]];
+  const irt_material& m = materials[a.material];
   const v3 sn = normalize(ni);
   const float rough = clampf(m.params[0], 0.045f, 1.0f), metal = clampf(m.params[1], 0.0f, 1.0f);
   depth[id] = __uint_as_float((uint32_t)(key >> 32));
   st_r16(normal, id, make_float4(sn.x, sn.y, sn.z, 1.0f));
   st_r16(color, id, make_float4(m.color[0], m.color[1], m.color[2], 1.0f));
   st_rg16(params, id, rough, metal);
-  const float mx = (cur.x / cur.w - prev.x / prev.w) * 0.5f;
-  const float my = (cur.y / cur.w - prev.y / prev.w) * -0.5f;
-  st_rg16(motion, id, mx, my);
+  st_rg16(motion, id, (cx / cw - qx / qw) * 0.5f, (cy / cw - qy / qw) * -0.5f);
 }
 
 }  // namespace irt
