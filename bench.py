@@ -23,7 +23,7 @@ headline frame takes its G-buffer and VPL pool as staged inputs):
   gbuffer    the G-buffer pass (gbuffer.wgsl raster) of the same scene:
              launch time and the oracle's time for the same frame
   full       the whole reference frame per view on the GPU: G-buffer pass,
-             VPL pass, lighting, SVGF, history (nothing staged)
+             VPL pass, lighting, SVGF, history, presentation (nothing staged)
   batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
              launch per pass, autotuned tiles, HIP graph replay
   serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
@@ -349,13 +349,15 @@ def run_gbuffer(cs, settings, host_frames, steps: int, threads: int, cpu: bool) 
 
 def run_full_frame(cs, settings, host_frames, warmup: int, steps: int) -> dict:
     """The whole reference frame on the GPU for one view: G-buffer pass, VPL
-    pass, lighting, SVGF and history (Renderer::encode's passes), fused
-    kernels, mixed storage; the host uploads only the objects, lights and
-    the Frame record each frame."""
+    pass, lighting, SVGF, history and presentation into the sRGB display
+    (Renderer::encode's passes), fused kernels, mixed storage; the host
+    uploads only the objects, lights and the Frame record each frame."""
     from instantradiositytest_amd._lib import check
     from instantradiositytest_amd.renderer import Renderer
     from instantradiositytest_amd.scenes import FrameInputs
+    from instantradiositytest_amd import hip
     r = Renderer(cs.scene, settings, precision="mixed", schedule="fused")
+    display = hip.DeviceBuffer(settings.render_width * settings.render_height * 4)
     try:
         r.set_geometry(*cs.scene.geometry())
 
@@ -363,6 +365,7 @@ def run_full_frame(cs, settings, host_frames, warmup: int, steps: int) -> dict:
             vp, cam = host_frames[fi]
             r.begin_frame(cs.scene, FrameInputs(None, None, None, None, None, None, vp, cam))
             check(r.L.irt_encode(r.ctx), r.ctx)
+            check(r.L.irt_present_device(r.ctx, display.ptr), r.ctx)  # renderer.rs:1961
             r.finish_frame()
 
         for fi in range(warmup):

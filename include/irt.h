@@ -317,6 +317,18 @@ int irt_generate_vpls(irt_ctx* ctx);
  * materials.  Enqueued on irt_stream; call after irt_set_frame. */
 int irt_render_gbuffer(irt_ctx* ctx);
 
+/* Presentation (present.wgsl:1-8 into the Rgba8UnormSrgb capture target of
+ * renderer.rs:1300-1306, drawn at renderer.rs:1961; read back as
+ * Renderer::capture does, renderer.rs:2109-2163): the last frame's denoised
+ * history as RGBA8, row-major W*H*4 bytes: each rgb channel clamped to
+ * [0, 1], sRGB-encoded (IEC 61966-2-1) and rounded to the nearest code
+ * (irt_host_srgb_thresholds), alpha 255.  irt_present writes host memory
+ * and blocks until the frame is done; irt_present_device writes device
+ * memory, enqueued after the frame's SVGF on the stream that runs it
+ * (irt_stream_svgf under IRT_SCHEDULE_FUSED, irt_stream otherwise). */
+int irt_present(irt_ctx* ctx, uint8_t* rgba8);
+int irt_present_device(irt_ctx* ctx, void* rgba8);
+
 /* The six reservoir passes and the two resolves (renderer.rs:1889-1928). */
 int irt_lighting(irt_ctx* ctx);
 /* Svgf::encode (svgf.rs:140-261): temporal, variance, `iterations` a-trous,
@@ -535,6 +547,12 @@ int irt_host_pack_frame(const irt_settings* settings, const float vp[16], const 
                         uint32_t light_count, uint32_t object_count, irt_frame* out);
 /* 4x4 column-major inverse (double internally, rounded to float). Returns 0 if singular. */
 int irt_host_inverse(const float m[16], float out[16]);
+
+/* Decision thresholds of the display's sRGB encode: a channel value v
+ * (clamped to [0, 1]) gets the code #{k : out[k] <= v}.  out[k] is the
+ * linear value where the exact encode times 255 reaches k + 0.5, rounded up
+ * to a float. */
+void irt_host_srgb_thresholds(float out[255]);
 
 /* Node of a mesh's triangle hierarchy, 32 B, in preorder.  count == 0: an
  * inner node whose left child is the next node, link = the node after its

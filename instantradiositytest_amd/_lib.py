@@ -60,6 +60,9 @@ def irt() -> ctypes.CDLL:
         "irt_set_geometry": (c_int, [P, P, u32, P, u32, P, u32]),
         "irt_generate_vpls": (c_int, [P]),
         "irt_render_gbuffer": (c_int, [P]),
+        "irt_present": (c_int, [P, P]),
+        "irt_present_device": (c_int, [P, P]),
+        "irt_host_srgb_thresholds": (None, [P]),
         "irt_lighting": (c_int, [P]),
         "irt_svgf": (c_int, [P, u32, u32]),
         "irt_end_frame": (c_int, [P]),

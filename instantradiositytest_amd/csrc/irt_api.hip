@@ -15,6 +15,7 @@
 #include "irt_kernels.h"
 #include "irt_vpl.h"
 #include "irt_raster.h"
+#include "irt_present.h"
 
 #include <hipcub/device/device_scan.hpp>
 
@@ -128,6 +129,11 @@ struct irt_ctx {
   unsigned long long* gb_vis = nullptr;
   uint32_t cap_gb_vis = 0;
   bool gb_vis_clear = false;  // gb_vis holds the clear key (each resolve leaves it so)
+  // presentation (irt_present)
+  float* srgb_t = nullptr;  // sRGB decision thresholds
+  uint32_t cap_srgb_t = 0;
+  uchar4* present_buf = nullptr;
+  uint32_t cap_present_buf = 0;
   // G-buffer slots (internal formats).  The current and previous frames'
   // G-buffers are slots gb_cur / gb_prev (the same slot right after a frame
   // ends, as when the reference copies the G-buffer); a write into the
@@ -1560,6 +1566,42 @@ int irt_render_gbuffer(irt_ctx* c) {
                        (float2*)c->params, (float2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);
+  return IRT_OK;
+}
+
+int irt_present_device(irt_ctx* c, void* rgba8) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!rgba8) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null display buffer");
+  const bool fresh = !c->srgb_t;
+  int rc = ensure_array(c, &c->srgb_t, &c->cap_srgb_t, SRGB_THRESHOLDS);
+  if (rc) return rc;
+  if (fresh) {
+    float t[SRGB_THRESHOLDS];
+    irt_host::srgb_thresholds(t);
+    HIP_CHECK(c, hipMemcpyAsync(c->srgb_t, t, sizeof(t), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
+  // after the frame's SVGF, on the stream that runs it
+  hipStream_t s = c->schedule == IRT_SCHEDULE_FUSED ? c->svgf_stream : c->stream;
+  const uint32_t n = (uint32_t)c->npx;
+  if (mixed(c))
+    hipLaunchKernelGGL(k_present<ProfileMixed>, dim3((n + 255) / 256), dim3(256), 0, s, (const h4*)c->history, n,
+                       c->srgb_t, (uchar4*)rgba8);
+  else
+    hipLaunchKernelGGL(k_present<ProfileF32>, dim3((n + 255) / 256), dim3(256), 0, s, (const float4*)c->history, n,
+                       c->srgb_t, (uchar4*)rgba8);
+  HIP_CHECK(c, hipGetLastError());
+  return IRT_OK;
+}
+
+int irt_present(irt_ctx* c, uint8_t* rgba8) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!rgba8) return c->fail(IRT_ERR_INVALID_ARGUMENT, "null display buffer");
+  int rc = ensure_array(c, &c->present_buf, &c->cap_present_buf, (uint32_t)c->npx);
+  if (!rc) rc = irt_present_device(c, c->present_buf);
+  if (!rc) rc = sync_all(c);
+  if (rc) return rc;
+  HIP_CHECK(c, hipMemcpy(rgba8, c->present_buf, c->npx * 4, hipMemcpyDeviceToHost));
   return IRT_OK;
 }
 

@@ -348,6 +348,20 @@ float f16_to_f32(uint16_t h) {
   return f;
 }
 
+// sRGB decision thresholds of the Rgba8UnormSrgb display (IEC 61966-2-1):
+// code k + 0.5 decoded to linear in double, rounded up to the next float, so
+// that "v >= threshold" holds exactly when the exact encode of v reaches
+// k + 0.5.
+void srgb_thresholds(float out[255]) {
+  for (int k = 0; k < 255; ++k) {
+    const double c = (k + 0.5) / 255.0;
+    const double lin = c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4);
+    float f = (float)lin;
+    if ((double)f < lin) f = std::nextafter(f, INFINITY);
+    out[k] = f;
+  }
+}
+
 // ------------------------------------------------ mesh triangle hierarchy
 // The software counterpart of a triangle BLAS (renderer.rs:1061-1106): a
 // binary hierarchy over a mesh's triangles in preorder with escape links,
@@ -575,6 +589,10 @@ int irt_host_pack_frame(const irt_settings* s, const float vp[16], const float p
 int irt_host_inverse(const float m[16], float out[16]) {
   if (!m || !out) return 0;
   return irt_host::inverse4(m, out) ? 1 : 0;
+}
+
+void irt_host_srgb_thresholds(float out[255]) {
+  if (out) irt_host::srgb_thresholds(out);
 }
 
 int irt_host_mesh_hierarchy(const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,

@@ -26,6 +26,9 @@ int pack_frame(const irt_settings& s, const float* vp, const float* pvp, const f
 uint16_t f32_to_f16(float f);
 float f16_to_f32(uint16_t h);
 uint16_t f32_to_f16_down(float f);
+// The 255 linear values at which the correctly rounded 8-bit sRGB code of
+// the exact encode steps up (threshold k: code k -> k + 1).
+void srgb_thresholds(float out[255]);
 // Triangle hierarchy of one mesh (mesh-local indices); node and triangle
 // links are local to the returned arrays.
 int mesh_hierarchy(const irt_vertex* v, uint32_t nv, const uint32_t* idx, uint32_t ni,

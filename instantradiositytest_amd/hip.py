@@ -64,6 +64,14 @@ class DeviceBuffer:
         _check(lib().hipMemcpy(b.ptr, c_void_p(a.ctypes.data), a.nbytes, H2D), "hipMemcpy H2D")
         return b
 
+    def to_array(self, dtype, shape) -> np.ndarray:
+        """Blocking copy of the buffer's first bytes into a new host array."""
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().hipMemcpy(c_void_p(out.ctypes.data), self.ptr, out.nbytes, D2H), "hipMemcpy D2H")
+        return out
+
     def free(self):
         if self.ptr:
             lib().hipFree(self.ptr)

@@ -123,6 +123,19 @@ class Renderer:
         m = np.ascontiguousarray(materials)
         check(self.L.irt_set_geometry(self.ctx, ptr(v), len(v), ptr(i), len(i), ptr(m), len(m)), self.ctx)
 
+    def present(self) -> np.ndarray:
+        """The display image of the last frame (irt_present): uint8 [H, W, 4] RGBA, sRGB."""
+        out = np.empty((self.H, self.W, 4), np.uint8)
+        check(self.L.irt_present(self.ctx, ptr(out)), self.ctx)
+        return out
+
+    def capture(self, path) -> np.ndarray:
+        """Renderer::capture (renderer.rs:2109-2163): the display image as an RGBA PNG."""
+        from .images import write_png
+        img = self.present()
+        write_png(path, img)
+        return img
+
     def render_gbuffer(self):
         """The G-buffer pass of the frame set last (irt_render_gbuffer, gbuffer.wgsl)."""
         check(self.L.irt_render_gbuffer(self.ctx), self.ctx)

@@ -103,6 +103,8 @@ def lib() -> ctypes.CDLL:
                                POINTER(c_uint32), POINTER(c_uint32)]
     L.oracle_render_gbuffer.restype = c_int
     L.oracle_render_gbuffer.argtypes = [P, P, P, P, P, P, P, P]
+    L.oracle_present.restype = None
+    L.oracle_present.argtypes = [P, c_uint32, P]
     L.oracle_sample_bsdf.restype = None
     L.oracle_sample_bsdf.argtypes = [P, P, c_float, c_float, P, POINTER(c_uint32), P, P]
     _L = L
@@ -189,6 +191,15 @@ def render_gbuffer(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRec
     if rc != 0:
         raise ValueError("oracle_render_gbuffer: bad arguments")
     return depth, normal, color, params, motion
+
+
+def present(history_rgba16f: np.ndarray) -> np.ndarray:
+    """present.wgsl into the sRGB display: uint16 [H, W, 4] binary16 history
+    -> uint8 [H, W, 4]."""
+    h = np.ascontiguousarray(history_rgba16f, np.uint16)
+    out = np.zeros(h.shape[:-1] + (4,), np.uint8)
+    lib().oracle_present(_p(h), h.size // 4, _p(out))
+    return out
 
 
 def trace(scene: SceneRecords, geometry: GeometryRecords, origin, direction):

@@ -1401,4 +1401,26 @@ int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, con
   return 0;
 }
 
+void oracle_present(const uint16_t* history, uint32_t n, uint8_t* out) {
+  // decision thresholds of the 8-bit sRGB encode (IEC 61966-2-1): code k+0.5 to linear, rounded up to a float
+  float t[255];
+  for (int k = 0; k < 255; ++k) {
+    const double c = (k + 0.5) / 255.0;
+    const double lin = c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4);
+    float f = (float)lin;
+    if ((double)f < lin) f = std::nextafter(f, INFINITY);
+    t[k] = f;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    for (int ch = 0; ch < 3; ++ch) {
+      float v = f16_to_f32(history[4 * i + ch]);
+      v = std::fmin(std::fmax(v, 0.0f), 1.0f);  // NaN -> 0
+      int code = 0;
+      while (code < 255 && t[code] <= v) ++code;
+      out[4 * i + ch] = (uint8_t)code;
+    }
+    out[4 * i + 3] = 255;
+  }
+}
+
 }  // extern "C"

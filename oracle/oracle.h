@@ -142,6 +142,10 @@ int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, con
  * binary16[2*W*H].  Untextured materials.  Returns 0, or -1. */
 int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
                           float* depth, uint16_t* normal, uint16_t* color, uint16_t* params, uint16_t* motion);
+/* present.wgsl:1-8 into the Rgba8UnormSrgb display: n Rgba16F history
+ * texels to RGBA8 (rgb clamped to [0, 1], sRGB-encoded, nearest code;
+ * alpha 255). */
+void oracle_present(const uint16_t* history, uint32_t n, uint8_t* out);
 /* sample_bsdf (common.wgsl:86-97) for an untextured surface. */
 void oracle_sample_bsdf(const float n[3], const float color[3], float rough, float metal, const float wo[3],
                         uint32_t* rng, float direction[3], float weight[3]);

@@ -1,0 +1,73 @@
+"""Presentation on the GPU (irt_present / irt_present_device) and captures.
+
+The display image must equal the oracle's encode of the read-back history
+bit for bit (the encode is a threshold search, no transcendental function).
+Repeated renders of the same frames must give identical captures, the
+determinism check of tools/verify.py:61-67.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from instantradiositytest_amd import hip, images
+from instantradiositytest_amd.renderer import Renderer
+from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
+
+pytestmark = pytest.mark.gpu
+
+
+def render(cfg, precision, frames, on_device_inputs=False):
+    cs = ConfigScene(cfg)
+    r = Renderer(cs.scene, cfg.settings(), precision=precision, schedule="fused")
+    if on_device_inputs:
+        r.set_geometry(*cs.scene.geometry())
+    for fi in range(frames):
+        inp = cs.frame_inputs(fi)
+        if on_device_inputs:
+            inp.depth = inp.vpls = None
+        r.encode(cs.scene, inp)
+    return r
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_display_matches_oracle_encode(require_gpu, precision):
+    print(f"[present] {precision}", flush=True)
+    cfg = CONFIGS["plain-640x400"].scaled(160, 100, 4)
+    r = render(cfg, precision, 4)
+    try:
+        img = r.present()
+        ref = oracle.present(r.read("history"))
+        assert np.array_equal(img, ref)
+        assert img[..., :3].max() > 0  # lit
+    finally:
+        r.close()
+
+
+def test_repeated_renders_capture_identically(require_gpu, tmp_path):
+    cfg = CONFIGS["plain-640x400"].scaled(160, 100, 6)
+    paths = []
+    for k in range(2):
+        r = render(cfg, "mixed", 6, on_device_inputs=True)
+        try:
+            paths.append(tmp_path / f"repeat_{k}.png")
+            img = r.capture(paths[-1])
+            assert np.array_equal(images.read_png(paths[-1]), img)
+        finally:
+            r.close()
+    a, b = (images.read_png(p) for p in paths)
+    assert images.match(a, b, (0, 0, a.shape[1], a.shape[0]), 0.0)["match"]
+
+
+def test_present_device_equals_present(require_gpu):
+    cfg = CONFIGS["plain-640x400"].scaled(96, 64, 3)
+    r = render(cfg, "mixed", 3)
+    try:
+        buf = hip.DeviceBuffer(96 * 64 * 4)
+        assert r.L.irt_present_device(r.ctx, buf.ptr) == 0
+        r.sync()
+        got = buf.to_array(np.uint8, (64, 96, 4))
+        assert np.array_equal(got, r.present())
+    finally:
+        r.close()
