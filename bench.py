@@ -24,6 +24,8 @@ headline frame takes its G-buffer and VPL pool as staged inputs):
              launch time and the oracle's time for the same frame
   full       the whole reference frame per view on the GPU: G-buffer pass,
              VPL pass, lighting, SVGF, history, presentation (nothing staged)
+  bake       SDF bakes of all the scene's meshes at their resolutions
+             (the exporter's algorithm on the GPU, for scene admission)
   batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
              launch per pass, autotuned tiles, HIP graph replay
   serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
@@ -347,6 +349,52 @@ def run_gbuffer(cs, settings, host_frames, steps: int, threads: int, cpu: bool) 
     return out
 
 
+def run_bake(cs, threads: int, cpu: bool) -> dict:
+    """SDF bakes of every mesh of the scene at its own resolution (irt_bake_sdf:
+    tools/export_scene.py:121-158 plus the Lipschitz bound), wall time of the
+    blocking calls; the oracle's rate on the largest mesh at 24^3."""
+    import oracle
+    from instantradiositytest_amd import _lib
+    from instantradiositytest_amd.types import ptr
+    L = _lib.irt()
+    vertices, indices, _ = cs.scene.geometry()
+    meshes = []
+    for m in cs.scene.meshes:
+        v0, nv, i0, ni = (int(x) for x in m["geometry"])
+        pos = np.ascontiguousarray(vertices["position"][v0:v0 + nv, :3], np.float32)
+        meshes.append((pos, np.ascontiguousarray(indices[i0:i0 + ni], np.uint32), int(m["field"][3])))
+    lip = ctypes.c_float()
+
+    def bake(pos, idx, res):
+        out = np.empty(res ** 3, np.float32)
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        rc = L.irt_bake_sdf(ptr(pos), 3, len(pos), ptr(idx), len(idx), res, ptr(out), ptr(lo), ptr(hi),
+                            ctypes.byref(lip))
+        if rc:
+            raise RuntimeError(L.irt_global_error().decode())
+        return out
+
+    bake(*meshes[0])  # first use: code objects
+    t0 = time.perf_counter()
+    samples = 0
+    for pos, idx, res in meshes:
+        bake(pos, idx, res)
+        samples += res ** 3
+    wall = time.perf_counter() - t0
+    out = {"meshes": len(meshes), "samples": samples, "triangles": int(sum(len(i) // 3 for _, i, _ in meshes)),
+           "wall_ms": round(wall * 1e3, 2), "msamples_s": round(samples / wall / 1e6, 1)}
+    if cpu:
+        pos, idx, _ = max(meshes, key=lambda m: len(m[1]))
+        t0 = time.perf_counter()
+        oracle.bake_sdf(pos, idx, 24, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(24 ** 3 / cpu_s / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+                               "kind": "port", "sample": f"the oracle's bake (every triangle tested) of the largest "
+                                                         f"mesh ({len(idx) // 3} triangles) at 24^3, {cpu_s:.2f} s"}
+    log(f"[bench] bake: {out}")
+    return out
+
+
 def run_full_frame(cs, settings, host_frames, warmup: int, steps: int) -> dict:
     """The whole reference frame on the GPU for one view: G-buffer pass, VPL
     pass, lighting, SVGF, history and presentation into the sRGB display
@@ -451,7 +499,7 @@ def main() -> int:
                     help="resident-session caps of the short serving runs after the main one")
     ap.add_argument("--sweep-seconds", type=float, default=10.0)
     ap.add_argument("--only", default="",
-                    help="comma list of legs: headline,plain,fused_fp32,vpls,gbuffer,full,batched,serving")
+                    help="comma list of legs: headline,plain,fused_fp32,vpls,gbuffer,full,bake,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true")
     args = ap.parse_args()
@@ -460,7 +508,7 @@ def main() -> int:
               f"cross-GPU work; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
     legs = set(args.only.split(",")) if args.only else {"headline", "plain", "fused_fp32", "vpls", "gbuffer", "full",
-                                                        "batched", "serving"}
+                                                        "bake", "batched", "serving"}
 
     from instantradiositytest_amd import hip
     from instantradiositytest_amd.scenes import CONFIGS, ConfigScene
@@ -558,6 +606,8 @@ def main() -> int:
                 line["gbuffer_pass"]["launch_ms"] / head["ms_per_step"], 4)
     if "full" in legs:
         line["full_frame"] = run_full_frame(cs, settings, host_frames, args.warmup, args.steps)
+    if "bake" in legs:
+        line["sdf_bake"] = run_bake(cs, threads, not args.no_cpu_baseline)
 
     if "batched" in legs:
         line["batched"] = run_batched(12, 4, args.batch_steps)

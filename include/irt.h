@@ -548,6 +548,24 @@ int irt_host_pack_frame(const irt_settings* settings, const float vp[16], const 
 /* 4x4 column-major inverse (double internally, rounded to float). Returns 0 if singular. */
 int irt_host_inverse(const float m[16], float out[16]);
 
+/* SDF bake of one mesh on the GPU (tools/export_scene.py:121-158) and the
+ * Lipschitz bound of the result (field_lipschitz_bound, renderer.rs:330-358),
+ * for building per-session scenes at admission.  positions: xyz of vertex v
+ * at positions[v * stride_floats]; indices: triangles (index_count a
+ * multiple of 3).  The grid is the vertex bounds padded by
+ * |hi - lo| / (resolution - 1) * 2 on every axis (returned in bounds_lo /
+ * bounds_hi), sampled on its nodes, x fastest: samples[x + r*(y + r*z)] =
+ * sign * (distance to the nearest triangle) - |extent| * 2e-6, negative
+ * inside by a majority of three ray-parity votes along the exporter's
+ * directions.  resolution in [4, 256]; samples holds resolution^3 floats.
+ * Blocking; runs on the current HIP device, allocating its own scratch.
+ * IRT_ERR_INVALID_ARGUMENT with "SDF classification exceeded 256
+ * intersections" when a parity ray crosses 256 surfaces, as the exporter
+ * refuses such meshes. */
+int irt_bake_sdf(const float* positions, uint32_t stride_floats, uint32_t vertex_count, const uint32_t* indices,
+                 uint32_t index_count, uint32_t resolution, float* samples, float bounds_lo[3], float bounds_hi[3],
+                 float* lipschitz);
+
 /* Decision thresholds of the display's sRGB encode: a channel value v
  * (clamped to [0, 1]) gets the code #{k : out[k] <= v}.  out[k] is the
  * linear value where the exact encode times 255 reaches k + 0.5, rounded up

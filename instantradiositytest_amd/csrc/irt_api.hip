@@ -16,6 +16,7 @@
 #include "irt_vpl.h"
 #include "irt_raster.h"
 #include "irt_present.h"
+#include "irt_bake.h"
 
 #include <hipcub/device/device_scan.hpp>
 
@@ -1602,6 +1603,89 @@ int irt_present(irt_ctx* c, uint8_t* rgba8) {
   if (!rc) rc = sync_all(c);
   if (rc) return rc;
   HIP_CHECK(c, hipMemcpy(rgba8, c->present_buf, c->npx * 4, hipMemcpyDeviceToHost));
+  return IRT_OK;
+}
+
+int irt_bake_sdf(const float* positions, uint32_t stride_floats, uint32_t vertex_count, const uint32_t* indices,
+                 uint32_t index_count, uint32_t resolution, float* samples, float bounds_lo[3], float bounds_hi[3],
+                 float* lipschitz) {
+  auto fail = [](int code, const std::string& m) {
+    irt_host::set_global_error(m);
+    return code;
+  };
+  if (!positions || !indices || !samples || !bounds_lo || !bounds_hi || !lipschitz || stride_floats < 3)
+    return fail(IRT_ERR_INVALID_ARGUMENT, "null argument or position stride below 3");
+  if (resolution < 4 || resolution > 256) return fail(IRT_ERR_INVALID_ARGUMENT, "SDF resolution must be in [4,256]");
+  if (index_count == 0 || vertex_count == 0) return fail(IRT_ERR_INVALID_ARGUMENT, "mesh has no triangles");
+  std::vector<irt_vertex> verts(vertex_count);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < vertex_count; ++v) {
+    std::memset(&verts[v], 0, sizeof(irt_vertex));
+    for (int a = 0; a < 3; ++a) {
+      const float x = positions[(size_t)v * stride_floats + a];
+      verts[v].position[a] = x;
+      lo[a] = std::min(lo[a], x);
+      hi[a] = std::max(hi[a], x);
+    }
+    verts[v].position[3] = 1.0f;
+  }
+  std::vector<irt_tri_node> nodes;
+  std::vector<irt_tri> tris;
+  std::string msg;
+  int rc = irt_host::mesh_hierarchy(verts.data(), vertex_count, indices, index_count, nodes, tris, &msg);
+  if (rc) return fail(rc, msg);
+  // export_scene.py:136-139: padding = |hi - lo| / (res - 1) * 2 on every axis
+  const float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  const double padding = std::sqrt((double)e[0] * e[0] + (double)e[1] * e[1] + (double)e[2] * e[2]) /
+                         (double)(resolution - 1) * 2.0;
+  const float pf = (float)padding;
+  BakeGrid g{};
+  for (int a = 0; a < 3; ++a) {
+    bounds_lo[a] = lo[a] - pf;
+    bounds_hi[a] = hi[a] + pf;
+    g.lo[a] = bounds_lo[a];
+    g.hi[a] = bounds_hi[a];
+  }
+  const float pe[3] = {bounds_hi[0] - bounds_lo[0], bounds_hi[1] - bounds_lo[1], bounds_hi[2] - bounds_lo[2]};
+  g.extent = std::sqrt((double)pe[0] * pe[0] + (double)pe[1] * pe[1] + (double)pe[2] * pe[2]);
+  g.res = resolution;
+  g.n_nodes = (uint32_t)nodes.size();
+  const uint64_t n = (uint64_t)resolution * resolution * resolution;
+  // spacing and threshold of field_lipschitz_bound, in float as the host computes them
+  float sp[3];
+  for (int a = 0; a < 3; ++a) sp[a] = (bounds_hi[a] - bounds_lo[a]) / (float)(resolution - 1);
+  const float sl = std::sqrt(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]);
+  const float threshold = sl * (0.5f + 1.0f / 128.0f);
+  void *d_nodes = nullptr, *d_tris = nullptr, *d_out = nullptr, *d_flags = nullptr;
+  hipError_t e1 = hipMalloc(&d_nodes, sizeof(irt_tri_node) * nodes.size());
+  if (e1 == hipSuccess) e1 = hipMalloc(&d_tris, sizeof(irt_tri) * tris.size());
+  if (e1 == hipSuccess) e1 = hipMalloc(&d_out, sizeof(float) * n);
+  if (e1 == hipSuccess) e1 = hipMalloc(&d_flags, 2 * sizeof(uint32_t));
+  const uint32_t init[2] = {0u, 0x3f800000u};  // overflow flag, bits of the squared maximum (starts at 1)
+  if (e1 == hipSuccess) e1 = hipMemcpy(d_nodes, nodes.data(), sizeof(irt_tri_node) * nodes.size(), hipMemcpyHostToDevice);
+  if (e1 == hipSuccess) e1 = hipMemcpy(d_tris, tris.data(), sizeof(irt_tri) * tris.size(), hipMemcpyHostToDevice);
+  if (e1 == hipSuccess) e1 = hipMemcpy(d_flags, init, sizeof(init), hipMemcpyHostToDevice);
+  if (e1 == hipSuccess) {
+    hipLaunchKernelGGL(k_bake_sdf, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, nullptr,
+                       (const irt_tri_node*)d_nodes, (const irt_tri*)d_tris, g, (float*)d_out, (uint32_t*)d_flags);
+    e1 = hipGetLastError();
+  }
+  const uint64_t cells = (uint64_t)(resolution - 1) * (resolution - 1) * (resolution - 1);
+  if (e1 == hipSuccess) {
+    hipLaunchKernelGGL(k_lipschitz, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, nullptr,
+                       (const float*)d_out, resolution, sp[0], sp[1], sp[2], threshold, (uint32_t*)d_flags + 1);
+    e1 = hipGetLastError();
+  }
+  uint32_t flags[2] = {0, 0};
+  if (e1 == hipSuccess) e1 = hipMemcpy(samples, d_out, sizeof(float) * n, hipMemcpyDeviceToHost);
+  if (e1 == hipSuccess) e1 = hipMemcpy(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost);
+  for (void* p : {d_nodes, d_tris, d_out, d_flags})
+    if (p) (void)hipFree(p);
+  if (e1 != hipSuccess) return fail(IRT_ERR_DEVICE, std::string("SDF bake: ") + hipGetErrorString(e1));
+  if (flags[0]) return fail(IRT_ERR_INVALID_ARGUMENT, "SDF classification exceeded 256 intersections");
+  float max_sq;
+  std::memcpy(&max_sq, &flags[1], 4);
+  *lipschitz = std::sqrt(max_sq) * 1.0001f;
   return IRT_OK;
 }
 

@@ -103,6 +103,8 @@ def lib() -> ctypes.CDLL:
                                POINTER(c_uint32), POINTER(c_uint32)]
     L.oracle_render_gbuffer.restype = c_int
     L.oracle_render_gbuffer.argtypes = [P, P, P, P, P, P, P, P]
+    L.oracle_bake_sdf.restype = c_int
+    L.oracle_bake_sdf.argtypes = [P, c_uint32, c_uint32, P, c_uint32, c_uint32, P, P, P, c_int]
     L.oracle_present.restype = None
     L.oracle_present.argtypes = [P, c_uint32, P]
     L.oracle_sample_bsdf.restype = None
@@ -191,6 +193,21 @@ def render_gbuffer(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRec
     if rc != 0:
         raise ValueError("oracle_render_gbuffer: bad arguments")
     return depth, normal, color, params, motion
+
+
+def bake_sdf(positions: np.ndarray, indices: np.ndarray, res: int, threads: int = 0):
+    """export_scene.py:121-158 for a mesh: (samples float32 [res, res, res] as
+    [z, y, x], bounds lo, hi)."""
+    pos = np.ascontiguousarray(positions, np.float32)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    out = np.zeros((res, res, res), np.float32)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    rc = lib().oracle_bake_sdf(_p(pos), pos.shape[1], len(pos), _p(idx), len(idx), res, _p(out), _p(lo), _p(hi),
+                               threads)
+    if rc != 0:
+        raise ValueError("oracle_bake_sdf: " + ("crossing limit" if rc == -2 else "bad arguments"))
+    return out, lo, hi
 
 
 def present(history_rgba16f: np.ndarray) -> np.ndarray:

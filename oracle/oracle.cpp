@@ -1150,6 +1150,31 @@ bool tri_bounds(const Clip c[3], uint32_t W, uint32_t H, uint32_t* x0, uint32_t*
   return true;
 }
 
+// ------------------------------------------------------------ SDF bake
+// tools/export_scene.py:121-158 with every triangle tested for the nearest
+// point and for each parity cast (closest point: Ericson 2005, 5.1.5).
+Vf closest_on_triangle(Vf p, Vf a, Vf b, Vf c) {
+  const Vf ab = sub3(b, a), ac = sub3(c, a), ap = sub3(p, a);
+  const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return a;
+  const Vf bp = sub3(p, b);
+  const float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return b;
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return add3(a, scale3(ab, d1 / (d1 - d3)));
+  const Vf cp = sub3(p, c);
+  const float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return c;
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return add3(a, scale3(ac, d2 / (d2 - d6)));
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f)
+    return add3(b, scale3(sub3(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+  const float denom = 1.0f / (va + vb + vc);
+  const float v = vb * denom, w = vc * denom;
+  return add3(add3(a, scale3(ab, v)), scale3(ac, w));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1421,6 +1446,90 @@ void oracle_present(const uint16_t* history, uint32_t n, uint8_t* out) {
     }
     out[4 * i + 3] = 255;
   }
+}
+
+int oracle_bake_sdf(const float* positions, uint32_t stride, uint32_t nv, const uint32_t* indices, uint32_t ni,
+                    uint32_t res, float* out, float lo_out[3], float hi_out[3], int threads) {
+  if (!positions || !indices || !out || res < 4 || ni % 3u || stride < 3) return -1;
+  const uint32_t nt = ni / 3u;
+  std::vector<Vf> A(nt), B(nt), C(nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    for (int k = 0; k < 3; ++k)
+      if (indices[3 * t + k] >= nv) return -1;
+    A[t] = F3<float>(positions + (size_t)indices[3 * t] * stride);
+    B[t] = F3<float>(positions + (size_t)indices[3 * t + 1] * stride);
+    C[t] = F3<float>(positions + (size_t)indices[3 * t + 2] * stride);
+  }
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::fmin(lo[a], positions[(size_t)v * stride + a]);
+      hi[a] = std::fmax(hi[a], positions[(size_t)v * stride + a]);
+    }
+  const float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  const double padding = std::sqrt((double)e[0] * e[0] + (double)e[1] * e[1] + (double)e[2] * e[2]) / (double)(res - 1) * 2.0;
+  const float pf = (float)padding;
+  double glo[3], ghi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo_out[a] = lo[a] - pf;
+    hi_out[a] = hi[a] + pf;
+    glo[a] = lo_out[a];
+    ghi[a] = hi_out[a];
+  }
+  const float pe[3] = {hi_out[0] - lo_out[0], hi_out[1] - lo_out[1], hi_out[2] - lo_out[2]};
+  const double extent = std::sqrt((double)pe[0] * pe[0] + (double)pe[1] * pe[1] + (double)pe[2] * pe[2]);
+  const float tmax = (float)(extent * 4.0), step = std::fmax((float)(extent * 1e-6), 1e-7f);
+  const Vf dirs[3] = {normalize3(V<float>(1.0f, 0.371f, 0.129f)), normalize3(V<float>(0.173f, 1.0f, 0.419f)),
+                      normalize3(V<float>(0.319f, 0.217f, 1.0f))};
+  int overflow = 0;
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads) reduction(| : overflow)
+  for (int64_t i = 0; i < (int64_t)res * res * res; ++i) {
+    const uint32_t c[3] = {(uint32_t)(i % res), (uint32_t)((i / res) % res), (uint32_t)(i / ((int64_t)res * res))};
+    float pfv[3];
+    for (int a = 0; a < 3; ++a) pfv[a] = (float)(glo[a] + (ghi[a] - glo[a]) * (double)c[a] / (double)(res - 1));
+    const Vf p = V<float>(pfv[0], pfv[1], pfv[2]);
+    float best = INFINITY;
+    for (uint32_t t = 0; t < nt; ++t) {
+      const Vf d = sub3(p, closest_on_triangle(p, A[t], B[t], C[t]));
+      best = std::fmin(best, dot3(d, d));
+    }
+    const float dist = std::sqrt(best);
+    int votes = 0;
+    for (int k = 0; k < 3; ++k) {
+      Vf start = p;
+      uint32_t hits = 0;
+      while (true) {
+        Hit h{tmax, 0.0f, 0.0f, 0xffffffffu, 0xffffffffu};
+        for (uint32_t t = 0; t < nt; ++t) {
+          const Vf e1 = sub3(B[t], A[t]), e2 = sub3(C[t], A[t]);
+          const Vf pv = cross3(dirs[k], e2);
+          const float det = dot3(e1, pv);
+          if (det == 0.0f) continue;
+          const float inv = 1.0f / det;
+          const Vf sv = sub3(start, A[t]);
+          const float u = dot3(sv, pv) * inv;
+          if (!(u >= 0.0f && u <= 1.0f)) continue;
+          const Vf q = cross3(sv, e1);
+          const float v = dot3(dirs[k], q) * inv;
+          if (!(v >= 0.0f && u + v <= 1.0f)) continue;
+          const float tt = dot3(e2, q) * inv;
+          if (!(tt >= 0.0f) || !closer(tt, 0u, t, h)) continue;
+          h = Hit{tt, u, v, 0u, t};
+        }
+        if (h.object == 0xffffffffu) break;
+        if (++hits >= 256u) {
+          overflow = 1;
+          break;
+        }
+        start = add3(add3(start, scale3(dirs[k], h.t)), scale3(dirs[k], step));
+      }
+      votes += (int)(hits & 1u);
+    }
+    const double sign = votes >= 2 ? -1.0 : 1.0;
+    out[i] = (float)(sign * (double)dist - extent * 2e-6);
+  }
+  return overflow ? -2 : 0;
 }
 
 }  // extern "C"

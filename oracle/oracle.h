@@ -142,6 +142,11 @@ int oracle_trace(const oracle_scene* scene, const oracle_geometry* geometry, con
  * binary16[2*W*H].  Untextured materials.  Returns 0, or -1. */
 int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, const oracle_geometry* geometry,
                           float* depth, uint16_t* normal, uint16_t* color, uint16_t* params, uint16_t* motion);
+/* tools/export_scene.py:121-158: the SDF grid of one mesh (every triangle
+ * tested), res^3 floats x fastest, and its padded bounds.  Returns 0, -1
+ * for bad arguments, -2 when a parity ray crosses 256 surfaces. */
+int oracle_bake_sdf(const float* positions, uint32_t stride, uint32_t nv, const uint32_t* indices, uint32_t ni,
+                    uint32_t res, float* out, float lo[3], float hi[3], int threads);
 /* present.wgsl:1-8 into the Rgba8UnormSrgb display: n Rgba16F history
  * texels to RGBA8 (rgb clamped to [0, 1], sRGB-encoded, nearest code;
  * alpha 255). */
