@@ -1672,7 +1672,8 @@ int irt_bake_sdf(const float* positions, uint32_t stride_floats, uint32_t vertex
   }
   const uint64_t cells = (uint64_t)(resolution - 1) * (resolution - 1) * (resolution - 1);
   if (e1 == hipSuccess) {
-    hipLaunchKernelGGL(k_lipschitz, dim3((uint32_t)((cells + 255) / 256)), dim3(256), 0, nullptr,
+    const uint32_t lip_blocks = (uint32_t)std::min<uint64_t>((cells + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_lipschitz, dim3(lip_blocks), dim3(256), 0, nullptr,
                        (const float*)d_out, resolution, sp[0], sp[1], sp[2], threshold, (uint32_t*)d_flags + 1);
     e1 = hipGetLastError();
   }

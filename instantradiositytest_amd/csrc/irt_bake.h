@@ -164,15 +164,17 @@ __global__ __launch_bounds__(256) void k_bake_sdf(const irt_tri_node* __restrict
 
 // field_lipschitz_bound (renderer.rs:330-358) over the baked samples: the
 // largest corner-gradient norm over cells with a sample above
-// |spacing| * (0.5 + 1/128), at least 1, times 1.0001.  Squared maxima are
-// combined as float bits (non-negative floats order as integers).
+// |spacing| * (0.5 + 1/128), at least 1, times 1.0001.  Grid-stride over the
+// cells; per-lane, wave and workgroup maxima of the squared norms, then one
+// atomicMax per workgroup on the float bits (non-negative floats order as
+// integers).
 __global__ __launch_bounds__(256) void k_lipschitz(const float* __restrict__ s, uint32_t r, float sx, float sy,
                                                    float sz, float threshold, uint32_t* __restrict__ max_bits) {
-  const uint64_t cells = (uint64_t)(r - 1) * (r - 1) * (r - 1);
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ float wave_max[4];
+  const uint64_t n = r, m = r - 1;
+  const uint64_t cells = m * m * m;
   float best = 0.0f;
-  if (i < cells) {
-    const uint64_t n = r, m = r - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t x = i % m, y = (i / m) % m, z = i / (m * m);
     float v[8];
     bool any = false;
@@ -188,9 +190,14 @@ __global__ __launch_bounds__(256) void k_lipschitz(const float* __restrict__ s, 
         best = fmaxf(best, dx * dx + dy * dy + dz * dz);
       }
   }
-  // wave maximum, then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off, 64));
-  if (__lane_id() == 0 && best > 0.0f) atomicMax(max_bits, __float_as_uint(best));
+  if (__lane_id() == 0) wave_max[threadIdx.x / 64] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = wave_max[0];
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) b = fmaxf(b, wave_max[w]);
+    if (b > 0.0f) atomicMax(max_bits, __float_as_uint(b));
+  }
 }
 
 }  // namespace irt
