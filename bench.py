@@ -28,7 +28,8 @@ headline frame takes its G-buffer and VPL pool as staged inputs):
              (the exporter's algorithm on the GPU, for scene admission)
   batched    16 views (4 x 1920x1200, 4 x 1280x720, 8 x 640x360), one
              launch per pass, autotuned tiles, HIP graph replay
-  serving    Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
+  serving    (also with each session's G-buffer and VPL pool produced on the
+             device: serving.device_inputs) Poisson arrivals (40/s) of sessions (640x360 / 1280x720 /
              1920x1200 at 0.4 / 0.4 / 0.2, lifetimes U[120,600] frames),
              up to 32 resident sessions in a 200 GB budget, 60 Hz clients,
              continuous batching: p50 / p99 frame latency and Mpix/s
@@ -628,6 +629,10 @@ def main() -> int:
             sweep[str(n)] = {k: d[k] for k in ("mpix_s", "p50_ms", "p99_ms", "mean_batch", "frames")}
         if sweep:
             line["serving"]["by_sessions"] = sweep
+        # the same workload with each session rendering its own G-buffer and VPL pool on the device
+        dev = run_serving([st.with_device_inputs() for st in streams], rng, args.sweep_seconds, 3.0, args.sessions, 2,
+                          24, shapes)
+        line["serving"]["device_inputs"] = {k: dev[k] for k in ("mpix_s", "p50_ms", "p99_ms", "mean_batch", "frames")}
         del streams
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))

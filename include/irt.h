@@ -466,12 +466,12 @@ typedef struct irt_frame_request {
   uint32_t debug_mode;       /* 0..2 */
   uint32_t vpl_count;        /* must equal the session's settings.vpl_count */
   uint32_t reserved;
-  const float* depth;        /* device pointers, reference formats (irt_set_gbuffer) */
-  const uint16_t* normal;
-  const uint16_t* color;
+  const float* depth;        /* device pointers, reference formats (irt_set_gbuffer); all five */
+  const uint16_t* normal;    /* null: the G-buffer pass renders them (irt_render_gbuffer) from */
+  const uint16_t* color;     /* the session's scene, which then needs irt_set_geometry */
   const uint16_t* params;
   const uint16_t* motion;
-  const irt_vpl* vpls;
+  const irt_vpl* vpls;       /* null: the VPL pass generates the pool (irt_generate_vpls) */
   uint64_t tag;              /* returned in the completion */
 } irt_frame_request;
 typedef struct irt_completion {

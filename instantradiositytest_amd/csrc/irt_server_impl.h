@@ -275,8 +275,12 @@ int irt_server_submit(irt_server* s, uint32_t session, const irt_frame_request* 
     return s->fail(IRT_ERR_INVALID_ARGUMENT, "no such session");
   ServerSession& ss = s->sessions[session];
   irt_ctx* c = ss.ctx;
-  if (!req->depth || !req->normal || !req->color || !req->params || !req->motion || !req->vpls)
-    return s->fail(IRT_ERR_INVALID_ARGUMENT, "null G-buffer plane or VPL pool");
+  // null planes: the G-buffer pass renders them; a null pool: the VPL pass generates it
+  const int planes = !!req->depth + !!req->normal + !!req->color + !!req->params + !!req->motion;
+  if (planes != 0 && planes != 5)
+    return s->fail(IRT_ERR_INVALID_ARGUMENT, "G-buffer planes must be all given or all null");
+  if ((planes == 0 || !req->vpls) && !c->have_geometry)
+    return s->fail(IRT_ERR_STATE, "the session's scene has no triangle geometry to render its inputs from");
   if (req->vpl_count != c->V) return s->fail(IRT_ERR_INVALID_ARGUMENT, "VPL pool size differs from the session's settings");
   if (req->debug_mode >= 3u) return s->fail(IRT_ERR_INVALID_ARGUMENT, "served sessions render debug modes 0..2");
   if (req->light_count > c->n_lights) return s->fail(IRT_ERR_INVALID_ARGUMENT, "light count exceeds the scene's lights");
@@ -331,8 +335,10 @@ int irt_server_step(irt_server* s) {
     const ServerRequest& r = ss.pending.front();
     irt_ctx* c = ss.ctx;
     rc = irt_set_frame(c, &r.frame);
-    if (!rc) rc = irt_set_gbuffer_device(c, r.in.depth, r.in.normal, r.in.color, r.in.params, r.in.motion);
-    if (!rc) rc = irt_set_vpls_device(c, r.in.vpls, r.in.vpl_count);
+    if (!rc)
+      rc = r.in.depth ? irt_set_gbuffer_device(c, r.in.depth, r.in.normal, r.in.color, r.in.params, r.in.motion)
+                      : irt_render_gbuffer(c);
+    if (!rc) rc = r.in.vpls ? irt_set_vpls_device(c, r.in.vpls, r.in.vpl_count) : irt_generate_vpls(c);
     if (rc) return s->fail(rc, "session " + std::to_string(ids[k]) + ": " + c->err);
     irt_completion cm{};
     cm.tag = r.in.tag;

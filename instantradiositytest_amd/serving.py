@@ -24,6 +24,7 @@ Workload pieces:
 """
 from __future__ import annotations
 
+import copy
 import ctypes
 import time
 from collections import deque
@@ -70,25 +71,36 @@ def batched_view_configs(frames: int, seed: int = 0xC4) -> list:
 
 @dataclass
 class StagedFrame:
-    bufs: tuple  # DeviceBuffers: depth, normal, color, params, motion, vpls
+    bufs: tuple | None  # DeviceBuffers: depth, normal, color, params, motion, vpls; None: produced on the device
     vp: np.ndarray
     camera: np.ndarray
 
 
 class SceneStream:
-    """A scene on the device (template context) plus staged frame inputs."""
+    """A scene on the device (template context) plus staged frame inputs.
 
-    def __init__(self, cfg: SceneConfig, precision: str, frames: int, log=None):
+    With ``device_inputs`` the scene carries its triangle geometry and no
+    inputs are staged: each frame's G-buffer and VPL pool are produced on
+    the device by the server (null request pointers)."""
+
+    def __init__(self, cfg: SceneConfig, precision: str, frames: int, log=None, device_inputs: bool = False):
         from . import hip
         self.cfg = cfg
         self.cs = ConfigScene(cfg)
         self.settings = cfg.settings()
         self.pixels = cfg.width * cfg.height
+        self.device_inputs = device_inputs
         # the template only holds the scene: an 8x8 view keeps its state small
         self.template = Renderer(self.cs.scene, cfg.settings(render_width=8, render_height=8), precision=precision,
                                  schedule="fused")
+        self.template.set_geometry(*self.cs.scene.geometry())
         self.frames = []
         for fi in range(frames):
+            if device_inputs:
+                vp = self.cs.scene.camera_matrix(self.cs.camera(fi), cfg.width / cfg.height)
+                cam = np.asarray(self.cs.camera(fi)["position"], np.float32)
+                self.frames.append(StagedFrame(None, np.asarray(vp, np.float32), cam))
+                continue
             inp = self.cs.frame_inputs(fi)
             bufs = tuple(hip.DeviceBuffer.from_array(a) for a in
                          (inp.depth, inp.normal, inp.color, inp.params, inp.motion, inp.vpls))
@@ -100,6 +112,14 @@ class SceneStream:
     @property
     def scene_ctx(self):
         return self.template.ctx
+
+    def with_device_inputs(self) -> "SceneStream":
+        """The same scene and camera path with every frame's inputs produced on
+        the device (the requests carry no input pointers)."""
+        v = copy.copy(self)
+        v.frames = [StagedFrame(None, f.vp, f.camera) for f in self.frames]
+        v.device_inputs = True
+        return v
 
 
 # ------------------------------------------------------------------ server
@@ -158,7 +178,10 @@ class Server:
         r.debug_mode = debug_mode
         r.vpl_count = vpl_count
         b = frame.bufs
-        r.depth, r.normal, r.color, r.params, r.motion, r.vpls = (x.ptr.value for x in b)
+        if b is None:  # the server renders the G-buffer and generates the VPL pool
+            r.depth = r.normal = r.color = r.params = r.motion = r.vpls = None
+        else:
+            r.depth, r.normal, r.color, r.params, r.motion, r.vpls = (x.ptr.value for x in b)
         r.tag = tag
         self._check(self.L.irt_server_submit(self.h, sid, ctypes.byref(r)))
 

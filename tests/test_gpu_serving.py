@@ -8,6 +8,8 @@
 * Admission: the slot count and the HBM budget are enforced; a closed
   session's context is reused.
 * A short Poisson-arrival run completes frames with finite latencies.
+* Sessions whose requests carry no inputs get them from the G-buffer and
+  VPL passes: bit-identical to per-view frames with the same passes.
 """
 from __future__ import annotations
 
@@ -165,3 +167,29 @@ def test_short_poisson_run(require_gpu):
     assert rep.sessions_admitted == 6 and rep.sessions_finished == 6
     assert rep.frames == sum(a.lifetime for a in arrivals)
     assert np.isfinite(rep.p99_ms) and rep.p99_ms > 0
+
+
+def test_sessions_render_their_own_inputs(require_gpu):
+    print("[serving] device-produced inputs", flush=True)
+    A = SceneStream(CONFIGS["plain-640x400"].scaled(96, 64, FRAMES), "mixed", FRAMES, device_inputs=True)
+    B = SceneStream(replace(CONFIGS["plain-640x400"], seed=0x53).scaled(80, 48, FRAMES), "mixed", FRAMES,
+                    device_inputs=True)
+    srv = Server(max_sessions=2, precision="mixed", budget_bytes=1e12, max_in_flight=2)
+    a = srv.open(A.settings, A.scene_ctx)
+    b = srv.open(B.settings, B.scene_ctx)
+    for k in range(FRAMES):
+        submit(srv, a, A, k, k)
+        submit(srv, b, B, k, 100 + k)
+        srv.step()
+    flush(srv)
+    for st, sid in ((A, a), (B, b)):
+        r = Renderer(st.cs.scene, st.settings, precision="mixed", schedule="fused")
+        r.set_geometry(*st.cs.scene.geometry())
+        for k in range(FRAMES):
+            inp = st.cs.frame_inputs(k)
+            inp.depth = inp.vpls = None
+            r.encode(st.cs.scene, inp)
+        r.sync()
+        same({k: r.read(k) for k in BUFS}, {k: srv.read(sid, k) for k in BUFS}, "served with device inputs")
+        r.close()
+    srv.close_server()
