@@ -290,12 +290,21 @@ typedef struct irt_material {
  * and its mesh-local indices through irt_mesh.geometry (vertex offset and
  * count, index offset and count); objects name their material in info[1].
  * Builds the per-mesh hierarchies (the BLAS build of renderer.rs:1061-1106)
- * and uploads everything; blocking.  Only untextured materials are
- * accepted (textures[] all -1): the material textures and their sampler
- * (renderer.rs:951-1030) are not part of this library.  Call after
- * irt_set_meshes; a later irt_set_meshes drops the geometry. */
+ * and uploads everything; blocking.  Textured materials name layers of the
+ * texture array of irt_set_textures.  Call after irt_set_meshes; a later
+ * irt_set_meshes drops the geometry. */
 int irt_set_geometry(irt_ctx* ctx, const irt_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
                      uint32_t index_count, const irt_material* materials, uint32_t material_count);
+
+/* Material textures (group 0 bindings 9, 10, 13 of renderer.rs:1115-1204,
+ * uploaded at renderer.rs:951-1046): `layers` square RGBA8 layers of
+ * size x size texels, layer-major, rows top to bottom, already resized as
+ * Renderer::new resizes them (renderer.rs:230-262).  Sampled as the
+ * reference's views and sampler: the linear view for roughness, metalness
+ * and normal maps and for base colours without the sRGB flag (params.w),
+ * the sRGB view for the others; repeat addressing, bilinear filtering,
+ * level 0 (material_at, common.wgsl:50-66).  Blocking. */
+int irt_set_textures(irt_ctx* ctx, const uint8_t* rgba8, uint32_t size, uint32_t layers);
 
 /* The VPL pass (vpl.wgsl:2-47, dispatched at renderer.rs:1863-1872):
  * frame.counts.y light paths of up to frame.counts.z surface hits each,
@@ -313,8 +322,8 @@ int irt_generate_vpls(irt_ctx* ctx);
  * (model and previous matrices of irt_set_objects, triangles of
  * irt_set_geometry, frame.vp and frame.previous_vp) into the current
  * G-buffer, in place of irt_set_gbuffer.  Values are rounded as the
- * reference's Depth32Float / Rgba16F / Rg16F targets hold them.  Untextured
- * materials.  Enqueued on irt_stream; call after irt_set_frame. */
+ * reference's Depth32Float / Rgba16F / Rg16F targets hold them.  Enqueued
+ * on irt_stream; call after irt_set_frame. */
 int irt_render_gbuffer(irt_ctx* ctx);
 
 /* Presentation (present.wgsl:1-8 into the Rgba8UnormSrgb capture target of
@@ -382,8 +391,8 @@ int irt_enable_counters(irt_ctx* ctx, int enable);
 int irt_read_counters(irt_ctx* ctx, uint64_t counts[IRT_COUNTER_COUNT]);
 
 /* Device-to-device copy of src's scene (SDF atlas, meshes, objects,
- * hierarchy, lights, candidate grid, and the triangle geometry of
- * irt_set_geometry when src has it) into dst; both contexts must use the
+ * hierarchy, lights, candidate grid, and the triangle geometry and
+ * textures of irt_set_geometry / irt_set_textures when src has them) into dst; both contexts must use the
  * same precision profile.  Synchronises both contexts' streams.  The
  * reference uploads one scene per Renderer (renderer.rs:713-817); this lets
  * a server give each session its own copy of a prepared scene without a

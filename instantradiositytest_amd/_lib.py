@@ -59,6 +59,7 @@ def irt() -> ctypes.CDLL:
         "irt_set_vpls_device": (c_int, [P, P, u32]),
         "irt_set_geometry": (c_int, [P, P, u32, P, u32, P, u32]),
         "irt_generate_vpls": (c_int, [P]),
+        "irt_set_textures": (c_int, [P, P, u32, u32]),
         "irt_render_gbuffer": (c_int, [P]),
         "irt_present": (c_int, [P, P]),
         "irt_present_device": (c_int, [P, P]),

@@ -110,6 +110,13 @@ struct irt_ctx {
   uint2* mesh_nodes = nullptr;  // per mesh: [first, end) of its triangle hierarchy
   uint32_t cap_mesh_nodes = 0;
   bool have_geometry = false;
+  int32_t max_texture_layer = -1;  // largest texture layer a material names (-1: none)
+  // material textures (irt_set_textures)
+  uint32_t* tex = nullptr;
+  size_t cap_tex = 0;
+  uint32_t tex_size = 0, tex_layers = 0;
+  float* srgb_lut = nullptr;
+  uint32_t cap_srgb_lut = 0;
   // G-buffer pass (irt_render_gbuffer)
   std::vector<uint32_t> host_obj_mesh;   // mesh index of each object
   std::vector<uint32_t> host_tri_base;   // instance-triangle prefix sums the device holds
@@ -1237,7 +1244,26 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
     dst->n_materials = src->n_materials;
     dst->n_tri_nodes = src->n_tri_nodes;
     dst->n_tris = src->n_tris;
+    dst->max_texture_layer = src->max_texture_layer;
     dst->have_geometry = true;
+  }
+  if (src->tex_layers) {
+    const size_t texels = (size_t)src->tex_size * src->tex_size * src->tex_layers;
+    if (dst->cap_tex < texels) {
+      rc = dev_free(dst, dst->tex);
+      dst->tex = nullptr;
+      dst->cap_tex = 0;
+      if (!rc) rc = dev_alloc(dst, (void**)&dst->tex, texels * 4);
+      if (rc) return rc;
+      dst->cap_tex = texels;
+    }
+    rc = ensure_array(dst, &dst->srgb_lut, &dst->cap_srgb_lut, 256);
+    if (rc) return rc;
+    HIP_CHECK(dst, hipMemcpyAsync(dst->tex, src->tex, texels * 4, hipMemcpyDeviceToDevice, dst->stream));
+    HIP_CHECK(dst, hipMemcpyAsync(dst->srgb_lut, src->srgb_lut, 256 * sizeof(float), hipMemcpyDeviceToDevice,
+                                  dst->stream));
+    dst->tex_size = src->tex_size;
+    dst->tex_layers = src->tex_layers;
   }
   dst->grid = src->grid;
   dst->grid.cells = nullptr;
@@ -1404,10 +1430,13 @@ int irt_set_geometry(irt_ctx* c, const irt_vertex* vertices, uint32_t vertex_cou
   if ((vertex_count && !vertices) || (index_count && !indices) || !materials || material_count == 0)
     return c->fail(IRT_ERR_INVALID_ARGUMENT, "null geometry array or no materials");
   if (c->host_meshes.empty()) return c->fail(IRT_ERR_STATE, "set the meshes before their geometry");
-  for (uint32_t i = 0; i < material_count; ++i)
-    for (int k = 0; k < 4; ++k)
-      if (materials[i].textures[k] >= 0)
-        return c->fail(IRT_ERR_INVALID_ARGUMENT, "textured materials are not supported (textures must be -1)");
+  int32_t max_layer = -1;
+  for (uint32_t i = 0; i < material_count; ++i) {
+    for (int k = 0; k < 4; ++k) max_layer = std::max(max_layer, materials[i].textures[k]);
+    if ((materials[i].textures[1] >= 0 && materials[i].channels[1] > 3) ||
+        (materials[i].textures[2] >= 0 && materials[i].channels[2] > 3))
+      return c->fail(IRT_ERR_INVALID_ARGUMENT, "material texture channel must be 0..3");
+  }
   std::vector<irt_tri_node> all_nodes, nodes;
   std::vector<irt_tri> all_tris, tris;
   std::vector<uint2> ranges(c->n_meshes);
@@ -1454,7 +1483,49 @@ int irt_set_geometry(irt_ctx* c, const irt_vertex* vertices, uint32_t vertex_cou
   c->n_materials = material_count;
   c->n_tri_nodes = (uint32_t)all_nodes.size();
   c->n_tris = (uint32_t)all_tris.size();
+  c->max_texture_layer = max_layer;
   c->have_geometry = true;
+  return IRT_OK;
+}
+
+static TextureView texture_view(const irt_ctx* c) {
+  return TextureView{c->tex, c->srgb_lut, c->tex_size, c->tex_layers};
+}
+
+static int check_textures(irt_ctx* c) {
+  if (c->max_texture_layer >= 0 && (uint32_t)c->max_texture_layer >= c->tex_layers)
+    return c->fail(IRT_ERR_STATE, "a material names texture layer " + std::to_string(c->max_texture_layer) +
+                                      " but " + std::to_string(c->tex_layers) + " layers are set (irt_set_textures)");
+  return IRT_OK;
+}
+
+int irt_set_textures(irt_ctx* c, const uint8_t* rgba8, uint32_t size, uint32_t layers) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!rgba8 || size == 0 || layers == 0) return c->fail(IRT_ERR_INVALID_ARGUMENT, "no texture data");
+  if (size > 8192 || (uint64_t)size * size * layers > (1ull << 31))
+    return c->fail(IRT_ERR_LIMIT, "texture array too large");
+  const size_t texels = (size_t)size * size * layers;
+  if (c->cap_tex < texels) {
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    int rc = dev_free(c, c->tex);
+    c->tex = nullptr;
+    c->cap_tex = 0;
+    if (!rc) rc = dev_alloc(c, (void**)&c->tex, texels * 4);
+    if (rc) return rc;
+    c->cap_tex = texels;
+  }
+  int rc = ensure_array(c, &c->srgb_lut, &c->cap_srgb_lut, 256);
+  if (rc) return rc;
+  float lut[256];
+  for (int k = 0; k < 256; ++k) {
+    const double v = k / 255.0;
+    lut[k] = (float)(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+  }
+  HIP_CHECK(c, hipMemcpyAsync(c->tex, rgba8, texels * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipMemcpyAsync(c->srgb_lut, lut, sizeof(lut), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->tex_size = size;
+  c->tex_layers = layers;
   return IRT_OK;
 }
 
@@ -1469,9 +1540,11 @@ int irt_generate_vpls(irt_ctx* c) {
   if (F.counts[2] < 1 || F.counts[2] > 32) return c->fail(IRT_ERR_INVALID_ARGUMENT, "invalid VPL bounce count");
   if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "object references a missing mesh");
   if (c->max_material_ref >= c->n_materials) return c->fail(IRT_ERR_STATE, "object references a missing material");
+  int trc = check_textures(c);
+  if (trc) return trc;
   const uint32_t slot = F.size[2] % c->slots;
   TriScene sc{c->objects, c->meshes,  c->mesh_nodes, c->tri_nodes, c->tris,
-              c->vertices, c->indices, c->materials,  c->lights};
+              c->vertices, c->indices, c->materials,  c->lights,   texture_view(c)};
   // lanes per path: enough for one object per lane up to a wave
   uint32_t group = 8;
   while (group < 64 && group < c->n_objects) group <<= 1;
@@ -1507,6 +1580,8 @@ int irt_render_gbuffer(irt_ctx* c) {
   if (!c->n_objects) return c->fail(IRT_ERR_STATE, "no objects");
   if (c->max_mesh_ref >= c->n_meshes) return c->fail(IRT_ERR_STATE, "object references a missing mesh");
   if (c->max_material_ref >= c->n_materials) return c->fail(IRT_ERR_STATE, "object references a missing material");
+  int trc = check_textures(c);
+  if (trc) return trc;
   // instance triangles: objects in order, each its mesh's triangles in index order (the draw order)
   std::vector<uint32_t> base(c->n_objects + 1, 0);
   for (uint32_t o = 0; o < c->n_objects; ++o) {
@@ -1540,7 +1615,8 @@ int irt_render_gbuffer(irt_ctx* c) {
   if (rc) return rc;
   c->guide_dirty = true;
   const uint32_t W = c->W, H = c->H, npx = (uint32_t)c->npx;
-  GbScene sc{c->objects, c->meshes, c->vertices, c->indices, c->materials, c->obj_tri_base, c->n_objects};
+  GbScene sc{c->objects,      c->meshes,    c->vertices,       c->indices,
+             c->materials,    c->obj_tri_base, c->n_objects,   texture_view(c)};
   timer_begin(c, "gbuffer");
   hipLaunchKernelGGL(k_gb_objects, dim3((c->n_objects + 63) / 64), dim3(64), 0, c->stream, c->objects, c->n_objects,
                      c->frame, c->gb_objs);
@@ -1559,11 +1635,11 @@ int irt_render_gbuffer(irt_ctx* c) {
   const dim3 tiles((W + GB_TILE - 1) / GB_TILE, (H + GB_TILE - 1) / GB_TILE);
   if (mixed(c))
     hipLaunchKernelGGL(k_gb_resolve<ProfileMixed>, tiles, dim3(256), 0, c->stream, c->gb_tris, c->gb_attrs,
-                       c->materials, c->gb_vis, W, H, c->depth, (h4*)c->normal, (h4*)c->color, (h2*)c->params,
+                       c->materials, texture_view(c), c->gb_vis, W, H, c->depth, (h4*)c->normal, (h4*)c->color, (h2*)c->params,
                        (h2*)c->motion);
   else
     hipLaunchKernelGGL(k_gb_resolve<ProfileF32>, tiles, dim3(256), 0, c->stream, c->gb_tris, c->gb_attrs,
-                       c->materials, c->gb_vis, W, H, c->depth, (float4*)c->normal, (float4*)c->color,
+                       c->materials, texture_view(c), c->gb_vis, W, H, c->depth, (float4*)c->normal, (float4*)c->color,
                        (float2*)c->params, (float2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);

@@ -69,7 +69,10 @@ __device__ __forceinline__ uint32_t f2u_sat(float x) {
   return (uint32_t)x;
 }
 // Rgba16F storage rounding (round to nearest even), as a texture write does.
-__device__ __forceinline__ float round_f16(float x) { return (float)(_Float16)x; }
+// The sign is taken back from x: the compiler may fold a product that feeds
+// the conversion into a mixed-precision FMA with a +0 addend
+// (v_fma_mixlo_f16), which turns a -0 product into +0.
+__device__ __forceinline__ float round_f16(float x) { return __builtin_copysignf((float)(_Float16)x, x); }
 __device__ __forceinline__ float4 round_f16(float4 v) {
   return make_float4(round_f16(v.x), round_f16(v.y), round_f16(v.z), round_f16(v.w));
 }

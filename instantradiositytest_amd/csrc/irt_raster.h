@@ -33,7 +33,7 @@
 // it leaves the visibility buffer cleared for the next frame.
 #pragma once
 
-#include "irt_device.h"
+#include "irt_material.h"
 
 namespace irt {
 
@@ -62,6 +62,8 @@ struct __align__(16) GbAttr {
   float cur[9];
   float prev[9];
   uint32_t material;
+  float uv[6];       // corner uv
+  float tangent[12]; // corner tangents: normalize(model * (t.xyz, 0)), t.w
 };
 
 struct GbScene {
@@ -72,6 +74,7 @@ struct GbScene {
   const irt_material* materials;
   const uint32_t* obj_tri_base;  // n_objects + 1 prefix sums of instance triangles
   uint32_t n_objects;
+  TextureView tex;
 };
 
 // (vp * m) with each entry a left-to-right sum over k of vp[k][r] * m[c][k].
@@ -198,6 +201,11 @@ __global__ __launch_bounds__(256) void k_gb_setup(GbScene sc, const GbObject* __
     a.n[3 * k] = nk.x; a.n[3 * k + 1] = nk.y; a.n[3 * k + 2] = nk.z;
     a.cur[3 * k] = c[k].x; a.cur[3 * k + 1] = c[k].y; a.cur[3 * k + 2] = c[k].w;
     a.prev[3 * k] = pk.x; a.prev[3 * k + 1] = pk.y; a.prev[3 * k + 2] = pk.w;
+    a.uv[2 * k] = vx.uv[0]; a.uv[2 * k + 1] = vx.uv[1];
+    const float4 wt = mat_mul(ob.model, vx.tangent[0], vx.tangent[1], vx.tangent[2], 0.0f);
+    const v3 tk = normalize(v3{wt.x, wt.y, wt.z});
+    a.tangent[4 * k] = tk.x; a.tangent[4 * k + 1] = tk.y; a.tangent[4 * k + 2] = tk.z;
+    a.tangent[4 * k + 3] = vx.tangent[3];
   }
   a.material = ob.info[1];
   attrs[t] = a;
@@ -252,7 +260,7 @@ __device__ __forceinline__ void st_rg16(h2* p, uint32_t i, float a, float b) { p
 // Rgba16F, Rg16F) and stored in the profile's G-buffer formats.
 template <class P>
 __global__ __launch_bounds__(256) void k_gb_resolve(const GbTri* __restrict__ tris, const GbAttr* __restrict__ attrs,
-                                                    const irt_material* __restrict__ materials,
+                                                    const irt_material* __restrict__ materials, TextureView tex,
                                                     unsigned long long* __restrict__ vis, uint32_t W,
                                                     uint32_t H, float* __restrict__ depth,
                                                     typename P::N4* __restrict__ normal,
@@ -288,14 +296,23 @@ __global__ __launch_bounds__(256) void k_gb_resolve(const GbTri* __restrict__ tr
   const float qx = a.prev[0] * l0 + a.prev[3] * l1 + a.prev[6] * l2;
   const float qy = a.prev[1] * l0 + a.prev[4] * l1 + a.prev[7] * l2;
   const float qw = a.prev[2] * l0 + a.prev[5] * l1 + a.prev[8] * l2;
-  // material_at (common.wgsl:50-66), untextured
   const irt_material& m = materials[a.material];
-  const v3 sn = normalize(ni);
-  const float rough = clampf(m.params[0], 0.045f, 1.0f), metal = clampf(m.params[1], 0.0f, 1.0f);
+  float tu = 0.0f, tv = 0.0f;
+  float4 tangent = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (textured(m)) {
+    tu = a.uv[0] * l0 + a.uv[2] * l1 + a.uv[4] * l2;
+    tv = a.uv[1] * l0 + a.uv[3] * l1 + a.uv[5] * l2;
+    tangent = make_float4(a.tangent[0] * l0 + a.tangent[4] * l1 + a.tangent[8] * l2,
+                          a.tangent[1] * l0 + a.tangent[5] * l1 + a.tangent[9] * l2,
+                          a.tangent[2] * l0 + a.tangent[6] * l1 + a.tangent[10] * l2,
+                          a.tangent[3] * l0 + a.tangent[7] * l1 + a.tangent[11] * l2);
+  }
+  Surface s;
+  material_at(m, tex, tu, tv, ni, tangent, s);
   depth[id] = __uint_as_float((uint32_t)(key >> 32));
-  st_r16(normal, id, make_float4(sn.x, sn.y, sn.z, 1.0f));
-  st_r16(color, id, make_float4(m.color[0], m.color[1], m.color[2], 1.0f));
-  st_rg16(params, id, rough, metal);
+  st_r16(normal, id, make_float4(s.n.x, s.n.y, s.n.z, 1.0f));
+  st_r16(color, id, make_float4(s.color.x, s.color.y, s.color.z, 1.0f));
+  st_rg16(params, id, s.rough, s.metal);
   st_rg16(motion, id, (cx / cw - qx / qw) * 0.5f, (cy / cw - qy / qw) * -0.5f);
 }
 

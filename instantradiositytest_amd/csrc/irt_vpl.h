@@ -20,7 +20,7 @@
 // rays' object tests run side by side across the group.
 #pragma once
 
-#include "irt_device.h"
+#include "irt_material.h"
 
 namespace irt {
 
@@ -34,6 +34,7 @@ struct TriScene {
   const uint32_t* indices;
   const irt_material* materials;
   const irt_light* lights;
+  TextureView tex;
 };
 
 struct RayHit {
@@ -136,6 +137,28 @@ __device__ __forceinline__ RayHit trace_group(const TriScene& sc, const float4* 
   return h;
 }
 
+// sin and cos of the pass's angles (|x| <= 2 pi: azimuths 2 pi u and half
+// cone angles), evaluated with the same float operations as the oracle so
+// the pool is reproducible bit for bit: reduction by quarter turns in two
+// constants (after Cody & Waite 1980: pi/2 as its float plus the float of the
+// remainder), then Taylor polynomials in r^2 to degree 11 / 12, whose
+// truncation error on |r| <= pi/4 is below 2e-9.
+__device__ __forceinline__ void vpl_sincos(float x, float* sn, float* cs) {
+  const float k = rintf(x * 0.63661977236758134f);  // x / (pi/2), to the nearest quarter turn
+  const float r = (x - k * 1.57079637050628662f) - k * -4.37113900018624283e-8f;
+  const float r2 = r * r;
+  const float s = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f + r2 * (-1.0f / 39916800.0f)))));
+  const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f + r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
+  const int q = ((int)k) & 3;
+  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+__device__ __forceinline__ float vpl_cos(float x) {
+  float s, c;
+  vpl_sincos(x, &s, &c);
+  return c;
+}
+
 // basis — common.wgsl:84: mat3x3(t, cross(n, t), n) * v.
 __device__ __forceinline__ v3 basis_mul(v3 n, v3 v) {
   const v3 axis = fabsf(n.z) > 0.9f ? v3{1.0f, 0.0f, 0.0f} : v3{0.0f, 0.0f, 1.0f};
@@ -149,7 +172,7 @@ __device__ __forceinline__ v3 cosine_direction(v3 n, uint32_t& rng) {
   const float u = rnd(rng);
   const float phi = 2.0f * IRT_PI * rnd(rng);
   float sn, cs;
-  sincosf(phi, &sn, &cs);
+  vpl_sincos(phi, &sn, &cs);
   return basis_mul(n, v3{sqrtf(u) * cs, sqrtf(u) * sn, sqrtf(1.0f - u)});
 }
 
@@ -169,7 +192,7 @@ __device__ __forceinline__ Scatter sample_bsdf(const Surface& s, uint32_t& rng) 
     const float cos_theta = sqrtf((1.0f - u) / (1.0f + (a2 - 1.0f) * u));
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    sincosf(phi, &sn, &cs);
+    vpl_sincos(phi, &sn, &cs);
     const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
     const v3 h = basis_mul(s.n, v3{sin_theta * cs, sin_theta * sn, cos_theta});
     const v3 e = -wo;  // reflect(e, h) = e - 2 dot(h, e) h
@@ -217,20 +240,20 @@ __global__ __launch_bounds__(256) void k_generate_vpls(TriScene sc, irt_frame F,
   v3 direction;
   float omega = 4.0f * IRT_PI;
   if (l.color_type[3] > 0.5f) {
-    const float ch = cosf(l.direction_angle[3] * 0.5f);
+    const float ch = vpl_cos(l.direction_angle[3] * 0.5f);
     omega = 2.0f * IRT_PI * (1.0f - ch);
     const float r = rnd(rng);
     const float z = ch * (1.0f - r) + 1.0f * r;  // mix(cos(angle/2), 1, r)
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    sincosf(phi, &sn, &cs);
+    vpl_sincos(phi, &sn, &cs);
     const float sz = sqrtf(1.0f - z * z);
     direction = basis_mul(normalize(ld3(l.direction_angle)), v3{sz * cs, sz * sn, z});
   } else {
     const float z = 2.0f * rnd(rng) - 1.0f;
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    sincosf(phi, &sn, &cs);
+    vpl_sincos(phi, &sn, &cs);
     const float sz = sqrtf(1.0f - z * z);
     direction = v3{sz * cs, sz * sn, z};
   }
@@ -259,13 +282,21 @@ __global__ __launch_bounds__(256) void k_generate_vpls(TriScene sc, irt_frame F,
                      a.normal[2] * bx + b.normal[2] * by + c.normal[2] * bz};
     const float4 wn = mat_mul(ob.model, ln.x, ln.y, ln.z, 0.0f);
     const v3 n = normalize(v3{wn.x, wn.y, wn.z});
-    // material_at (common.wgsl:50-66), untextured
     const irt_material& m = sc.materials[ob.info[1]];
     Surface s;
-    s.color = v3{m.color[0], m.color[1], m.color[2]};
-    s.n = normalize(n);
-    s.rough = clampf(m.params[0], 0.045f, 1.0f);
-    s.metal = clampf(m.params[1], 0.0f, 1.0f);
+    float tu = 0.0f, tv = 0.0f;
+    float4 tangent = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (textured(m)) {  // uv and tangent only feed the textures (vpl.wgsl:27-29)
+      tu = a.uv[0] * bx + b.uv[0] * by + c.uv[0] * bz;
+      tv = a.uv[1] * bx + b.uv[1] * by + c.uv[1] * bz;
+      const float4 t4 = make_float4(a.tangent[0] * bx + b.tangent[0] * by + c.tangent[0] * bz,
+                                    a.tangent[1] * bx + b.tangent[1] * by + c.tangent[1] * bz,
+                                    a.tangent[2] * bx + b.tangent[2] * by + c.tangent[2] * bz,
+                                    a.tangent[3] * bx + b.tangent[3] * by + c.tangent[3] * bz);
+      const float4 wt = mat_mul(ob.model, t4.x, t4.y, t4.z, 0.0f);
+      tangent = make_float4(wt.x, wt.y, wt.z, t4.w >= 0.0f ? 1.0f : -1.0f);
+    }
+    material_at(m, sc.tex, tu, tv, n, tangent, s);
     s.p = origin + direction * hit.t;
     const v3 wo = -direction;
     if (dot(s.n, wo) <= 0.0f) {

@@ -140,6 +140,13 @@ class Renderer:
         """The G-buffer pass of the frame set last (irt_render_gbuffer, gbuffer.wgsl)."""
         check(self.L.irt_render_gbuffer(self.ctx), self.ctx)
 
+    def set_textures(self, textures: np.ndarray):
+        """Material texture array (irt_set_textures): uint8 [layers, size, size, 4] RGBA8."""
+        t = np.ascontiguousarray(textures, np.uint8)
+        if t.ndim != 4 or t.shape[1] != t.shape[2] or t.shape[3] != 4:
+            raise ValueError("textures must be [layers, size, size, 4]")
+        check(self.L.irt_set_textures(self.ctx, ptr(t), t.shape[1], t.shape[0]), self.ctx)
+
     def generate_vpls(self):
         """The VPL pass of the frame set last (irt_generate_vpls, vpl.wgsl:2-47)."""
         check(self.L.irt_generate_vpls(self.ctx), self.ctx)

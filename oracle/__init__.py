@@ -145,7 +145,8 @@ class SceneRecords:
 
 class Geometry(ctypes.Structure):
     _fields_ = [("vertices", c_void_p), ("indices", c_void_p), ("materials", c_void_p), ("n_vertices", c_uint32),
-                ("n_indices", c_uint32), ("n_materials", c_uint32)]
+                ("n_indices", c_uint32), ("n_materials", c_uint32), ("textures", c_void_p),
+                ("texture_size", c_uint32), ("texture_layers", c_uint32)]
 
 
 class GeometryRecords:
@@ -153,13 +154,16 @@ class GeometryRecords:
     (64 B: position, normal, tangent, uv), mesh-local uint32 indices and
     material records (64 B)."""
 
-    def __init__(self, vertices, indices, materials):
+    def __init__(self, vertices, indices, materials, textures: np.ndarray | None = None):
         self.vertices = np.ascontiguousarray(vertices)
         self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
         self.materials = np.ascontiguousarray(materials)
         assert self.vertices.itemsize == 64 and self.materials.itemsize == 64
+        # textures: uint8 [layers, size, size, 4]
+        self.textures = None if textures is None else np.ascontiguousarray(textures, np.uint8)
+        layers, size = (0, 0) if self.textures is None else self.textures.shape[:2]
         self.c = Geometry(_p(self.vertices), _p(self.indices), _p(self.materials), len(self.vertices),
-                          len(self.indices), len(self.materials))
+                          len(self.indices), len(self.materials), _p(self.textures), size, layers)
 
 
 def generate_vpls(frame: np.ndarray, scene: SceneRecords, geometry: GeometryRecords, threads: int = 0,

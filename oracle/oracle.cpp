@@ -895,6 +895,76 @@ using Vf = Vec<float>;
 
 Vf cross3(Vf a, Vf b) { return V<float>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
+// material_at — common.wgsl:50-66 — with textureSampleLevel on the
+// repeat / linear sampler (renderer.rs:1024-1031): texel coordinates
+// uv * size - 0.5, four texels with repeat wrapping, decoded (unorm c / 255,
+// or the sRGB decode for the sRGB view), blended bilinearly.
+struct Texel4 {
+  float v[4];
+};
+float srgb_decode8(uint32_t k) {
+  const double c = k / 255.0;
+  return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+Texel4 texel_at(const oracle_geometry* g, uint32_t layer, uint32_t x, uint32_t y, bool srgb) {
+  const uint8_t* p = g->textures + 4 * (((size_t)layer * g->texture_size + y) * g->texture_size + x);
+  Texel4 t;
+  for (int k = 0; k < 3; ++k) t.v[k] = srgb ? srgb_decode8(p[k]) : (float)p[k] / 255.0f;
+  t.v[3] = (float)p[3] / 255.0f;
+  return t;
+}
+uint32_t wrap_i(int32_t i, uint32_t n) {
+  const int32_t m = i % (int32_t)n;
+  return (uint32_t)(m < 0 ? m + (int32_t)n : m);
+}
+Texel4 sample_layer(const oracle_geometry* g, uint32_t layer, float u, float v, bool srgb) {
+  const uint32_t S = g->texture_size;
+  const float x = u * (float)S - 0.5f, y = v * (float)S - 0.5f;
+  const float fx0 = std::floor(x), fy0 = std::floor(y);
+  const float fx = x - fx0, fy = y - fy0;
+  const int32_t ix = (int32_t)std::fmin(std::fmax(fx0, -1073741824.0f), 1073741824.0f);
+  const int32_t iy = (int32_t)std::fmin(std::fmax(fy0, -1073741824.0f), 1073741824.0f);
+  const Texel4 a = texel_at(g, layer, wrap_i(ix, S), wrap_i(iy, S), srgb);
+  const Texel4 b = texel_at(g, layer, wrap_i(ix + 1, S), wrap_i(iy, S), srgb);
+  const Texel4 c = texel_at(g, layer, wrap_i(ix, S), wrap_i(iy + 1, S), srgb);
+  const Texel4 d = texel_at(g, layer, wrap_i(ix + 1, S), wrap_i(iy + 1, S), srgb);
+  const float gx = 1.0f - fx, gy = 1.0f - fy;
+  Texel4 r;
+  for (int k = 0; k < 4; ++k) r.v[k] = (a.v[k] * gx + b.v[k] * fx) * gy + (c.v[k] * gx + d.v[k] * fx) * fy;
+  return r;
+}
+bool is_textured(const irt_material& m) {
+  return m.textures[0] >= 0 || m.textures[1] >= 0 || m.textures[2] >= 0 || m.textures[3] >= 0;
+}
+// n: interpolated normal; tangent xyz + handedness w.
+Surface<float> material_at(const oracle_geometry* g, const irt_material& m, float u, float v, Vf n, const float tangent[4]) {
+  Surface<float> s;
+  s.p = V<float>(0, 0, 0);
+  const float tu = u, tv = 1.0f - v;
+  s.color = F3<float>(m.color);
+  s.rough = m.params[0];
+  s.metal = m.params[1];
+  s.n = normalize3(n);
+  if (m.textures[0] >= 0) {
+    const Texel4 c = sample_layer(g, (uint32_t)m.textures[0], tu, tv, m.params[3] > 0.5f);
+    s.color = mul3(s.color, V<float>(c.v[0], c.v[1], c.v[2]));
+  }
+  if (m.textures[1] >= 0) s.rough = s.rough * sample_layer(g, (uint32_t)m.textures[1], tu, tv, false).v[m.channels[1]];
+  if (m.textures[2] >= 0) s.metal = s.metal * sample_layer(g, (uint32_t)m.textures[2], tu, tv, false).v[m.channels[2]];
+  if (m.textures[3] >= 0) {
+    const Texel4 t4 = sample_layer(g, (uint32_t)m.textures[3], tu, tv, false);
+    Vf nm = V<float>(t4.v[0] * 2.0f - 1.0f, t4.v[1] * 2.0f - 1.0f, t4.v[2] * 2.0f - 1.0f);
+    nm = normalize3(V<float>(nm.x * m.params[2], nm.y * m.params[2], nm.z));
+    const Vf tg = V<float>(tangent[0], tangent[1], tangent[2]);
+    const Vf t = normalize3(sub3(tg, scale3(s.n, dot3(s.n, tg))));
+    const Vf b = scale3(cross3(s.n, t), tangent[3]);
+    s.n = normalize3(add3(add3(scale3(t, nm.x), scale3(b, nm.y)), scale3(s.n, nm.z)));
+  }
+  s.rough = std::fmin(std::fmax(s.rough, 0.045f), 1.0f);
+  s.metal = std::fmin(std::fmax(s.metal, 0.0f), 1.0f);
+  return s;
+}
+
 // column-major 4x4 times (x, y, z, w), row by row in column order
 Vf xform(const float* m, float x, float y, float z, float w) {
   return V<float>(m[0] * x + m[4] * y + m[8] * z + m[12] * w, m[1] * x + m[5] * y + m[9] * z + m[13] * w,
@@ -943,6 +1013,25 @@ Hit trace_all(const oracle_scene* sc, const oracle_geometry* g, Vf o, Vf d) {
   return h;
 }
 
+// sin / cos of the VPL pass's angles: quarter-turn reduction in two
+// constants (after Cody & Waite 1980), Taylor polynomials to degree 11 / 12
+// (truncation below 2e-9 on |r| <= pi/4), the device's float operations.
+void vpl_sincos(float x, float* sn, float* cs) {
+  const float k = std::nearbyint(x * 0.63661977236758134f);
+  const float r = (x - k * 1.57079637050628662f) - k * -4.37113900018624283e-8f;
+  const float r2 = r * r;
+  const float s = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f + r2 * (-1.0f / 39916800.0f)))));
+  const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f + r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
+  const int q = ((int)k) & 3;
+  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+float vpl_cos(float x) {
+  float s, c;
+  vpl_sincos(x, &s, &c);
+  return c;
+}
+
 // basis — common.wgsl:84, applied to v.
 Vf basis_mul(Vf n, Vf v) {
   const Vf axis = std::fabs(n.z) > 0.9f ? V<float>(1, 0, 0) : V<float>(0, 0, 1);
@@ -958,7 +1047,9 @@ const float PI32 = 3.14159265359f;
 Vf cosine_direction(Vf n, uint32_t* rng) {
   const float u = random_f32(rng);
   const float phi = 2.0f * PI32 * random_f32(rng);
-  return basis_mul(n, V<float>(std::sqrt(u) * std::cos(phi), std::sqrt(u) * std::sin(phi), std::sqrt(1.0f - u)));
+  float sn, cs;
+  vpl_sincos(phi, &sn, &cs);
+  return basis_mul(n, V<float>(std::sqrt(u) * cs, std::sqrt(u) * sn, std::sqrt(1.0f - u)));
 }
 
 // sample_bsdf — common.wgsl:86-97.  pow(rough, 4) as (rough^2)^2.
@@ -972,7 +1063,9 @@ void sample_bsdf(Oracle<float>& o, const Surface<float>& s, Vf wo, uint32_t* rng
     const float cos_theta = std::sqrt((1.0f - u) / (1.0f + (a2 - 1.0f) * u));
     const float phi = 2.0f * PI32 * random_f32(rng);
     const float st = std::sqrt(1.0f - cos_theta * cos_theta);
-    const Vf h = basis_mul(s.n, V<float>(st * std::cos(phi), st * std::sin(phi), cos_theta));
+    float sn, cs;
+    vpl_sincos(phi, &sn, &cs);
+    const Vf h = basis_mul(s.n, V<float>(st * cs, st * sn, cos_theta));
     const Vf e = scale3(wo, -1.0f);
     wi = sub3(e, scale3(h, 2.0f * dot3(h, e)));  // reflect(-wo, h)
   } else {
@@ -1010,16 +1103,21 @@ irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geom
   Vf origin = F3<float>(l.position_strength), direction;
   float omega = 4.0f * PI32;
   if (l.color_type[3] > 0.5f) {  // vpl.wgsl:8-10
-    omega = 2.0f * PI32 * (1.0f - std::cos(l.direction_angle[3] * 0.5f));
+    const float ch = vpl_cos(l.direction_angle[3] * 0.5f);
+    omega = 2.0f * PI32 * (1.0f - ch);
     const float r = random_f32(&rng);
-    const float z = std::cos(l.direction_angle[3] * 0.5f) * (1.0f - r) + 1.0f * r;
+    const float z = ch * (1.0f - r) + 1.0f * r;
     const float phi = 2.0f * PI32 * random_f32(&rng);
+    float sn, cs;
+    vpl_sincos(phi, &sn, &cs);
     direction = basis_mul(normalize3(F3<float>(l.direction_angle)),
-                          V<float>(std::sqrt(1.0f - z * z) * std::cos(phi), std::sqrt(1.0f - z * z) * std::sin(phi), z));
+                          V<float>(std::sqrt(1.0f - z * z) * cs, std::sqrt(1.0f - z * z) * sn, z));
   } else {  // vpl.wgsl:11
     const float z = 2.0f * random_f32(&rng) - 1.0f;
     const float phi = 2.0f * PI32 * random_f32(&rng);
-    direction = V<float>(std::sqrt(1.0f - z * z) * std::cos(phi), std::sqrt(1.0f - z * z) * std::sin(phi), z);
+    float sn, cs;
+    vpl_sincos(phi, &sn, &cs);
+    direction = V<float>(std::sqrt(1.0f - z * z) * cs, std::sqrt(1.0f - z * z) * sn, z);
   }
   const float Lf = (float)L;  // vpl.wgsl:14
   const Vf source_flux = V<float>(l.color_type[0] * l.position_strength[3] * omega * Lf,
@@ -1045,12 +1143,20 @@ irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geom
                            a.normal[1] * bx + b.normal[1] * by + c.normal[1] * bz,
                            a.normal[2] * bx + b.normal[2] * by + c.normal[2] * bz);
     const Vf n = normalize3(xform(ob.model, ln.x, ln.y, ln.z, 0.0f));
-    const irt_material& m = g->materials[ob.info[1]];  // material_at, common.wgsl:50-66 (untextured)
-    Surface<float> s;
-    s.color = F3<float>(m.color);
-    s.rough = clamp32(m.params[0], 0.045f, 1.0f);
-    s.metal = clamp32(m.params[1], 0.0f, 1.0f);
-    s.n = normalize3(n);
+    const irt_material& m = g->materials[ob.info[1]];  // material_at, vpl.wgsl:27-29
+    float tu = 0.0f, tv = 0.0f, tangent[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    if (is_textured(m)) {
+      tu = a.uv[0] * bx + b.uv[0] * by + c.uv[0] * bz;
+      tv = a.uv[1] * bx + b.uv[1] * by + c.uv[1] * bz;
+      float t4[4];
+      for (int k = 0; k < 4; ++k) t4[k] = a.tangent[k] * bx + b.tangent[k] * by + c.tangent[k] * bz;
+      const Vf wt = xform(ob.model, t4[0], t4[1], t4[2], 0.0f);
+      tangent[0] = wt.x;
+      tangent[1] = wt.y;
+      tangent[2] = wt.z;
+      tangent[3] = t4[3] >= 0.0f ? 1.0f : -1.0f;
+    }
+    Surface<float> s = material_at(g, m, tu, tv, n, tangent);
     s.p = add3(origin, scale3(direction, hit.t));  // vpl.wgsl:30
     const Vf wo = scale3(direction, -1.0f);
     if (dot3(s.n, wo) <= 0.0f) {  // vpl.wgsl:32
@@ -1412,14 +1518,28 @@ int oracle_render_gbuffer(const irt_frame* frame, const oracle_scene* scene, con
       const float qx = pc[0].x * l0 + pc[1].x * l1 + pc[2].x * l2, qy = pc[0].y * l0 + pc[1].y * l1 + pc[2].y * l2;
       const float qw = pc[0].w * l0 + pc[1].w * l1 + pc[2].w * l2;
       const irt_material& m = g->materials[ob.info[1]];
-      const Vf sn = normalize3(ni);
+      float tu = 0.0f, tv = 0.0f, tangent[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+      if (is_textured(m)) {
+        float ut[3], vt[3], tg[3][4];
+        for (int k = 0; k < 3; ++k) {
+          const irt_vertex& vx = g->vertices[vi[k]];
+          ut[k] = vx.uv[0];
+          vt[k] = vx.uv[1];
+          const Vf wt = normalize3(xform(ob.model, vx.tangent[0], vx.tangent[1], vx.tangent[2], 0.0f));
+          tg[k][0] = wt.x; tg[k][1] = wt.y; tg[k][2] = wt.z; tg[k][3] = vx.tangent[3];
+        }
+        tu = ut[0] * l0 + ut[1] * l1 + ut[2] * l2;
+        tv = vt[0] * l0 + vt[1] * l1 + vt[2] * l2;
+        for (int k = 0; k < 4; ++k) tangent[k] = tg[0][k] * l0 + tg[1][k] * l1 + tg[2][k] * l2;
+      }
+      const Surface<float> sf = material_at(g, m, tu, tv, ni, tangent);
       uint32_t zb = (uint32_t)(key >> 32);
       std::memcpy(&depth[id], &zb, 4);
-      np[0] = f32_to_f16(sn.x); np[1] = f32_to_f16(sn.y); np[2] = f32_to_f16(sn.z); np[3] = f32_to_f16(1.0f);
-      cp[0] = f32_to_f16(m.color[0]); cp[1] = f32_to_f16(m.color[1]); cp[2] = f32_to_f16(m.color[2]);
+      np[0] = f32_to_f16(sf.n.x); np[1] = f32_to_f16(sf.n.y); np[2] = f32_to_f16(sf.n.z); np[3] = f32_to_f16(1.0f);
+      cp[0] = f32_to_f16(sf.color.x); cp[1] = f32_to_f16(sf.color.y); cp[2] = f32_to_f16(sf.color.z);
       cp[3] = f32_to_f16(1.0f);
-      params[2 * id] = f32_to_f16(clamp32(m.params[0], 0.045f, 1.0f));
-      params[2 * id + 1] = f32_to_f16(clamp32(m.params[1], 0.0f, 1.0f));
+      params[2 * id] = f32_to_f16(sf.rough);
+      params[2 * id + 1] = f32_to_f16(sf.metal);
       motion[2 * id] = f32_to_f16((cx / cw - qx / qw) * 0.5f);
       motion[2 * id + 1] = f32_to_f16((cy / cw - qy / qw) * -0.5f);
     }

@@ -125,6 +125,8 @@ typedef struct oracle_geometry {
   const uint32_t* indices; /* mesh-local, located by irt_mesh.geometry */
   const irt_material* materials;
   uint32_t n_vertices, n_indices, n_materials;
+  const uint8_t* textures; /* RGBA8 layers, size x size, rows top to bottom (may be null) */
+  uint32_t texture_size, texture_layers;
 } oracle_geometry;
 
 /* vpl.wgsl:2-47 (FP32): frame->counts.y paths into out[0..counts.y).  The

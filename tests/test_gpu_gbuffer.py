@@ -135,3 +135,36 @@ def test_render_gbuffer_needs_geometry(require_gpu):
         r.sync()
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_textured_quads_match_oracle(require_gpu, precision):
+    import ctypes
+    from instantradiositytest_amd._lib import check
+    from instantradiositytest_amd.renderer import pack_frame, read_buffer
+    from instantradiositytest_amd.types import make_settings, ptr
+    from vpl_scenes import textured_quads
+    W, H = 96, 64
+    sc = textured_quads()
+    L = _lib.irt()
+    s = make_settings(render_width=W, render_height=H)
+    ctx = ctypes.c_void_p()
+    check(L.irt_create(ctypes.byref(s), 1 if precision == "mixed" else 0, ctypes.byref(ctx)))
+    try:
+        check(L.irt_set_sdf_atlas(ctx, ptr(sc.atlas), 2, 2, 2), ctx)
+        check(L.irt_set_meshes(ctx, ptr(sc.meshes), len(sc.meshes)), ctx)
+        check(L.irt_set_objects(ctx, ptr(sc.objects), len(sc.objects), ptr(sc.nodes), len(sc.nodes)), ctx)
+        check(L.irt_set_lights(ctx, ptr(sc.lights), len(sc.lights)), ctx)
+        check(L.irt_set_geometry(ctx, ptr(sc.vertices), len(sc.vertices), ptr(sc.indices), len(sc.indices),
+                                 ptr(sc.materials), len(sc.materials)), ctx)
+        vp = np.eye(4, dtype=np.float32).reshape(-1)
+        f = pack_frame(s, vp, vp, np.zeros(3, np.float32), 0, 1, 0, False, 0, 1, 2)
+        check(L.irt_set_frame(ctx, ptr(f)), ctx)
+        assert L.irt_render_gbuffer(ctx) == -4  # textures named but not set
+        check(L.irt_set_textures(ctx, ptr(sc.textures), sc.textures.shape[1], sc.textures.shape[0]), ctx)
+        check(L.irt_render_gbuffer(ctx), ctx)
+        got = tuple(read_buffer(ctx, n, W, H) for n in PLANES)
+        ref = oracle.render_gbuffer(f, sc.oracle_scene(), sc.oracle_geometry())
+        compare(got, ref, f"textured quads {precision}")
+    finally:
+        L.irt_destroy(ctx)

@@ -74,28 +74,36 @@ class TriScene:
     indices: np.ndarray
     materials: np.ndarray
     atlas: np.ndarray
+    textures: np.ndarray | None = None  # uint8 [layers, size, size, 4]
 
     def oracle_scene(self) -> oracle.SceneRecords:
         return oracle.SceneRecords(self.objects, self.meshes, self.nodes, self.lights, self.atlas)
 
     def oracle_geometry(self) -> oracle.GeometryRecords:
-        return oracle.GeometryRecords(self.vertices, self.indices, self.materials)
+        return oracle.GeometryRecords(self.vertices, self.indices, self.materials, self.textures)
 
 
-def build(meshes, objects, materials, lights) -> TriScene:
-    """meshes: [(positions Nx3, normals Nx3, indices)]; objects: [(mesh, 4x4 model, material)];
-    materials: [(rgb, roughness, metallic)]; lights: LIGHT_DT array."""
+def build(meshes, objects, materials, lights, textures=None) -> TriScene:
+    """meshes: [(positions Nx3, normals Nx3, indices[, uv Nx2, tangents Nx4])]; objects: [(mesh, 4x4 model,
+    material)]; materials: [(rgb, roughness, metallic[, options])] where options may give "textures"
+    (4 layer indices, -1 none), "channels", "srgb" and "normal_strength"; lights: LIGHT_DT array;
+    textures: uint8 [layers, size, size, 4] or None."""
     M = np.zeros(len(meshes), MESH_DT)
     verts, idx = [], []
     v_off = i_off = 0
     local_lo, local_hi = [], []
-    for k, (p, n, ix) in enumerate(meshes):
+    for k, mesh in enumerate(meshes):
+        p, n, ix = mesh[:3]
         p = np.asarray(p, np.float32)
         V = np.zeros(len(p), VERTEX_DT)
         V["position"][:, :3] = p
         V["position"][:, 3] = 1
         V["normal"][:, :3] = n
         V["tangent"][:, 3] = 1
+        if len(mesh) > 3:
+            V["uv"][:, :2] = mesh[3]
+        if len(mesh) > 4:
+            V["tangent"][:] = mesh[4]
         verts.append(V)
         idx.append(np.asarray(ix, np.uint32))
         M[k]["geometry"] = (v_off, len(p), i_off, len(ix))
@@ -124,12 +132,16 @@ def build(meshes, objects, materials, lights) -> TriScene:
     N = np.zeros(2 * len(objects) - 1, NODE_DT)
     _lib.irt().irt_host_hierarchy(ptr(box_lo), ptr(box_hi), len(objects), ptr(N))
     MAT = np.zeros(len(materials), MATERIAL_DT)
-    for k, (rgb, rough, metal) in enumerate(materials):
+    for k, mat in enumerate(materials):
+        rgb, rough, metal = mat[:3]
+        opt = mat[3] if len(mat) > 3 else {}
         MAT[k]["color"] = (*rgb, 1)
-        MAT[k]["params"] = (rough, metal, 1, 0)
-        MAT[k]["textures"] = -1
+        MAT[k]["params"] = (rough, metal, opt.get("normal_strength", 1.0), 1.0 if opt.get("srgb") else 0.0)
+        MAT[k]["textures"] = opt.get("textures", (-1, -1, -1, -1))
+        MAT[k]["channels"] = opt.get("channels", (0, 0, 0, 0))
+    tex = None if textures is None else np.ascontiguousarray(textures, np.uint8)
     return TriScene(O, M, N, np.ascontiguousarray(lights, LIGHT_DT), np.concatenate(verts), np.concatenate(idx), MAT,
-                    np.zeros((2, 2, 2), np.float32))
+                    np.zeros((2, 2, 2), np.float32), tex)
 
 
 def vpl_frame(V: int, bounces: int, n_lights: int, n_objects: int, frame_index: int = 0, rr=(0.1, 0.95),
@@ -167,6 +179,8 @@ def device_pool(sc: TriScene, V: int, bounces: int, frame_index: int = 0, precis
         check(L.irt_set_lights(ctx, ptr(sc.lights), len(sc.lights)), ctx)
         check(L.irt_set_geometry(ctx, ptr(sc.vertices), len(sc.vertices), ptr(sc.indices), len(sc.indices),
                                  ptr(sc.materials), len(sc.materials)), ctx)
+        if sc.textures is not None:
+            check(L.irt_set_textures(ctx, ptr(sc.textures), sc.textures.shape[1], sc.textures.shape[0]), ctx)
         f = vpl_frame(V, bounces, len(sc.lights), len(sc.objects), frame_index, rr)
         check(L.irt_set_frame(ctx, ptr(f)), ctx)
         check(L.irt_generate_vpls(ctx), ctx)
@@ -175,3 +189,37 @@ def device_pool(sc: TriScene, V: int, bounces: int, frame_index: int = 0, precis
         return ring[slot * V:(slot + 1) * V].view(np.float32).reshape(V, 12).copy()
     finally:
         L.irt_destroy(ctx)
+
+
+def textured_sphere(lights, size: int = 16, seed: int = 7) -> TriScene:
+    """An inward UV sphere with spherical uv and tangents, an sRGB base-colour
+    texture, a roughness texture (green channel) and a normal map."""
+    p, n, ix = uv_sphere(2.0)
+    d = p / np.linalg.norm(p, axis=1, keepdims=True)
+    uv = np.stack([np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi) + 0.5, np.arccos(np.clip(d[:, 1], -1, 1)) / np.pi], 1)
+    tang = np.stack([-d[:, 2], np.zeros(len(d)), d[:, 0], np.ones(len(d))], 1)
+    tang[np.linalg.norm(tang[:, :3], axis=1) < 1e-6, :3] = (1, 0, 0)
+    rng = np.random.default_rng(seed)
+    tex = rng.integers(0, 256, (3, size, size, 4), dtype=np.uint8)
+    tex[2, ..., 2] = np.maximum(tex[2, ..., 2], 160)  # normal map leaning along +z
+    mat = ((0.9, 0.8, 0.7), 0.9, 0.3, {"textures": (0, 1, -1, 2), "channels": (0, 1, 0, 0), "srgb": True,
+                                       "normal_strength": 0.7})
+    return build([(p, n, ix, uv, tang)], [(0, np.eye(4), 0)], [mat], lights, tex)
+
+
+def textured_quads(size: int = 8, seed: int = 9) -> TriScene:
+    """Two overlapping quads in NDC space (identity view-projection), each with
+    its own textures, uv spanning two texture repeats, and a tangent."""
+    rng = np.random.default_rng(seed)
+    tex = rng.integers(0, 256, (4, size, size, 4), dtype=np.uint8)
+    quad = [0, 1, 2, 0, 2, 3]
+    n = np.tile([0, 0, 1.0], (4, 1))
+    a = np.array([[-1, -1, 0.6], [0.6, -1, 0.6], [0.6, 0.7, 0.6], [-1, 0.7, 0.6]])
+    b = np.array([[-0.3, -0.5, 0.4], [1, -0.6, 0.45], [1, 1, 0.5], [-0.3, 1, 0.42]])
+    uv = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float64)
+    t1 = np.tile([1.0, 0, 0, 1.0], (4, 1))
+    t2 = np.tile([0.0, 1.0, 0, -1.0], (4, 1))
+    mats = [((0.8, 0.7, 0.6), 0.7, 0.2, {"textures": (0, 1, 2, 3), "channels": (0, 2, 1, 0), "srgb": True}),
+            ((0.5, 0.9, 0.4), 0.4, 0.6, {"textures": (1, -1, 0, 3), "channels": (0, 0, 3, 0)})]
+    return build([(a, n, quad, uv, t1), (b, n, quad, uv * 1.5, t2)],
+                 [(0, np.eye(4), 0), (1, np.eye(4), 1)], mats, light((0, 0, 0)), tex)
