@@ -1143,7 +1143,7 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
       e[a] = nodes[0].hi[a] + margin - lo[a];
       ext = std::max(ext, e[a]);
     }
-    const float cell = std::max(ext / 48.0f, 1e-4f);
+    const float cell = std::max(ext / GRID_CELLS_LONG_AXIS, 1e-4f);
     c->grid.x0 = lo[0];
     c->grid.y0 = lo[1];
     c->grid.z0 = lo[2];

@@ -599,14 +599,21 @@ __global__ __launch_bounds__(256) void k_mesh_gradient(SceneView sc, uint32_t n_
   if (threadIdx.x % 64 == 0) atomicMax(k2 + m, __float_as_uint(best));
 }
 
-// Candidate lists, one thread per cell.  Lower bound of object j over the
-// cell: 0.9998 x the gap between the cell and j's world box (the hierarchy's
-// own pruning bound), or -inf when they overlap.  Upper bound: the object
-// distance at the cell centre plus K_j x the half diagonal, K_j a Lipschitz
-// constant of object_distance (1 for certified boxes, sqrt(K_mesh^2 + 1)
-// for grid meshes: the padded-box extension combines the field gradient with
-// a unit outside term), with a relative FP margin.
+// Candidate lists, one thread per cell.  With K_j a Lipschitz constant of
+// object_distance (1 for certified boxes, sqrt(K_mesh^2 + 1) for grid
+// meshes: the padded-box extension combines the field gradient with a unit
+// outside term) and d_j the object distance at the cell centre:
+//   lower bound of object j over the cell: the larger of 0.9998 x the gap
+//     between the cell and j's world box (the hierarchy's own pruning bound;
+//     -inf when they overlap) and d_j - K_j x the half diagonal;
+//   upper bound: d_j + K_j x the half diagonal;
+// both with a relative FP margin.  The second lower bound drops objects whose
+// world box overlaps the cell but whose surface is far from it.
+// K_mesh, the largest corner-gradient norm of the field, bounds the gradient
+// everywhere in a cell: each gradient component is multilinear, so the
+// squared norm is convex along each axis and peaks at a corner.
 constexpr uint32_t GRID_MAX_OBJECTS = 64;
+constexpr float GRID_CELLS_LONG_AXIS = 96.0f;  // cells on the root box's longest axis
 __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_objects, float hit_eps,
                                                     const unsigned int* k2_mesh, CellList* cells) {
   const GridView& g = sc.grid;
@@ -628,7 +635,7 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
     const float gy = fmaxf(fmaxf(blo.y - hi.y, lo.y - bhi.y), 0.0f);
     const float gz = fmaxf(fmaxf(blo.z - hi.z, lo.z - bhi.z), 0.0f);
     const float gap = sqrtf(gx * gx + gy * gy + gz * gz);
-    lower[j] = gap > 0.0f ? 0.9998f * gap * (1.0f - 1e-6f) : -INFINITY;
+    const float box_lower = gap > 0.0f ? 0.9998f * gap * (1.0f - 1e-6f) : -INFINITY;
     const irt_object& obj = sc.objects[j];
     const irt_mesh& mesh = sc.meshes[obj.info[0]];
     float k = 1.0f;
@@ -638,6 +645,9 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
     }
     float vx;
     const float dc = object_distance(sc, center, j, false, hit_eps, &vx, cnt);
+    float lip_lower = dc - k * 1.001f * half_diag;
+    lip_lower -= fabsf(lip_lower) * 1e-5f + 1e-6f;
+    lower[j] = fmaxf(box_lower, lip_lower);
     float up = dc + k * 1.001f * half_diag;
     up += fabsf(up) * 1e-5f + 1e-6f;
     upper_min = fminf(upper_min, up);

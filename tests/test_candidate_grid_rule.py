@@ -2,11 +2,20 @@
 the CPU restatement: for every point, the minimum object distance over its
 cell's candidates equals the hierarchy walk's result (sdf.wgsl:42-60).
 
-Rule (as built on the GPU): lower bound of object j over a cell = 0.9998 x
-the cell-to-world-box gap (-inf when they overlap); upper bound = distance at
-the cell centre + K_j x half diagonal, K_j = 1 for certified boxes and
-sqrt(K_mesh^2 + 1) for grid meshes, K_mesh the largest corner gradient of
-the field; candidates = lower <= min upper."""
+Rule (as built on the GPU): with d_j the distance at the cell centre and
+K_j = 1 for certified boxes, sqrt(K_mesh^2 + 1) for grid meshes (K_mesh the
+largest corner gradient of the field), the lower bound of object j over a
+cell is the larger of 0.9998 x the cell-to-world-box gap (-inf when they
+overlap) and d_j - K_j x half diagonal; the upper bound is d_j + K_j x half
+diagonal; candidates = lower <= min upper.
+
+The bounds are also checked directly: every sampled point of a cell has each
+object's distance inside [lower_j, upper_j].  On this scene the object
+distances are close to 1-Lipschitz where the field decides them (deep inside
+a mesh the box and sphere bounds take over), so that check cannot tell K_j
+from 1; the corner rule behind K_mesh is pinned separately on random
+trilinear cells, where the largest gradient norm over the cell is reached at
+a corner and the cell-centre gradient falls short of it."""
 from __future__ import annotations
 
 import ctypes
@@ -75,19 +84,61 @@ def test_candidate_lists_preserve_the_minimum():
         c1 = c0 + cell
         center = (c0 + c1) / 2
         half = 0.5 * cell * np.sqrt(3) * 1.0001
-        lower, upper = [], []
+        lower, upper, centre_d = [], [], []
         for j in range(n):
             blo, bhi = boxes[j]
             gap = np.linalg.norm(np.maximum(np.maximum(blo - c1, c0 - bhi), 0))
-            lower.append(0.9998 * gap if gap > 0 else -np.inf)
-            up = dist(singles[j], one, center) + K[j] * 1.001 * half
+            dc = dist(singles[j], one, center)
+            centre_d.append(dc)
+            lip = dc - K[j] * 1.001 * half
+            lip -= abs(lip) * 1e-5 + 1e-6
+            lower.append(max(0.9998 * gap if gap > 0 else -np.inf, lip))
+            up = dc + K[j] * 1.001 * half
             upper.append(up + abs(up) * 1e-5 + 1e-6)
         cand = [j for j in range(n) if lower[j] <= min(upper)]
         total_cands += len(cand)
         for p in rng.uniform(c0, c1, (25, 3)):
             full = dist(rec, f, p)
-            best = min(dist(singles[j], one, p) for j in cand)
+            d = [dist(singles[j], one, p) for j in range(n)]
+            best = min(d[j] for j in cand)
             assert best == full
+            for j in range(n):
+                assert lower[j] <= d[j] <= upper[j]
             checked += 1
     assert checked == 1500
-    assert total_cands / 60 < n / 2  # the lists actually cull
+    assert total_cands / 60 < n / 3  # the lists actually cull
+
+
+def trilinear_gradient(v, x, y, z):
+    """Gradient of the trilinear interpolant of corner values v[z, y, x] at
+    (x, y, z) in the unit cell."""
+    gx = ((v[0, 0, 1] - v[0, 0, 0]) * (1 - y) * (1 - z) + (v[0, 1, 1] - v[0, 1, 0]) * y * (1 - z)
+          + (v[1, 0, 1] - v[1, 0, 0]) * (1 - y) * z + (v[1, 1, 1] - v[1, 1, 0]) * y * z)
+    gy = ((v[0, 1, 0] - v[0, 0, 0]) * (1 - x) * (1 - z) + (v[0, 1, 1] - v[0, 0, 1]) * x * (1 - z)
+          + (v[1, 1, 0] - v[1, 0, 0]) * (1 - x) * z + (v[1, 1, 1] - v[1, 0, 1]) * x * z)
+    gz = ((v[1, 0, 0] - v[0, 0, 0]) * (1 - x) * (1 - y) + (v[1, 0, 1] - v[0, 0, 1]) * x * (1 - y)
+          + (v[1, 1, 0] - v[0, 1, 0]) * (1 - x) * y + (v[1, 1, 1] - v[0, 1, 1]) * x * y)
+    return np.stack([gx, gy, gz], -1)
+
+
+def test_trilinear_gradient_peaks_at_a_corner():
+    """K_mesh (k_mesh_gradient, mesh_gradient_bound above) is the largest
+    corner-gradient norm: the bound on |grad| over a whole cell.  Dense
+    sampling of random cells finds no larger norm, reaches it at a corner,
+    and shows that the centre gradient would undershoot."""
+    rng = np.random.default_rng(5)
+    t = np.linspace(0, 1, 21)
+    X, Y, Z = np.meshgrid(t, t, t, indexing="ij")
+    centre_short = 0
+    for _ in range(200):
+        v = rng.normal(size=(2, 2, 2))
+        dense = np.sqrt((trilinear_gradient(v, X, Y, Z) ** 2).sum(-1)).max()
+        corners = max(np.linalg.norm(trilinear_gradient(v, x, y, z)) for z in (0, 1) for y in (0, 1) for x in (0, 1))
+        # the corner-gradient definition of mesh_gradient_bound, on one cell
+        assert np.isclose(max(1.0, corners ** 2), mesh_gradient_bound(v.astype(np.float32), {
+            "field": (0, 0, 0, 2), "grid": np.ones(4, np.float32)}), rtol=1e-5)
+        assert dense <= corners * (1 + 1e-12)
+        assert np.isclose(dense, corners)
+        centre = np.linalg.norm(trilinear_gradient(v, 0.5, 0.5, 0.5))
+        centre_short += centre < dense * 0.99
+    assert centre_short > 150
