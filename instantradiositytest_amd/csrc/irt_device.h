@@ -15,9 +15,32 @@
 
 #include "../../include/irt.h"
 
+// The arithmetic contract of the translation unit: IRT_MARCH_EXACT (one
+// IEEE rounding per source operation, correctly rounded division and square
+// root) unless the unit defines IRT_MARCH_MATH as IRT_MARCH_CONTRACTED before
+// its first include (the mixed profile's march, irt_march_contracted.hip).
+#define IRT_MARCH_EXACT 0
+#define IRT_MARCH_CONTRACTED 1
+#ifndef IRT_MARCH_MATH
+#define IRT_MARCH_MATH IRT_MARCH_EXACT
+#endif
+
 namespace irt {
 
 #define IRT_PI 3.14159265359f
+
+// Square root.  The exact contract keeps the correctly rounded sqrtf.  The
+// contracted unit (built without correctly rounded square roots) lowers sqrtf
+// to the hardware square root plus a rescaling of inputs below 2^-96; calling
+// the hardware instruction directly gives the same result for every input at
+// or above 2^-96 and drops the rescaling (five instructions per root).
+__device__ __forceinline__ float fsqrt(float x) {
+#if IRT_MARCH_MATH == IRT_MARCH_CONTRACTED
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
 
 // ------------------------------------------------------------------ vectors
 struct v3 {
@@ -32,10 +55,10 @@ __device__ __forceinline__ v3 operator*(v3 a, float s) { return v3{a.x * s, a.y 
 __device__ __forceinline__ v3 operator*(float s, v3 a) { return v3{s * a.x, s * a.y, s * a.z}; }
 __device__ __forceinline__ v3 operator/(v3 a, float s) { return v3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float length(v3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float length(v3 a) { return fsqrt(dot(a, a)); }
 // WGSL normalize(e) = e / length(e), divided per component.
 __device__ __forceinline__ v3 normalize(v3 a) {
-  float l = sqrtf(dot(a, a));
+  float l = fsqrt(dot(a, a));
   return v3{a.x / l, a.y / l, a.z / l};
 }
 __device__ __forceinline__ v3 vmax(v3 a, v3 b) { return v3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
@@ -428,7 +451,7 @@ __device__ __forceinline__ float mesh_distance(const SceneView& sc, v3 p, const 
   float outside_squared = dot(outside, outside);
   if (outside_squared > 0.0f) {
     float positive = fmaxf(bound, 0.0f);
-    bound = sqrtf(positive * positive + outside_squared);
+    bound = fsqrt(positive * positive + outside_squared);
   }
   if (bound > 0.0f) bound *= mesh.geometry_hi[3];
   float sphere_bound = length(p - ld3(mesh.enclosing_sphere)) - mesh.enclosing_sphere[3];
@@ -562,7 +585,7 @@ __device__ __forceinline__ float visibility(const SceneView& sc, const irt_frame
     closest = qd.object_id;
     if (h < eps0) return 0.0f;
     float y = h * h / (2.0f * previous);
-    float d = sqrtf(fmaxf(h * h - y * y, 0.0f));
+    float d = fsqrt(fmaxf(h * h - y * y, 0.0f));
     if (t > y) result = fminf(result, k * d / (t - y));
     if (result < 0.001f) return 0.0f;
     previous = h;
@@ -636,7 +659,7 @@ __device__ __forceinline__ bool march_step(const SceneView& sc, const irt_frame&
     return true;
   }
   const float y = h * h / (2.0f * s.previous);
-  const float d = sqrtf(fmaxf(h * h - y * y, 0.0f));
+  const float d = fsqrt(fmaxf(h * h - y * y, 0.0f));
   if (s.t > y) s.result = fminf(s.result, F.sdf[2] * d / (s.t - y));
   if (s.result < 0.001f) {
     *vis = 0.0f;

@@ -16,12 +16,6 @@
 
 #include "irt_passes.h"
 
-#define IRT_MARCH_EXACT 0
-#define IRT_MARCH_CONTRACTED 1
-#ifndef IRT_MARCH_MATH
-#define IRT_MARCH_MATH IRT_MARCH_EXACT
-#endif
-
 namespace irt {
 
 __device__ __forceinline__ uint64_t lanes_below() {
@@ -135,7 +129,7 @@ __device__ __forceinline__ void march_queue(const SceneView& lsc, const irt_fram
           done = true;
         } else {
           const float y = h * h / (2.0f * st.previous);
-          const float dd = sqrtf(fmaxf(h * h - y * y, 0.0f));
+          const float dd = fsqrt(fmaxf(h * h - y * y, 0.0f));
           if (st.t > y) st.result = fminf(st.result, F.sdf[2] * dd / (st.t - y));
           if (st.result < 0.001f) {
             v = 0.0f;
