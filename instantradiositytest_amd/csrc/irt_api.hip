@@ -119,6 +119,8 @@ struct irt_ctx {
   uint32_t cap_srgb_lut = 0;
   // G-buffer pass (irt_render_gbuffer)
   std::vector<uint32_t> host_obj_mesh;   // mesh index of each object
+  std::vector<irt_object> host_objects;  // the last uploaded object records and hierarchy
+  std::vector<irt_node> host_nodes;
   std::vector<uint32_t> host_tri_base;   // instance-triangle prefix sums the device holds
   uint32_t* obj_tri_base = nullptr;
   uint32_t cap_obj_tri_base = 0;
@@ -1110,9 +1112,18 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
       return c->fail(IRT_ERR_INVALID_ARGUMENT, "malformed hierarchy node");
     if (nd.info[3] <= i) return c->fail(IRT_ERR_INVALID_ARGUMENT, "hierarchy escape link must point forward");
   }
+  // A host that uploads its objects every frame (renderer.rs:1736-1782) mostly
+  // sends the records it sent last time: keep the device copies and the
+  // candidate grid built over them.
+  if (c->objects && c->host_objects.size() == count && c->host_nodes.size() == node_count &&
+      std::memcmp(c->host_objects.data(), objects, sizeof(irt_object) * count) == 0 &&
+      std::memcmp(c->host_nodes.data(), nodes, sizeof(irt_node) * node_count) == 0)
+    return IRT_OK;
   int rc = ensure_array(c, &c->objects, &c->cap_objects, count);
   if (!rc) rc = ensure_array(c, &c->nodes, &c->cap_nodes, node_count);
   if (rc) return rc;
+  c->host_objects.clear();  // stays empty unless the whole upload succeeds
+  c->host_nodes.clear();
   HIP_CHECK(c, hipMemcpyAsync(c->objects, objects, sizeof(irt_object) * count, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(c, hipMemcpyAsync(c->nodes, nodes, sizeof(irt_node) * node_count, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -1153,6 +1164,8 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
     c->grid.nz = std::max(1u, (uint32_t)std::ceil(e[2] / cell));
   }
   c->grid_dirty = true;
+  c->host_objects.assign(objects, objects + count);
+  c->host_nodes.assign(nodes, nodes + node_count);
   return IRT_OK;
 }
 
@@ -1220,6 +1233,8 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
   dst->max_mesh_ref = src->max_mesh_ref;
   dst->max_material_ref = src->max_material_ref;
   dst->host_obj_mesh = src->host_obj_mesh;
+  dst->host_objects = src->host_objects;
+  dst->host_nodes = src->host_nodes;
   dst->have_geometry = false;
   if (src->have_geometry) {
     rc = ensure_array(dst, &dst->vertices, &dst->cap_vertices, src->n_vertices);

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(256) void k_mesh_gradient(SceneView sc, uint32_t n_
 // everywhere in a cell: each gradient component is multilinear, so the
 // squared norm is convex along each axis and peaks at a corner.
 constexpr uint32_t GRID_MAX_OBJECTS = 64;
-constexpr float GRID_CELLS_LONG_AXIS = 96.0f;  // cells on the root box's longest axis
+constexpr float GRID_CELLS_LONG_AXIS = 160.0f;  // cells on the root box's longest axis
 __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_objects, float hit_eps,
                                                     const unsigned int* k2_mesh, CellList* cells) {
   const GridView& g = sc.grid;
@@ -626,17 +626,36 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
   const v3 hi = v3{lo.x + cs, lo.y + cs, lo.z + cs};
   const v3 center = v3{(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
   const float half_diag = 0.5f * cs * 1.7320508f * 1.0001f;
+  // Box-gap lower bounds of every object, then the objects in increasing
+  // gap order until the next gap exceeds the smallest upper bound so far.
+  // An object's distance at the centre is at least its gap (the world box
+  // holds the object; positive distances carry the 0.9999 factor, the gap
+  // 0.9998), so an object left out has an upper bound above that minimum and
+  // a lower bound above it too: it neither lowers the minimum nor becomes a
+  // candidate, and the lists equal those of evaluating every object.
   float lower[GRID_MAX_OBJECTS];
-  float upper_min = INFINITY;
-  Count<false> cnt;
   for (uint32_t j = 0; j < n_objects; ++j) {
     const float4 blo = g.boxes[2 * j], bhi = g.boxes[2 * j + 1];
     const float gx = fmaxf(fmaxf(blo.x - hi.x, lo.x - bhi.x), 0.0f);
     const float gy = fmaxf(fmaxf(blo.y - hi.y, lo.y - bhi.y), 0.0f);
     const float gz = fmaxf(fmaxf(blo.z - hi.z, lo.z - bhi.z), 0.0f);
     const float gap = sqrtf(gx * gx + gy * gy + gz * gz);
-    const float box_lower = gap > 0.0f ? 0.9998f * gap * (1.0f - 1e-6f) : -INFINITY;
-    const irt_object& obj = sc.objects[j];
+    lower[j] = gap > 0.0f ? 0.9998f * gap * (1.0f - 1e-6f) : -INFINITY;
+  }
+  float upper_min = INFINITY;
+  uint64_t evaluated = 0;
+  Count<false> cnt;
+  while (true) {
+    uint32_t next = n_objects;
+    float next_gap = INFINITY;
+    for (uint32_t j = 0; j < n_objects; ++j)
+      if (!((evaluated >> j) & 1ull) && lower[j] < next_gap) {
+        next_gap = lower[j];
+        next = j;
+      }
+    if (next == n_objects || !(next_gap <= upper_min)) break;
+    evaluated |= 1ull << next;
+    const irt_object& obj = sc.objects[next];
     const irt_mesh& mesh = sc.meshes[obj.info[0]];
     float k = 1.0f;
     if (mesh.geometry_lo[3] <= 0.5f) {
@@ -644,10 +663,10 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
       k = sqrtf(k2 + 1.0f);
     }
     float vx;
-    const float dc = object_distance(sc, center, j, false, hit_eps, &vx, cnt);
+    const float dc = object_distance(sc, center, next, false, hit_eps, &vx, cnt);
     float lip_lower = dc - k * 1.001f * half_diag;
     lip_lower -= fabsf(lip_lower) * 1e-5f + 1e-6f;
-    lower[j] = fmaxf(box_lower, lip_lower);
+    lower[next] = fmaxf(lower[next], lip_lower);
     float up = dc + k * 1.001f * half_diag;
     up += fabsf(up) * 1e-5f + 1e-6f;
     upper_min = fminf(upper_min, up);
@@ -658,7 +677,7 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
   uint32_t n = 0;
   uint8_t ids[GRID_MAX_OBJECTS];
   for (uint32_t j = 0; j < n_objects; ++j) {
-    if (!(lower[j] <= upper_min)) continue;
+    if (!((evaluated >> j) & 1ull) || !(lower[j] <= upper_min)) continue;
     uint32_t k = n++;
     while (k > 0 && lower[ids[k - 1]] > lower[j]) {
       ids[k] = ids[k - 1];
