@@ -363,9 +363,10 @@ __device__ __forceinline__ irt_reservoir cap(irt_reservoir a, uint32_t limit) {
 // queries in that cell walk the hierarchy instead.
 struct CellList {
   uint8_t count;
-  uint8_t ids[15];
+  uint8_t ids[7];
 };
-static_assert(sizeof(CellList) == 16, "cell list is 16 bytes");
+static_assert(sizeof(CellList) == 8, "cell list is 8 bytes");
+constexpr uint32_t CELL_MAX = 7;  // longer lists overflow
 constexpr uint8_t CELL_OVERFLOW = 255;
 struct GridView {
   const CellList* cells;  // null: no grid, hierarchy walk only
@@ -489,17 +490,16 @@ __device__ __forceinline__ bool scene_distance_grid(const SceneView& sc, float h
   if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)g.nx && fy < (float)g.ny && fz < (float)g.nz))
     return false;
   const uint32_t cx = min((uint32_t)fx, g.nx - 1u), cy = min((uint32_t)fy, g.ny - 1u), cz = min((uint32_t)fz, g.nz - 1u);
-  // The 16-byte list as four dwords; byte 0 is the count, byte k+1 the k-th id
+  // The 8-byte list as two dwords; byte 0 is the count, byte k+1 the k-th id
   // (register selects rather than a dynamically indexed array).
-  const uint4 L = reinterpret_cast<const uint4*>(g.cells)[((size_t)cz * g.ny + cy) * g.nx + cx];
+  const uint2 L = reinterpret_cast<const uint2*>(g.cells)[((size_t)cz * g.ny + cy) * g.nx + cx];
   const uint32_t count = L.x & 0xffu;
   if (count == CELL_OVERFLOW) return false;
   float best = 1e20f, voxel = hit_eps;
   uint32_t closest = hint;
   for (uint32_t k = 0; k < count; ++k) {
-    const uint32_t b = k + 1u, word = b >> 2;
-    const uint32_t w = word == 0u ? L.x : (word == 1u ? L.y : (word == 2u ? L.z : L.w));
-    const uint32_t j = (w >> ((b & 3u) * 8u)) & 0xffu;
+    const uint32_t b = k + 1u;
+    const uint32_t j = ((b < 4u ? L.x : L.y) >> ((b & 3u) * 8u)) & 0xffu;
     cnt.add(C_NODE, 1);
     const float4 lo = g.boxes[2 * j], hi = g.boxes[2 * j + 1];
     v3 delta = vmaxs(vmax(v3{lo.x - p.x, lo.y - p.y, lo.z - p.z}, v3{p.x - hi.x, p.y - hi.y, p.z - hi.z}), 0.0f);

@@ -685,11 +685,12 @@ __global__ __launch_bounds__(64) void k_build_cells(SceneView sc, uint32_t n_obj
     }
     ids[k] = (uint8_t)j;
   }
-  if (n > 15) {
+  if (n > CELL_MAX) {
     out.count = CELL_OVERFLOW;
+    for (uint32_t k = 0; k < CELL_MAX; ++k) out.ids[k] = 0;
   } else {
     out.count = (uint8_t)n;
-    for (uint32_t k = 0; k < 15; ++k) out.ids[k] = k < n ? ids[k] : 0;
+    for (uint32_t k = 0; k < CELL_MAX; ++k) out.ids[k] = k < n ? ids[k] : 0;
   }
   cells[cell] = out;
 }
