@@ -216,8 +216,8 @@ __device__ __forceinline__ void body_spatial_setup(const LightArgs<P>& a, const 
   bool want_d = false, want_i = false;
   RayReq ray_d{}, ray_i{};
   if (p.active) {
-    irt_reservoir rd = spatial_direct(a, a.direct_in[p.id], p.x, p.y, valid, s, cnt);
-    irt_reservoir ri = spatial_indirect(a, a.indirect_in[p.id], p.x, p.y, valid, s, cnt);
+    irt_reservoir rd = a.direct_in[p.id], ri = a.indirect_in[p.id];
+    spatial_both(a, rd, ri, p.x, p.y, valid, s, cnt);
     a.direct_out[p.id] = rd;
     a.indirect_out[p.id] = ri;
     ResolveRec r{};

@@ -256,6 +256,52 @@ __device__ __forceinline__ irt_reservoir spatial_indirect(const LightArgs<P>& a,
   return cap(r, f2u_sat(F.rr[2]) * F.restir[1] * 8u);
 }
 
+// spatial_direct and spatial_indirect of one pixel, neighbour i of both
+// domains per iteration (the fused set-up).  Each domain keeps its own random
+// stream and its own order of neighbour, compatibility test, reservoir load
+// and add, so the reservoirs equal those of the two loops run one after the
+// other; interleaving only lets the two domains' dependent loads overlap.
+// The compatibility tests read the guide records (a.guide is set).
+template <class P, class C>
+__device__ __forceinline__ void spatial_both(const LightArgs<P>& a, irt_reservoir& rd, irt_reservoir& ri, int32_t x,
+                                             int32_t y, bool valid, const Surface& s, C& cnt) {
+  const irt_frame& F = a.F;
+  const int32_t W = (int32_t)F.size[0], H = (int32_t)F.size[1];
+  const uint32_t id = (uint32_t)y * (uint32_t)W + (uint32_t)x;
+  uint32_t rng_d = seed(id, 23u, F.size[2]), rng_i = seed(id, 29u, F.size[2]);
+  if (valid) {
+    for (uint32_t i = 0; i < F.restir[2]; ++i) {
+      int32_t dx, dy, ix, iy;
+      neighbor(F, x, y, rng_d, &dx, &dy);
+      neighbor(F, x, y, rng_i, &ix, &iy);
+      const bool in_d = !(dx < 0 || dy < 0 || dx >= W || dy >= H);
+      const bool in_i = !(ix < 0 || iy < 0 || ix >= W || iy >= H);
+      // both guide records are loaded before either is tested (clamped
+      // coordinates keep an outside neighbour's load inside the image)
+      const bool ok_d = in_d && compatible(a, s, min(max(dx, 0), W - 1), min(max(dy, 0), H - 1));
+      const bool ok_i = in_i && compatible(a, s, min(max(ix, 0), W - 1), min(max(iy, 0), H - 1));
+      if (ok_d) {
+        irt_reservoir prev = a.direct_in[(uint32_t)dy * (uint32_t)W + (uint32_t)dx];
+        if (prev.selected_target > 0.0f && prev.id < F.counts[0]) {
+          float target = luminance(direct(s, a.sc.lights[prev.id], cnt));
+          add(rd, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count,
+              prev.count, prev.age, rng_d);
+        }
+      }
+      if (ok_i) {
+        irt_reservoir prev = a.indirect_in[(uint32_t)iy * (uint32_t)W + (uint32_t)ix];
+        if (vpl_sample_valid(F, prev)) {
+          float target = luminance(indirect(s, selected_vpl(F, a.vpl_ring, prev), cnt));
+          add(ri, prev.id, prev.source, target, prev.pdf, target * normalization(prev) * (float)prev.count,
+              prev.count, prev.age, rng_i);
+        }
+      }
+    }
+  }
+  rd = cap(rd, f2u_sat(F.rr[2]) * F.restir[0] * 8u);
+  ri = cap(ri, f2u_sat(F.rr[2]) * F.restir[1] * 8u);
+}
+
 // resolve_direct — lighting.wgsl:106-122.  `d` is the final direct reservoir.
 template <class P, class C>
 __device__ __forceinline__ v3 resolve_direct(const LightArgs<P>& a, const irt_reservoir& d, uint32_t id, bool valid,

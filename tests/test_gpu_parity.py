@@ -188,6 +188,32 @@ def test_fused_schedule_matches_pass_schedule(require_gpu, name, w, h):
     assert s["within"] >= IMG_WITHIN
 
 
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_fused_reservoirs_equal_pass_reservoirs(require_gpu, precision):
+    """Reservoirs never depend on visibility, so the fused set-up (both
+    domains' spatial reuse interleaved per neighbour, guide-record
+    compatibility tests) must leave the same reservoir bits as the per-pass
+    kernels, frame after frame."""
+    log(f"[parity] fused vs passes reservoirs {precision} 320x200 x4 frames")
+    cfg, cs = small_config(320, 200, 4, "plain-1920x1200")
+    a = Renderer(cs.scene, cfg.settings(), precision=precision, schedule="passes")
+    b = Renderer(cs.scene, cfg.settings(), precision=precision, schedule="fused")
+    try:
+        for fi in range(4):
+            inputs = cs.frame_inputs(fi)
+            a.encode(cs.scene, inputs)
+            b.encode(cs.scene, inputs)
+            a.sync()
+            b.sync()
+            for buf in ("direct2", "indirect2"):
+                s = compare_reservoirs(b.read(buf), to_oracle_reservoirs(a.read(buf)))
+                log(fi, buf, s)
+                assert s["mismatches"] == 0 and s["max_rel"] == 0.0, (fi, buf, s)
+    finally:
+        a.close()
+        b.close()
+
+
 def test_moved_object_rebuilds_the_candidate_grid(require_gpu):
     """An object jumps half a metre between frames 1 and 2.  Every frame
     re-uploads the objects; the fused schedule keeps its candidate grid while
