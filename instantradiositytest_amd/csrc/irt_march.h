@@ -35,8 +35,8 @@ struct RayReq {
 };
 static_assert(sizeof(RayReq) == 64, "ray request is 64 bytes");
 
-constexpr uint32_t MARCH_CHUNK = 256;  // request slots claimed per atomic
-constexpr uint32_t MARCH_REFILL = 8;   // idle lanes that trigger a refill
+constexpr uint32_t MARCH_CHUNK = 128;  // request slots claimed per atomic
+constexpr uint32_t MARCH_REFILL = 12;   // idle lanes that trigger a refill
 
 // The persistent march loop over one view's request slots (queue[1] = slots
 // taken).  Every loop iteration runs one scene query per busy lane.
