@@ -142,3 +142,76 @@ def test_trilinear_gradient_peaks_at_a_corner():
         centre = np.linalg.norm(trilinear_gradient(v, 0.5, 0.5, 0.5))
         centre_short += centre < dense * 0.99
     assert centre_short > 150
+
+
+def test_gap_ordered_build_gives_the_full_lists():
+    """k_build_cells evaluates objects in increasing box-gap order and stops
+    once the next gap exceeds the smallest upper bound so far.  That yields
+    the lists of evaluating every object because an object's distance at the
+    cell centre is never below its gap bound; both are checked here on cells
+    of the 1920x1200 scene at the grid's own size (160 cells on the longest
+    axis)."""
+    cs = ConfigScene(CONFIGS["plain-1920x1200"].scaled(64, 40, 1))
+    rec = oracle.SceneRecords.from_scene(cs.scene)
+    n = len(rec.objects)
+    one = np.zeros(1, FRAME_DT)
+    one["counts"][0] = (rec.n_lights, 1, 1, 1)
+    one["sdf"][0] = (96, 0.001, 16, 0)
+    leaf = np.zeros(1, NODE_DT)
+    leaf[0]["info"] = (0, 0, 1, 1)
+    singles = [oracle.SceneRecords(rec.objects[[j]].copy(), rec.meshes, leaf, rec.lights[:rec.n_lights], rec.atlas)
+               for j in range(n)]
+    boxes = {int(nd["info"][0]): (nd["lo"][:3].astype(np.float64), nd["hi"][:3].astype(np.float64))
+             for nd in rec.nodes if nd["info"][2] == 1}
+    K = []
+    for j in range(n):
+        mesh = rec.meshes[int(rec.objects[j]["info"][0])]
+        K.append(1.0 if mesh["geometry_lo"][3] > 0.5 else float(np.sqrt(mesh_gradient_bound(rec.atlas, mesh) + 1.0)))
+
+    def dist(s, p):
+        return L.oracle_scene_distance(32, one.ctypes.data, ctypes.byref(s.c),
+                                       np.ascontiguousarray(p, np.float64).ctypes.data, 0, 0xFFFFFFFF, None, None)
+
+    lo = rec.nodes[0]["lo"][:3].astype(np.float64) - 1e-3
+    hi = rec.nodes[0]["hi"][:3].astype(np.float64) + 1e-3
+    cell = (hi - lo).max() / 160
+    dims = np.ceil((hi - lo) / cell).astype(int)
+    half = 0.5 * cell * np.sqrt(3) * 1.0001
+    rng = np.random.default_rng(17)
+    skipped = 0
+    for c in rng.integers(0, dims, (300, 3)):
+        c0 = lo + c * cell
+        c1 = c0 + cell
+        centre = (c0 + c1) / 2
+        gap = []
+        for j in range(n):
+            blo, bhi = boxes[j]
+            g = np.linalg.norm(np.maximum(np.maximum(blo - c1, c0 - bhi), 0))
+            gap.append(0.9998 * g * (1 - 1e-6) if g > 0 else -np.inf)
+        dc = {}
+
+        def bounds(j):
+            if j not in dc:
+                dc[j] = dist(singles[j], centre)
+            lip = dc[j] - K[j] * 1.001 * half
+            lip -= abs(lip) * 1e-5 + 1e-6
+            up = dc[j] + K[j] * 1.001 * half
+            return max(gap[j], lip), up + abs(up) * 1e-5 + 1e-6
+
+        full = [bounds(j) for j in range(n)]
+        U = min(u for _, u in full)
+        want = sorted((j for j in range(n) if full[j][0] <= U), key=lambda j: full[j][0])
+        for j in range(n):  # the inequality the early stop rests on
+            assert dc[j] >= gap[j]
+        # the gap-ordered build with its early stop
+        order = sorted(range(n), key=lambda j: gap[j])
+        U2, seen = np.inf, []
+        for j in order:
+            if not gap[j] <= U2:
+                break
+            seen.append(j)
+            U2 = min(U2, bounds(j)[1])
+        got = sorted((j for j in sorted(seen) if bounds(j)[0] <= U2), key=lambda j: bounds(j)[0])
+        assert U2 == U and got == want
+        skipped += n - len(seen)
+    assert skipped > 300 * n // 2  # the early stop skips most objects
