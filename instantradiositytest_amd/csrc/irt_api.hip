@@ -223,6 +223,21 @@ namespace {
 // Bytes dev_alloc allocates for a request (never zero).
 size_t alloc_size(size_t bytes) { return bytes ? bytes : 256; }
 
+// Waits for all work of both of the context's streams.  A batch step that
+// used the context's buffers is included: after every step the context's
+// streams wait for the step's events.
+int sync_all(irt_ctx* c) {
+  HIP_CHECK(c, hipStreamSynchronize(c->svgf_stream));
+  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return IRT_OK;
+}
+
+// Allocates and zero-fills.  The fill has finished when this returns, so a
+// first consumer on any stream (the SVGF stream, a batch stream) sees zeros:
+// the fill is enqueued on the lighting stream, and nothing else would order
+// it before work on the other streams.  Allocation happens at set-up and when
+// a grow-only array grows, never per frame, so the wait costs nothing in
+// steady state.
 int dev_alloc(irt_ctx* c, void** p, size_t bytes) {
   bytes = alloc_size(bytes);
   hipError_t e = hipMalloc(p, bytes);
@@ -231,6 +246,8 @@ int dev_alloc(irt_ctx* c, void** p, size_t bytes) {
   c->device_bytes += bytes;
   e = hipMemsetAsync(*p, 0, bytes, c->stream);
   if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   return IRT_OK;
 }
 
@@ -252,8 +269,9 @@ int ensure_array(irt_ctx* c, T** p, uint32_t* cap, uint32_t count) {
   uint32_t need = count > 0 ? count : 1;
   if (*p && *cap >= need) return IRT_OK;
   if (*p) {
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    int rc = dev_free(c, *p);
+    int rc = sync_all(c);  // readers may be on either stream (presentation runs on the SVGF stream)
+    if (rc) return rc;
+    rc = dev_free(c, *p);
     if (rc) return rc;
     *p = nullptr;
   }
@@ -323,12 +341,6 @@ int end_frame_slots(irt_ctx* c) {
 int join(irt_ctx* c, hipStream_t from, hipStream_t to) {
   HIP_CHECK(c, hipEventRecord(c->ev_lit, from));
   HIP_CHECK(c, hipStreamWaitEvent(to, c->ev_lit, 0));
-  return IRT_OK;
-}
-
-int sync_all(irt_ctx* c) {
-  HIP_CHECK(c, hipStreamSynchronize(c->svgf_stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return IRT_OK;
 }
 
