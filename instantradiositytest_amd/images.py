@@ -5,8 +5,9 @@ the RGBA8 display target saved as an 8-bit RGBA PNG).  The analysis follows
 the reference's image_analysis_tools/render_debug, the checks that
 tools/verify.py:61-80 runs on captures:
 
-  - ``uv_rect``: the pixels whose centres ((x + 0.5) / W, (y + 0.5) / H) lie in
-    [u_min, u_max] x [v_min, v_max], bounds included (selection.py:149-164);
+  - ``select_uv_rect``: a mask of the pixels whose centres ((x + 0.5) / W,
+    (y + 0.5) / H) lie in [u_min, u_max] x [v_min, v_max], bounds included
+    (the rule of selection.py:149-164);
   - ``occupancy``: how many selected pixels satisfy a predicate over the
     normalised channels R, G, B, A (0..1), written with comparisons, AND, OR
     and parentheses, as in ``R < 0.03 AND G < 0.03 AND B < 0.03``
@@ -27,7 +28,6 @@ from __future__ import annotations
 import argparse
 import json
 import math
-import re
 import struct
 import sys
 import zlib
@@ -132,138 +132,206 @@ def read_png(path) -> np.ndarray:
 
 
 # ----------------------------------------------------------------- selection
-def uv_rect(width: int, height: int, u_min: float, v_min: float, u_max: float, v_max: float):
-    """Pixel bounds (x0, y0, x1, y1), exclusive ends, of the pixels whose centres
-    lie in the rectangle (bounds included); None when no pixel does."""
-    for v in (u_min, v_min, u_max, v_max):
-        if not (math.isfinite(v) and 0.0 <= v <= 1.0):
-            raise ValueError("uv coordinates must lie in [0, 1]")
-    if u_min > u_max or v_min > v_max:
-        raise ValueError("uv-rect minimums must not exceed maximums")
-    if u_min == u_max or v_min == v_max:
-        return None
-    xs = [x for x in range(width) if u_min <= (x + 0.5) / width <= u_max]
-    ys = [y for y in range(height) if v_min <= (y + 0.5) / height <= v_max]
-    if not xs or not ys:
-        return None
-    return xs[0], ys[0], xs[-1] + 1, ys[-1] + 1
+# A selection is a boolean mask over the image's pixels, [H, W].  The uv
+# rectangle keeps the pixels whose centres ((x + 0.5) / W, (y + 0.5) / H)
+# lie inside it, edges included; the mask is the outer product of a column
+# test and a row test on those centres.
+
+def select_all(width: int, height: int) -> np.ndarray:
+    return np.ones((height, width), bool)
 
 
-def region(image: np.ndarray, rect) -> np.ndarray:
-    """Normalised float64 RGBA of the selected pixels, [N, 4]."""
-    if rect is None:
-        return np.zeros((0, 4))
-    x0, y0, x1, y1 = rect
-    return image[y0:y1, x0:x1].reshape(-1, 4).astype(np.float64) / 255.0
+def select_uv_rect(width: int, height: int, u_min: float, v_min: float, u_max: float, v_max: float) -> np.ndarray:
+    """Mask of the pixels whose centres lie in [u_min, u_max] x [v_min, v_max]."""
+    box = np.array([u_min, v_min, u_max, v_max], np.float64)
+    if not np.all(np.isfinite(box)) or box.min() < 0.0 or box.max() > 1.0:
+        raise ValueError(f"uv-rect corners must be finite numbers in [0, 1], got {box.tolist()}")
+    if box[2] < box[0] or box[3] < box[1]:
+        raise ValueError(f"uv-rect is inverted: ({u_min}, {v_min}) lies beyond ({u_max}, {v_max})")
+    cu = (np.arange(width) + 0.5) / width
+    cv = (np.arange(height) + 0.5) / height
+    cols = (cu >= box[0]) & (cu <= box[2])
+    rows = (cv >= box[1]) & (cv <= box[3])
+    return rows[:, None] & cols[None, :]
+
+
+def selected(image: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """Normalised float64 RGBA of the selected pixels in row-major order, [N, 4]."""
+    if mask.shape != image.shape[:2]:
+        raise ValueError(f"selection of shape {mask.shape} does not fit a {image.shape[1]}x{image.shape[0]} image")
+    return image[mask].astype(np.float64) / 255.0
 
 
 # ----------------------------------------------------------------- predicate
-_TOKEN = re.compile(r"\s*(>=|<=|==|!=|>|<|\(|\)|AND\b|OR\b|[RGBA]\b|[+-]?(?:\d+(?:\.\d*)?|\.\d+)(?:[eE][+-]?\d+)?)",
-                    re.IGNORECASE)
-_OPS = {"<": np.less, "<=": np.less_equal, ">": np.greater, ">=": np.greater_equal, "==": np.equal,
-        "!=": np.not_equal}
+# Grammar: a condition is comparisons "<channel> <op> <number>" (or
+# "<number> <op> <channel>") over the normalised channels R, G, B, A, joined
+# by AND and OR, with AND binding tighter, and parentheses.  Channel names
+# and the words AND / OR are case-insensitive.
+#
+# The scanner below classifies characters and emits typed tokens; the
+# compiler turns them into reverse Polish notation with Dijkstra's
+# shunting-yard method, where a comparison is one operand; evaluation runs
+# the RPN with a stack of boolean arrays.
+
+_CHANNELS = "RGBA"
+_COMPARE = {"<": np.less, "<=": np.less_equal, ">": np.greater, ">=": np.greater_equal, "==": np.equal,
+            "!=": np.not_equal}
+_MIRROR = {"<": ">", "<=": ">=", ">": "<", ">=": "<=", "==": "==", "!=": "!="}
+_PRECEDENCE = {"OR": 1, "AND": 2}
 
 
-def _tokens(expression: str) -> list[str]:
-    out, pos = [], 0
-    while pos < len(expression):
-        if not expression[pos:].strip():
-            break
-        m = _TOKEN.match(expression, pos)
-        if not m:
-            raise ValueError(f"invalid token in predicate at {pos}: {expression!r}")
-        out.append(m.group(1).upper())
-        pos = m.end()
-    if not out:
-        raise ValueError("predicate must not be empty")
+def _scan(text: str) -> list[tuple[str, object, int]]:
+    """Tokens (kind, value, column): kinds 'chan', 'cmp', 'num', 'bool', '(' and ')'."""
+    toks: list[tuple[str, object, int]] = []
+    i, n = 0, len(text)
+    while i < n:
+        c = text[i]
+        if c.isspace():
+            i += 1
+        elif c in "()":
+            toks.append((c, c, i))
+            i += 1
+        elif c in "<>=!":
+            two = text[i:i + 2]
+            if two in _COMPARE:
+                toks.append(("cmp", two, i))
+                i += 2
+            elif c in "<>":
+                toks.append(("cmp", c, i))
+                i += 1
+            else:
+                raise ValueError(f"column {i}: '{c}' must be followed by '=' in {text!r}")
+        elif c.isdigit() or c in "+-.":
+            j = i + 1 if c in "+-" else i
+            while j < n and (text[j].isdigit() or text[j] == "."):
+                j += 1
+            if j < n and text[j] in "eE":
+                k = j + 1
+                if k < n and text[k] in "+-":
+                    k += 1
+                if k < n and text[k].isdigit():
+                    j = k
+                    while j < n and text[j].isdigit():
+                        j += 1
+            try:
+                value = float(text[i:j])
+            except ValueError:
+                raise ValueError(f"column {i}: malformed number {text[i:j]!r} in {text!r}") from None
+            toks.append(("num", value, i))
+            i = j
+        elif c.isalpha():
+            j = i
+            while j < n and (text[j].isalnum() or text[j] == "_"):
+                j += 1
+            word = text[i:j].upper()
+            if word in _PRECEDENCE:
+                toks.append(("bool", word, i))
+            elif len(word) == 1 and word in _CHANNELS:
+                toks.append(("chan", _CHANNELS.index(word), i))
+            else:
+                raise ValueError(f"column {i}: unknown word {text[i:j]!r} (channels are R, G, B, A) in {text!r}")
+            i = j
+        else:
+            raise ValueError(f"column {i}: unexpected character {c!r} in {text!r}")
+    return toks
+
+
+def _compile(text: str) -> list[tuple]:
+    """RPN program: ('test', channel, op, value) operands and ('AND',) / ('OR',) operators."""
+    toks = _scan(text)
+    if not toks:
+        raise ValueError("empty condition")
+    out: list[tuple] = []
+    ops: list[tuple[str, int]] = []  # pending 'AND' / 'OR' / '(' with columns
+    expect_operand = True
+    i = 0
+    while i < len(toks):
+        kind, value, col = toks[i]
+        if expect_operand:
+            if kind == "(":
+                ops.append(("(", col))
+                i += 1
+                continue
+            atom = toks[i:i + 3]
+            kinds = [t[0] for t in atom]
+            if kinds == ["chan", "cmp", "num"]:
+                out.append(("test", atom[0][1], atom[1][1], atom[2][1]))
+            elif kinds == ["num", "cmp", "chan"]:
+                out.append(("test", atom[2][1], _MIRROR[atom[1][1]], atom[0][1]))
+            else:
+                raise ValueError(f"column {col}: expected a comparison like 'R < 0.5' in {text!r}")
+            i += 3
+            expect_operand = False
+        else:
+            if kind == "bool":
+                while ops and ops[-1][0] != "(" and _PRECEDENCE[ops[-1][0]] >= _PRECEDENCE[value]:
+                    out.append((ops.pop()[0],))
+                ops.append((value, col))
+                expect_operand = True
+            elif kind == ")":
+                while ops and ops[-1][0] != "(":
+                    out.append((ops.pop()[0],))
+                if not ops:
+                    raise ValueError(f"column {col}: ')' without a matching '(' in {text!r}")
+                ops.pop()
+            else:
+                raise ValueError(f"column {col}: expected AND, OR or ')' in {text!r}")
+            i += 1
+    if expect_operand:
+        raise ValueError(f"condition ends where a comparison is expected: {text!r}")
+    while ops:
+        op, col = ops.pop()
+        if op == "(":
+            raise ValueError(f"column {col}: '(' is never closed in {text!r}")
+        out.append((op,))
     return out
 
 
-def predicate_mask(expression: str, px: np.ndarray) -> np.ndarray:
-    """Boolean mask over the [N, 4] normalised pixels: comparisons of a channel
-    with a number, AND binding tighter than OR, parentheses."""
-    toks = _tokens(expression)
-    pos = 0
-    env = {"R": px[:, 0], "G": px[:, 1], "B": px[:, 2], "A": px[:, 3]}
-
-    def take():
-        nonlocal pos
-        if pos >= len(toks):
-            raise ValueError("unexpected end of predicate")
-        pos += 1
-        return toks[pos - 1]
-
-    def peek():
-        return toks[pos] if pos < len(toks) else ""
-
-    def parse_or():
-        node = parse_and()
-        while peek() == "OR":
-            take()
-            node = node | parse_and()
-        return node
-
-    def parse_and():
-        node = parse_term()
-        while peek() == "AND":
-            take()
-            node = node & parse_term()
-        return node
-
-    def parse_term():
-        if peek() == "(":
-            take()
-            node = parse_or()
-            if take() != ")":
-                raise ValueError("expected ')'")
-            return node
-        ch = take()
-        if ch not in env:
-            raise ValueError(f"expected a channel, got {ch!r}")
-        op = take()
-        if op not in _OPS:
-            raise ValueError(f"expected a comparison, got {op!r}")
-        try:
-            value = float(take())
-        except ValueError as e:
-            raise ValueError("expected a number") from e
-        return _OPS[op](env[ch], value)
-
-    node = parse_or()
-    if peek():
-        raise ValueError(f"unexpected predicate token {peek()!r}")
-    return np.asarray(node, bool)
+def predicate_mask(condition: str, px: np.ndarray) -> np.ndarray:
+    """Boolean mask over [N, 4] normalised pixels for the condition."""
+    stack: list[np.ndarray] = []
+    for instr in _compile(condition):
+        if instr[0] == "test":
+            _, ch, op, value = instr
+            stack.append(_COMPARE[op](px[:, ch], value))
+        else:
+            b, a = stack.pop(), stack.pop()
+            stack.append(a & b if instr[0] == "AND" else a | b)
+    (mask,) = stack
+    return np.asarray(mask, bool)
 
 
 # ---------------------------------------------------------------- evaluations
-def occupancy(image: np.ndarray, rect, condition: str) -> dict:
-    px = region(image, rect)
-    count = int(predicate_mask(condition, px).sum()) if len(px) else 0
-    total = len(px)
-    fraction = count / total if total else 0.0
-    return {"condition": condition, "count": count, "total": total, "fraction": fraction,
-            "percentage": fraction * 100.0}
+def occupancy(image: np.ndarray, mask: np.ndarray, condition: str) -> dict:
+    """How many selected pixels satisfy the condition."""
+    px = selected(image, mask)
+    hits = int(np.count_nonzero(predicate_mask(condition, px)))
+    total = int(px.shape[0])
+    share = hits / total if total else 0.0
+    return {"condition": condition, "count": hits, "total": total, "fraction": share, "percentage": 100.0 * share}
 
 
-def match(reference: np.ndarray, actual: np.ndarray, rect, tolerance: float) -> dict:
+def match(reference: np.ndarray, actual: np.ndarray, mask: np.ndarray, tolerance: float) -> dict:
+    """Largest absolute difference of the selected normalised channel values against a tolerance."""
     if reference.shape != actual.shape:
-        raise ValueError("images must have identical dimensions")
-    if not (math.isfinite(tolerance) and 0.0 <= tolerance <= 1.0):
-        raise ValueError("tolerance must be finite and within [0, 1]")
-    diff = np.abs(region(reference, rect) - region(actual, rect))
-    maximum = float(diff.max()) if diff.size else 0.0
-    return {"match": maximum <= tolerance, "tolerance": tolerance, "max_error": maximum,
-            "metric": "maximum absolute error over corresponding normalized channel samples"}
+        raise ValueError(f"image sizes differ: {reference.shape} and {actual.shape}")
+    if not (0.0 <= tolerance <= 1.0):  # also refuses NaN
+        raise ValueError(f"tolerance {tolerance} is not a number in [0, 1]")
+    a, b = selected(reference, mask), selected(actual, mask)
+    worst = float(np.max(np.abs(a - b))) if a.size else 0.0
+    return {"match": worst <= tolerance, "tolerance": tolerance, "max_error": worst,
+            "metric": "largest |reference - actual| over the selected RGBA values, each scaled to [0, 1]"}
 
 
-def psnr(reference: np.ndarray, actual: np.ndarray, rect=None) -> dict:
+def psnr(reference: np.ndarray, actual: np.ndarray, mask: np.ndarray | None = None) -> dict:
     """PSNR of the normalised RGB channels (peak 1); infinite for identical images."""
     if reference.shape != actual.shape:
-        raise ValueError("images must have identical dimensions")
-    if rect is None:
-        rect = (0, 0, reference.shape[1], reference.shape[0])
-    mse = float(np.mean((region(reference, rect)[:, :3] - region(actual, rect)[:, :3]) ** 2))
-    return {"psnr_db": math.inf if mse == 0 else 10.0 * math.log10(1.0 / mse), "mse": mse}
+        raise ValueError(f"image sizes differ: {reference.shape} and {actual.shape}")
+    if mask is None:
+        mask = select_all(reference.shape[1], reference.shape[0])
+    err = selected(reference, mask)[:, :3] - selected(actual, mask)[:, :3]
+    mse = float(np.mean(err * err))
+    return {"psnr_db": math.inf if mse == 0 else -10.0 * math.log10(mse), "mse": mse}
 
 
 def main(argv=None) -> int:
@@ -277,9 +345,9 @@ def main(argv=None) -> int:
     images = [read_png(p) for p in args.images]
     h, w, _ = images[0].shape
     if args.select[0] == "all":
-        rect = (0, 0, w, h)
+        rect = select_all(w, h)
     elif args.select[0] == "uv-rect" and len(args.select) == 5:
-        rect = uv_rect(w, h, *(float(v) for v in args.select[1:]))
+        rect = select_uv_rect(w, h, *(float(v) for v in args.select[1:]))
     else:
         ap.error("--select all | uv-rect u_min v_min u_max v_max")
     if args.evaluate == "occupancy":

@@ -57,7 +57,7 @@ def test_repeated_renders_capture_identically(require_gpu, tmp_path):
         finally:
             r.close()
     a, b = (images.read_png(p) for p in paths)
-    assert images.match(a, b, (0, 0, a.shape[1], a.shape[0]), 0.0)["match"]
+    assert images.match(a, b, images.select_all(a.shape[1], a.shape[0]), 0.0)["match"]
 
 
 def test_present_device_equals_present(require_gpu):

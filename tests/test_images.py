@@ -4,8 +4,9 @@ and the reference analysis tool's selection, predicate and metrics.
 Sources of truth: Pillow reads and writes the PNGs independently (every
 filter type it chooses); the sRGB encode is checked against the textbook
 IEC 61966-2-1 formula evaluated in float64 for every binary16 value in
-[0, 1]; selection and predicates against hand-worked cases following
-image_analysis_tools/render_debug (selection.py:149-164, predicate.py);
+[0, 1]; selection and predicates against hand-worked cases and, for predicates,
+against Python's own boolean evaluation of random conditions (the rules of
+image_analysis_tools/render_debug: selection.py:149-164, predicate.py);
 PSNR and match against closed forms.
 """
 from __future__ import annotations
@@ -78,14 +79,22 @@ def test_oracle_present_encodes_the_history():
 
 
 def test_uv_rect_selects_pixel_centres_inside():
+    def cols_rows(mask):
+        ys, xs = np.nonzero(mask)
+        return sorted(set(xs.tolist())), sorted(set(ys.tolist())), int(mask.sum())
+
     # centres (x + 0.5) / 10: 0.05, 0.15, 0.25, 0.35 ... -> [0.1, 0.3] holds x = 1, 2
-    assert images.uv_rect(10, 4, 0.1, 0.0, 0.3, 1.0) == (1, 0, 3, 4)
-    # bounds included: 0.15 and 0.25 on the edges
-    assert images.uv_rect(10, 4, 0.15, 0.125, 0.25, 0.375) == (1, 0, 3, 2)
-    assert images.uv_rect(10, 4, 0.2, 0.2, 0.2, 0.9) is None
-    assert images.uv_rect(10, 4, 0.96, 0.0, 1.0, 1.0) is None
-    with pytest.raises(ValueError):
-        images.uv_rect(10, 4, 0.5, 0.0, 0.4, 1.0)
+    assert cols_rows(images.select_uv_rect(10, 4, 0.1, 0.0, 0.3, 1.0)) == ([1, 2], [0, 1, 2, 3], 8)
+    # bounds included: 0.15 and 0.25 on the edges; rows: centres 0.125, 0.375 on the edges
+    assert cols_rows(images.select_uv_rect(10, 4, 0.15, 0.125, 0.25, 0.375)) == ([1, 2], [0, 1], 4)
+    # degenerate rectangles between centres select nothing
+    assert not images.select_uv_rect(10, 4, 0.2, 0.2, 0.2, 0.9).any()
+    assert not images.select_uv_rect(10, 4, 0.96, 0.0, 1.0, 1.0).any()
+    assert images.select_uv_rect(7, 5, 0.0, 0.0, 1.0, 1.0).all()
+    for bad in ((0.5, 0.0, 0.4, 1.0), (0.0, 0.6, 1.0, 0.5), (-0.1, 0.0, 0.5, 0.5), (0.0, 0.0, 1.5, 1.0),
+                (float("nan"), 0.0, 1.0, 1.0)):
+        with pytest.raises(ValueError):
+            images.select_uv_rect(10, 4, *bad)
 
 
 def test_predicate_grammar_and_occupancy():
@@ -95,14 +104,47 @@ def test_predicate_grammar_and_occupancy():
     # AND binds tighter than OR; parentheses override
     assert images.predicate_mask("R > 0.4 OR G < 0.015 AND A < 1", px).tolist() == [False, True, True]
     assert images.predicate_mask("(R > 0.4 OR G < 0.015) AND A < 1", px).tolist() == [False, False, True]
-    for bad in ("", "R <", "X < 1", "R < 1 AND", "(R < 1"):
+    # a number on the left mirrors the comparison; exponents and signs parse
+    assert images.predicate_mask("0.4 < R", px).tolist() == [False, True, False]
+    assert images.predicate_mask("B != 0 AND B > -1e-3 AND B <= 9E-1", px).tolist() == [True, False, True]
+    assert images.predicate_mask("((A == 1))", px).tolist() == [True, True, False]
+    for bad in ("", "  ", "R <", "X < 1", "R < 1 AND", "(R < 1", "R < 1)", "R = 1", "R < 1 R < 2",
+                "AND R < 1", "R < 1 ()", "R < 1.2.3", "R < $"):
         with pytest.raises(ValueError):
             images.predicate_mask(bad, px)
     img = np.zeros((4, 10, 4), np.uint8)
     img[:, :, 3] = 255
     img[1:3, 2:5, 0] = 200
-    res = images.occupancy(img, images.uv_rect(10, 4, 0.0, 0.0, 1.0, 1.0), "R < 0.03 AND G < 0.03 AND B < 0.03")
+    res = images.occupancy(img, images.select_all(10, 4), "R < 0.03 AND G < 0.03 AND B < 0.03")
     assert res["count"] == 40 - 6 and res["total"] == 40
+    assert res["fraction"] == 34 / 40 and res["percentage"] == 85.0
+
+
+def test_predicate_agrees_with_python_boolean_evaluation():
+    # random conditions over random pixels: the compiled program equals the
+    # same condition evaluated by Python's own and/or precedence per pixel
+    rng = np.random.default_rng(4)
+    px = rng.uniform(0, 1, (64, 4))
+    ops = ["<", "<=", ">", ">=", "==", "!="]
+
+    def term(depth):
+        if depth > 0 and rng.uniform() < 0.3:
+            return "(" + expr(depth - 1) + ")"
+        return f"{'RGBA'[rng.integers(4)]} {ops[rng.integers(6)]} {rng.uniform():.3f}"
+
+    def expr(depth):
+        parts = [term(depth)]
+        for _ in range(rng.integers(0, 3)):
+            parts += [["AND", "OR"][rng.integers(2)], term(depth)]
+        return " ".join(parts)
+
+    for _ in range(200):
+        cond = expr(2)
+        py = cond.replace("AND", "and").replace("OR", "or")
+        for c, k in zip("RGBA", range(4)):
+            py = py.replace(c, f"p[{k}]")
+        expect = [bool(eval(py, {}, {"p": p})) for p in px]
+        assert images.predicate_mask(cond, px).tolist() == expect, cond
 
 
 def test_match_and_psnr_closed_forms():
@@ -110,13 +152,15 @@ def test_match_and_psnr_closed_forms():
     a = rng.integers(1, 255, (20, 30, 4), dtype=np.uint8)
     b = a.copy()
     b[..., :3] += 1
-    full = (0, 0, 30, 20)
+    full = images.select_all(30, 20)
     assert images.match(a, a, full, 0.0)["match"]
     m = images.match(a, b, full, 0.0)
     assert not m["match"] and abs(m["max_error"] - 1 / 255) < 1e-12
     assert images.match(a, b, full, 1.5 / 255)["match"]
     assert abs(images.psnr(a, b)["psnr_db"] - 20 * math.log10(255)) < 1e-9
     assert images.psnr(a, a)["psnr_db"] == math.inf
+    with pytest.raises(ValueError):
+        images.match(a, b, full, float("nan"))
 
 
 def test_cli_reports_json(tmp_path, capsys):
