@@ -575,6 +575,27 @@ int irt_bake_sdf(const float* positions, uint32_t stride_floats, uint32_t vertex
                  uint32_t index_count, uint32_t resolution, float* samples, float bounds_lo[3], float bounds_hi[3],
                  float* lipschitz);
 
+/* irt_bake_sdf without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (a serving loop that
+ * prepares a session's scene at admission, code under HIP graph capture on
+ * another stream, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as irt_bake_sdf, bit for bit; it calls no hipMalloc/hipFree.
+ * Its scratch (hierarchy, sample grid, flag and reduction words) is one
+ * fixed array of IRT_BAKE_NOALLOC_SCRATCH_BYTES in the library's code
+ * object, which every device the library runs on holds until the process
+ * ends.  Limits: resolution <= IRT_BAKE_NOALLOC_MAX_RESOLUTION and at most
+ * IRT_BAKE_NOALLOC_MAX_TRIANGLES triangles; larger meshes return
+ * IRT_ERR_INVALID_ARGUMENT ("... exceeds its fixed scratch").  Calls on one
+ * device share the array and are serialised by a lock inside the library;
+ * the array keeps the last call's contents, and the words that must start
+ * from fixed values are set by every call. */
+#define IRT_BAKE_NOALLOC_MAX_RESOLUTION 128u
+#define IRT_BAKE_NOALLOC_MAX_TRIANGLES 16384u
+#define IRT_BAKE_NOALLOC_SCRATCH_BYTES 10223872u /* 256 + 1 MiB nodes + 768 KiB triangles + 8 MiB samples */
+int irt_bake_sdf_noalloc(const float* positions, uint32_t stride_floats, uint32_t vertex_count,
+                         const uint32_t* indices, uint32_t index_count, uint32_t resolution, float* samples,
+                         float bounds_lo[3], float bounds_hi[3], float* lipschitz);
+
 /* Decision thresholds of the display's sRGB encode: a channel value v
  * (clamped to [0, 1]) gets the code #{k : out[k] <= v}.  out[k] is the
  * linear value where the exact encode times 255 reaches k + 0.5, rounded up

@@ -65,6 +65,7 @@ def irt() -> ctypes.CDLL:
         "irt_present_device": (c_int, [P, P]),
         "irt_host_srgb_thresholds": (None, [P]),
         "irt_bake_sdf": (c_int, [P, u32, u32, P, u32, u32, P, P, P, POINTER(c_float)]),
+        "irt_bake_sdf_noalloc": (c_int, [P, u32, u32, P, u32, u32, P, P, P, POINTER(c_float)]),
         "irt_lighting": (c_int, [P]),
         "irt_svgf": (c_int, [P, u32, u32]),
         "irt_end_frame": (c_int, [P]),

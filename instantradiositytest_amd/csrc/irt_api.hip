@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -944,6 +945,164 @@ int ingest(irt_ctx* c, const float* depth, const uint16_t* n, const uint16_t* co
 }  // namespace
 
 // ====================================================================== ABI
+namespace {
+
+// ---------------------------------------------------------------- SDF bake
+// irt_bake_sdf and irt_bake_sdf_noalloc share everything but where the
+// device scratch comes from: the hierarchy nodes and triangles, the sample
+// grid the kernels write before it is copied out, and two words (the
+// parity-crossing overflow flag and the bits of the Lipschitz reduction's
+// squared maximum).
+
+struct BakePlan {
+  std::vector<irt_tri_node> nodes;
+  std::vector<irt_tri> tris;
+  BakeGrid g{};
+  uint64_t samples = 0;
+  float sp[3] = {0, 0, 0};
+  float threshold = 0;
+};
+
+int bake_fail(int code, const std::string& m) {
+  irt_host::set_global_error(m);
+  return code;
+}
+
+// Validation, the host hierarchy and the grid of export_scene.py:135-158.
+int bake_prepare(const float* positions, uint32_t stride_floats, uint32_t vertex_count, const uint32_t* indices,
+                 uint32_t index_count, uint32_t resolution, const float* samples, float bounds_lo[3],
+                 float bounds_hi[3], const float* lipschitz, BakePlan& plan) {
+  if (!positions || !indices || !samples || !bounds_lo || !bounds_hi || !lipschitz || stride_floats < 3)
+    return bake_fail(IRT_ERR_INVALID_ARGUMENT, "null argument or position stride below 3");
+  if (resolution < 4 || resolution > 256) return bake_fail(IRT_ERR_INVALID_ARGUMENT, "SDF resolution must be in [4,256]");
+  if (index_count == 0 || vertex_count == 0) return bake_fail(IRT_ERR_INVALID_ARGUMENT, "mesh has no triangles");
+  std::vector<irt_vertex> verts(vertex_count);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t v = 0; v < vertex_count; ++v) {
+    std::memset(&verts[v], 0, sizeof(irt_vertex));
+    for (int a = 0; a < 3; ++a) {
+      const float x = positions[(size_t)v * stride_floats + a];
+      verts[v].position[a] = x;
+      lo[a] = std::min(lo[a], x);
+      hi[a] = std::max(hi[a], x);
+    }
+    verts[v].position[3] = 1.0f;
+  }
+  std::string msg;
+  int rc = irt_host::mesh_hierarchy(verts.data(), vertex_count, indices, index_count, plan.nodes, plan.tris, &msg);
+  if (rc) return bake_fail(rc, msg);
+  // export_scene.py:136-139: padding = |hi - lo| / (res - 1) * 2 on every axis
+  const float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  const double padding = std::sqrt((double)e[0] * e[0] + (double)e[1] * e[1] + (double)e[2] * e[2]) /
+                         (double)(resolution - 1) * 2.0;
+  const float pf = (float)padding;
+  BakeGrid& g = plan.g;
+  for (int a = 0; a < 3; ++a) {
+    bounds_lo[a] = lo[a] - pf;
+    bounds_hi[a] = hi[a] + pf;
+    g.lo[a] = bounds_lo[a];
+    g.hi[a] = bounds_hi[a];
+  }
+  const float pe[3] = {bounds_hi[0] - bounds_lo[0], bounds_hi[1] - bounds_lo[1], bounds_hi[2] - bounds_lo[2]};
+  g.extent = std::sqrt((double)pe[0] * pe[0] + (double)pe[1] * pe[1] + (double)pe[2] * pe[2]);
+  g.res = resolution;
+  g.n_nodes = (uint32_t)plan.nodes.size();
+  plan.samples = (uint64_t)resolution * resolution * resolution;
+  // spacing and threshold of field_lipschitz_bound, in float as the host computes them
+  for (int a = 0; a < 3; ++a) plan.sp[a] = (bounds_hi[a] - bounds_lo[a]) / (float)(resolution - 1);
+  const float sl = std::sqrt(plan.sp[0] * plan.sp[0] + plan.sp[1] * plan.sp[1] + plan.sp[2] * plan.sp[2]);
+  plan.threshold = sl * (0.5f + 1.0f / 128.0f);
+  return IRT_OK;
+}
+
+// Uploads the hierarchy, clears the two words, runs the bake and the
+// Lipschitz reduction on the null stream and copies the samples and words
+// back.  Every sample of the grid is written by k_bake_sdf, so the sample
+// buffer needs no clearing.
+hipError_t bake_run(const BakePlan& plan, irt_tri_node* d_nodes, irt_tri* d_tris, float* d_out, uint32_t* d_flags,
+                    float* samples, uint32_t flags[2]) {
+  const uint32_t init[2] = {0u, 0x3f800000u};  // overflow flag, bits of the squared maximum (starts at 1)
+  hipError_t e1 = hipMemcpy(d_nodes, plan.nodes.data(), sizeof(irt_tri_node) * plan.nodes.size(), hipMemcpyHostToDevice);
+  if (e1 == hipSuccess) e1 = hipMemcpy(d_tris, plan.tris.data(), sizeof(irt_tri) * plan.tris.size(), hipMemcpyHostToDevice);
+  if (e1 == hipSuccess) e1 = hipMemcpy(d_flags, init, sizeof(init), hipMemcpyHostToDevice);
+  const uint64_t n = plan.samples;
+  if (e1 == hipSuccess) {
+    hipLaunchKernelGGL(k_bake_sdf, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, nullptr, d_nodes, d_tris, plan.g,
+                       d_out, d_flags);
+    e1 = hipGetLastError();
+  }
+  const uint64_t r1 = plan.g.res - 1, cells = r1 * r1 * r1;
+  if (e1 == hipSuccess) {
+    const uint32_t lip_blocks = (uint32_t)std::min<uint64_t>((cells + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_lipschitz, dim3(lip_blocks), dim3(256), 0, nullptr, (const float*)d_out, plan.g.res,
+                       plan.sp[0], plan.sp[1], plan.sp[2], plan.threshold, d_flags + 1);
+    e1 = hipGetLastError();
+  }
+  if (e1 == hipSuccess) e1 = hipMemcpy(samples, d_out, sizeof(float) * n, hipMemcpyDeviceToHost);
+  if (e1 == hipSuccess) e1 = hipMemcpy(flags, d_flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  return e1;
+}
+
+int bake_finish(hipError_t e1, const uint32_t flags[2], float* lipschitz) {
+  if (e1 != hipSuccess) return bake_fail(IRT_ERR_DEVICE, std::string("SDF bake: ") + hipGetErrorString(e1));
+  if (flags[0]) return bake_fail(IRT_ERR_INVALID_ARGUMENT, "SDF classification exceeded 256 intersections");
+  float max_sq;
+  std::memcpy(&max_sq, &flags[1], 4);
+  *lipschitz = std::sqrt(max_sq) * 1.0001f;
+  return IRT_OK;
+}
+
+// Fixed scratch of irt_bake_sdf_noalloc, laid out as
+// [flags 256 B | nodes | triangles | samples], each part 256-byte aligned.
+// A mesh of T triangles has at most 2T - 1 hierarchy nodes.
+constexpr size_t bake_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t BAKE_MAX_NODES = 2 * (size_t)IRT_BAKE_NOALLOC_MAX_TRIANGLES;
+constexpr size_t BAKE_OFF_FLAGS = 0;
+constexpr size_t BAKE_OFF_NODES = 256;
+constexpr size_t BAKE_OFF_TRIS = BAKE_OFF_NODES + bake_align(sizeof(irt_tri_node) * BAKE_MAX_NODES);
+constexpr size_t BAKE_OFF_SAMPLES = BAKE_OFF_TRIS + bake_align(sizeof(irt_tri) * IRT_BAKE_NOALLOC_MAX_TRIANGLES);
+constexpr size_t BAKE_SCRATCH_BYTES =
+    BAKE_OFF_SAMPLES + bake_align(sizeof(float) * (size_t)IRT_BAKE_NOALLOC_MAX_RESOLUTION *
+                                  IRT_BAKE_NOALLOC_MAX_RESOLUTION * IRT_BAKE_NOALLOC_MAX_RESOLUTION);
+static_assert(BAKE_SCRATCH_BYTES == IRT_BAKE_NOALLOC_SCRATCH_BYTES, "irt.h states the scratch size");
+
+constexpr int BAKE_MAX_DEVICES = 64;
+std::mutex bake_scratch_lock[BAKE_MAX_DEVICES];
+std::mutex bake_symbol_lock;
+unsigned char* bake_scratch_ptr[BAKE_MAX_DEVICES] = {};
+
+}  // namespace
+
+// The array lives in the library's code object; every device the library
+// runs on holds its own copy until the process ends.  Host code never uses
+// this name as an address: the device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[BAKE_SCRATCH_BYTES];
+
+namespace {
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device with the device current.
+int bake_scratch(unsigned char** base, int* device) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return bake_fail(IRT_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= BAKE_MAX_DEVICES) return bake_fail(IRT_ERR_DEVICE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(bake_symbol_lock);
+  if (!bake_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return bake_fail(IRT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    bake_scratch_ptr[dev] = (unsigned char*)ptr;
+  }
+  *base = bake_scratch_ptr[dev];
+  *device = dev;
+  return IRT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 uint32_t irt_abi_version(void) { return IRT_ABI_VERSION; }
@@ -1712,85 +1871,47 @@ int irt_present(irt_ctx* c, uint8_t* rgba8) {
 int irt_bake_sdf(const float* positions, uint32_t stride_floats, uint32_t vertex_count, const uint32_t* indices,
                  uint32_t index_count, uint32_t resolution, float* samples, float bounds_lo[3], float bounds_hi[3],
                  float* lipschitz) {
-  auto fail = [](int code, const std::string& m) {
-    irt_host::set_global_error(m);
-    return code;
-  };
-  if (!positions || !indices || !samples || !bounds_lo || !bounds_hi || !lipschitz || stride_floats < 3)
-    return fail(IRT_ERR_INVALID_ARGUMENT, "null argument or position stride below 3");
-  if (resolution < 4 || resolution > 256) return fail(IRT_ERR_INVALID_ARGUMENT, "SDF resolution must be in [4,256]");
-  if (index_count == 0 || vertex_count == 0) return fail(IRT_ERR_INVALID_ARGUMENT, "mesh has no triangles");
-  std::vector<irt_vertex> verts(vertex_count);
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t v = 0; v < vertex_count; ++v) {
-    std::memset(&verts[v], 0, sizeof(irt_vertex));
-    for (int a = 0; a < 3; ++a) {
-      const float x = positions[(size_t)v * stride_floats + a];
-      verts[v].position[a] = x;
-      lo[a] = std::min(lo[a], x);
-      hi[a] = std::max(hi[a], x);
-    }
-    verts[v].position[3] = 1.0f;
-  }
-  std::vector<irt_tri_node> nodes;
-  std::vector<irt_tri> tris;
-  std::string msg;
-  int rc = irt_host::mesh_hierarchy(verts.data(), vertex_count, indices, index_count, nodes, tris, &msg);
-  if (rc) return fail(rc, msg);
-  // export_scene.py:136-139: padding = |hi - lo| / (res - 1) * 2 on every axis
-  const float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-  const double padding = std::sqrt((double)e[0] * e[0] + (double)e[1] * e[1] + (double)e[2] * e[2]) /
-                         (double)(resolution - 1) * 2.0;
-  const float pf = (float)padding;
-  BakeGrid g{};
-  for (int a = 0; a < 3; ++a) {
-    bounds_lo[a] = lo[a] - pf;
-    bounds_hi[a] = hi[a] + pf;
-    g.lo[a] = bounds_lo[a];
-    g.hi[a] = bounds_hi[a];
-  }
-  const float pe[3] = {bounds_hi[0] - bounds_lo[0], bounds_hi[1] - bounds_lo[1], bounds_hi[2] - bounds_lo[2]};
-  g.extent = std::sqrt((double)pe[0] * pe[0] + (double)pe[1] * pe[1] + (double)pe[2] * pe[2]);
-  g.res = resolution;
-  g.n_nodes = (uint32_t)nodes.size();
-  const uint64_t n = (uint64_t)resolution * resolution * resolution;
-  // spacing and threshold of field_lipschitz_bound, in float as the host computes them
-  float sp[3];
-  for (int a = 0; a < 3; ++a) sp[a] = (bounds_hi[a] - bounds_lo[a]) / (float)(resolution - 1);
-  const float sl = std::sqrt(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]);
-  const float threshold = sl * (0.5f + 1.0f / 128.0f);
+  BakePlan plan;
+  int rc = bake_prepare(positions, stride_floats, vertex_count, indices, index_count, resolution, samples, bounds_lo,
+                        bounds_hi, lipschitz, plan);
+  if (rc) return rc;
   void *d_nodes = nullptr, *d_tris = nullptr, *d_out = nullptr, *d_flags = nullptr;
-  hipError_t e1 = hipMalloc(&d_nodes, sizeof(irt_tri_node) * nodes.size());
-  if (e1 == hipSuccess) e1 = hipMalloc(&d_tris, sizeof(irt_tri) * tris.size());
-  if (e1 == hipSuccess) e1 = hipMalloc(&d_out, sizeof(float) * n);
+  hipError_t e1 = hipMalloc(&d_nodes, sizeof(irt_tri_node) * plan.nodes.size());
+  if (e1 == hipSuccess) e1 = hipMalloc(&d_tris, sizeof(irt_tri) * plan.tris.size());
+  if (e1 == hipSuccess) e1 = hipMalloc(&d_out, sizeof(float) * plan.samples);
   if (e1 == hipSuccess) e1 = hipMalloc(&d_flags, 2 * sizeof(uint32_t));
-  const uint32_t init[2] = {0u, 0x3f800000u};  // overflow flag, bits of the squared maximum (starts at 1)
-  if (e1 == hipSuccess) e1 = hipMemcpy(d_nodes, nodes.data(), sizeof(irt_tri_node) * nodes.size(), hipMemcpyHostToDevice);
-  if (e1 == hipSuccess) e1 = hipMemcpy(d_tris, tris.data(), sizeof(irt_tri) * tris.size(), hipMemcpyHostToDevice);
-  if (e1 == hipSuccess) e1 = hipMemcpy(d_flags, init, sizeof(init), hipMemcpyHostToDevice);
-  if (e1 == hipSuccess) {
-    hipLaunchKernelGGL(k_bake_sdf, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, nullptr,
-                       (const irt_tri_node*)d_nodes, (const irt_tri*)d_tris, g, (float*)d_out, (uint32_t*)d_flags);
-    e1 = hipGetLastError();
-  }
-  const uint64_t cells = (uint64_t)(resolution - 1) * (resolution - 1) * (resolution - 1);
-  if (e1 == hipSuccess) {
-    const uint32_t lip_blocks = (uint32_t)std::min<uint64_t>((cells + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_lipschitz, dim3(lip_blocks), dim3(256), 0, nullptr,
-                       (const float*)d_out, resolution, sp[0], sp[1], sp[2], threshold, (uint32_t*)d_flags + 1);
-    e1 = hipGetLastError();
-  }
   uint32_t flags[2] = {0, 0};
-  if (e1 == hipSuccess) e1 = hipMemcpy(samples, d_out, sizeof(float) * n, hipMemcpyDeviceToHost);
-  if (e1 == hipSuccess) e1 = hipMemcpy(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost);
+  if (e1 == hipSuccess)
+    e1 = bake_run(plan, (irt_tri_node*)d_nodes, (irt_tri*)d_tris, (float*)d_out, (uint32_t*)d_flags, samples, flags);
   for (void* p : {d_nodes, d_tris, d_out, d_flags})
     if (p) (void)hipFree(p);
-  if (e1 != hipSuccess) return fail(IRT_ERR_DEVICE, std::string("SDF bake: ") + hipGetErrorString(e1));
-  if (flags[0]) return fail(IRT_ERR_INVALID_ARGUMENT, "SDF classification exceeded 256 intersections");
-  float max_sq;
-  std::memcpy(&max_sq, &flags[1], 4);
-  *lipschitz = std::sqrt(max_sq) * 1.0001f;
-  return IRT_OK;
+  return bake_finish(e1, flags, lipschitz);
+}
+
+int irt_bake_sdf_noalloc(const float* positions, uint32_t stride_floats, uint32_t vertex_count,
+                         const uint32_t* indices, uint32_t index_count, uint32_t resolution, float* samples,
+                         float bounds_lo[3], float bounds_hi[3], float* lipschitz) {
+  if (resolution > IRT_BAKE_NOALLOC_MAX_RESOLUTION || index_count / 3 > IRT_BAKE_NOALLOC_MAX_TRIANGLES)
+    return bake_fail(IRT_ERR_INVALID_ARGUMENT,
+                     "irt_bake_sdf_noalloc: resolution above " + std::to_string(IRT_BAKE_NOALLOC_MAX_RESOLUTION) +
+                         " or more than " + std::to_string(IRT_BAKE_NOALLOC_MAX_TRIANGLES) +
+                         " triangles exceeds its fixed scratch; use irt_bake_sdf");
+  BakePlan plan;
+  int rc = bake_prepare(positions, stride_floats, vertex_count, indices, index_count, resolution, samples, bounds_lo,
+                        bounds_hi, lipschitz, plan);
+  if (rc) return rc;
+  if (plan.nodes.size() > BAKE_MAX_NODES || plan.tris.size() > IRT_BAKE_NOALLOC_MAX_TRIANGLES)
+    return bake_fail(IRT_ERR_INVALID_ARGUMENT, "irt_bake_sdf_noalloc: triangle hierarchy exceeds its fixed scratch");
+  unsigned char* base = nullptr;
+  int dev = 0;
+  rc = bake_scratch(&base, &dev);
+  if (rc) return rc;
+  // One scratch array per device: calls on the same device take turns.
+  std::lock_guard<std::mutex> lock(bake_scratch_lock[dev]);
+  uint32_t flags[2] = {0, 0};
+  const hipError_t e1 = bake_run(plan, (irt_tri_node*)(base + BAKE_OFF_NODES), (irt_tri*)(base + BAKE_OFF_TRIS),
+                                 (float*)(base + BAKE_OFF_SAMPLES), (uint32_t*)(base + BAKE_OFF_FLAGS), samples, flags);
+  return bake_finish(e1, flags, lipschitz);
 }
 
 int irt_lighting(irt_ctx* c) {
