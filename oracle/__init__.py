@@ -92,6 +92,8 @@ def lib() -> ctypes.CDLL:
     L.oracle_normalization.argtypes = [c_int, P]
     L.oracle_shadow_step.restype = d
     L.oracle_shadow_step.argtypes = [c_int, d, d, d, d, d]
+    L.oracle_neighbor_offsets.restype = None
+    L.oracle_neighbor_offsets.argtypes = [c_int, c_uint32, c_uint32, c_uint32, P]
     L.oracle_f32_to_f16.restype = ctypes.c_uint16
     L.oracle_f32_to_f16.argtypes = [c_float]
     L.oracle_f16_to_f32.restype = c_float
@@ -244,6 +246,13 @@ def sample_bsdf(n, color, rough: float, metal: float, wo, state: int):
     weight = np.zeros(3, np.float32)
     lib().oracle_sample_bsdf(_p(nn), _p(cc), rough, metal, _p(w), ctypes.byref(st), _p(direction), _p(weight))
     return direction, weight, st.value
+
+
+def neighbor_offsets(bits: int, state: int, spatial_radius: int, count: int) -> np.ndarray:
+    """[count, 2] spatial-reuse neighbour offsets drawn in sequence from one PCG state."""
+    out = np.zeros((count, 2), np.int32)
+    lib().oracle_neighbor_offsets(bits, state, spatial_radius, count, _p(out))
+    return out
 
 
 class GBufferPlanes:

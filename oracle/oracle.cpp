@@ -462,12 +462,19 @@ struct Oracle {
     if (r.source + 1u == F.size[2]) return io->previous_vpls[r.id];
     return io->archived_vpls[(size_t)(r.source % (to_u32((R)F.rr[2]) + 1u)) * F.counts[1] + r.id];
   }
-  void neighbor(int64_t x, int64_t y, uint32_t* rng, int64_t* qx, int64_t* qy) {  // lighting.wgsl:86-89
+  // lighting.wgsl:86-89: angle 2*PI*u, radius sqrt(u')*spatial_radius, round() half to even
+  static void neighbor_offset(uint32_t* rng, uint32_t spatial_radius, int64_t* dx, int64_t* dy) {
     const R two_pi = (R)(2.0f * 3.14159265359f);  // 2.*PI folded in f32
-    const R angle = two_pi * random(rng);
-    const R radius = std::sqrt(random(rng)) * (R)(float)F.restir[3];
-    *qx = x + to_i32(std::nearbyint(std::cos(angle) * radius));  // round(): half to even
-    *qy = y + to_i32(std::nearbyint(std::sin(angle) * radius));
+    const R angle = two_pi * (R)random_f32(rng);
+    const R radius = std::sqrt((R)random_f32(rng)) * (R)(float)spatial_radius;
+    *dx = to_i32(std::nearbyint(std::cos(angle) * radius));
+    *dy = to_i32(std::nearbyint(std::sin(angle) * radius));
+  }
+  void neighbor(int64_t x, int64_t y, uint32_t* rng, int64_t* qx, int64_t* qy) {  // lighting.wgsl:86-89
+    int64_t dx, dy;
+    neighbor_offset(rng, F.restir[3], &dx, &dy);
+    *qx = x + dx;
+    *qy = y + dy;
   }
 
   void candidates_direct(int64_t x, int64_t y) {  // lighting.wgsl:52-58
@@ -1377,6 +1384,18 @@ double oracle_normalization(int bits, const oracle_reservoir* r) {
 double oracle_shadow_step(int bits, double result, double h, double previous, double t, double k) {
   if (bits == 64) return Oracle<double>::shadow_step(result, h, previous, t, k);
   return (double)Oracle<float>::shadow_step((float)result, (float)h, (float)previous, (float)t, (float)k);
+}
+
+void oracle_neighbor_offsets(int bits, uint32_t state, uint32_t spatial_radius, uint32_t count, int32_t* out) {
+  for (uint32_t i = 0; i < count; ++i) {
+    int64_t dx, dy;
+    if (bits == 64)
+      Oracle<double>::neighbor_offset(&state, spatial_radius, &dx, &dy);
+    else
+      Oracle<float>::neighbor_offset(&state, spatial_radius, &dx, &dy);
+    out[2 * i] = (int32_t)dx;
+    out[2 * i + 1] = (int32_t)dy;
+  }
 }
 
 uint16_t oracle_f32_to_f16(float f) { return f32_to_f16(f); }

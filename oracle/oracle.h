@@ -116,6 +116,9 @@ int oracle_reservoir_add(int bits, oracle_reservoir* r, uint32_t id, uint32_t so
 double oracle_normalization(int bits, const oracle_reservoir* r);
 /* One soft-shadow update of the march (sdf.wgsl:180-181): returns the new result. */
 double oracle_shadow_step(int bits, double result, double h, double previous, double t, double k);
+/* `count` spatial-reuse neighbour offsets (lighting.wgsl:86-89) drawn in
+ * sequence from one PCG state: out[2i], out[2i+1] = (dx, dy). */
+void oracle_neighbor_offsets(int bits, uint32_t state, uint32_t spatial_radius, uint32_t count, int32_t* out);
 uint16_t oracle_f32_to_f16(float f);
 float oracle_f16_to_f32(uint16_t h);
 
