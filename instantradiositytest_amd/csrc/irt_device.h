@@ -29,6 +29,30 @@ namespace irt {
 
 #define IRT_PI 3.14159265359f
 
+// sin and cos for angles |x| <= 2 pi (the spatial neighbour's 2 pi u, the VPL
+// pass's azimuths and half cone angles), evaluated with the same float
+// operations as the oracle so that rounding at pixel and path decisions is
+// reproducible bit for bit (platform sin/cos differ by an ulp between the
+// device and the host): reduction by quarter turns in two
+// constants (after Cody & Waite 1980: pi/2 as its float plus the float of the
+// remainder), then Taylor polynomials in r^2 to degree 11 / 12, whose
+// truncation error on |r| <= pi/4 is below 2e-9.
+__device__ __forceinline__ void shared_sincos(float x, float* sn, float* cs) {
+  const float k = rintf(x * 0.63661977236758134f);  // x / (pi/2), to the nearest quarter turn
+  const float r = (x - k * 1.57079637050628662f) - k * -4.37113900018624283e-8f;
+  const float r2 = r * r;
+  const float s = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f + r2 * (-1.0f / 39916800.0f)))));
+  const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f + r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
+  const int q = ((int)k) & 3;
+  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+__device__ __forceinline__ float shared_cos(float x) {
+  float s, c;
+  shared_sincos(x, &s, &c);
+  return c;
+}
+
 // Square root.  The exact contract keeps the correctly rounded sqrtf.  The
 // contracted unit (built without correctly rounded square roots) lowers sqrtf
 // to the hardware square root plus a rescaling of inputs below 2^-96; calling

@@ -193,13 +193,14 @@ __device__ __forceinline__ irt_reservoir temporal_indirect(const LightArgs<P>& a
   return r;
 }
 
-// neighbor — lighting.wgsl:86-89 (round() is half-to-even: rintf).
+// neighbor — lighting.wgsl:86-89 (round() is half-to-even: rintf; cos and sin
+// shared with the oracle, so the rounding lands on the same pixel).
 __device__ __forceinline__ void neighbor(const irt_frame& F, int32_t x, int32_t y, uint32_t& rng, int32_t* qx,
                                          int32_t* qy) {
   float angle = (2.0f * IRT_PI) * rnd(rng);
   float radius = sqrtf(rnd(rng)) * (float)F.restir[3];
   float sn, cs;
-  sincosf(angle, &sn, &cs);
+  shared_sincos(angle, &sn, &cs);
   *qx = x + f2i_sat(rintf(cs * radius));
   *qy = y + f2i_sat(rintf(sn * radius));
 }

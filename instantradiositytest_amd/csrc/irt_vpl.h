@@ -137,27 +137,6 @@ __device__ __forceinline__ RayHit trace_group(const TriScene& sc, const float4* 
   return h;
 }
 
-// sin and cos of the pass's angles (|x| <= 2 pi: azimuths 2 pi u and half
-// cone angles), evaluated with the same float operations as the oracle so
-// the pool is reproducible bit for bit: reduction by quarter turns in two
-// constants (after Cody & Waite 1980: pi/2 as its float plus the float of the
-// remainder), then Taylor polynomials in r^2 to degree 11 / 12, whose
-// truncation error on |r| <= pi/4 is below 2e-9.
-__device__ __forceinline__ void vpl_sincos(float x, float* sn, float* cs) {
-  const float k = rintf(x * 0.63661977236758134f);  // x / (pi/2), to the nearest quarter turn
-  const float r = (x - k * 1.57079637050628662f) - k * -4.37113900018624283e-8f;
-  const float r2 = r * r;
-  const float s = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f + r2 * (-1.0f / 39916800.0f)))));
-  const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f + r2 * (-1.0f / 3628800.0f + r2 * (1.0f / 479001600.0f))))));
-  const int q = ((int)k) & 3;
-  *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
-  *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
-}
-__device__ __forceinline__ float vpl_cos(float x) {
-  float s, c;
-  vpl_sincos(x, &s, &c);
-  return c;
-}
 
 // basis — common.wgsl:84: mat3x3(t, cross(n, t), n) * v.
 __device__ __forceinline__ v3 basis_mul(v3 n, v3 v) {
@@ -172,7 +151,7 @@ __device__ __forceinline__ v3 cosine_direction(v3 n, uint32_t& rng) {
   const float u = rnd(rng);
   const float phi = 2.0f * IRT_PI * rnd(rng);
   float sn, cs;
-  vpl_sincos(phi, &sn, &cs);
+  shared_sincos(phi, &sn, &cs);
   return basis_mul(n, v3{sqrtf(u) * cs, sqrtf(u) * sn, sqrtf(1.0f - u)});
 }
 
@@ -192,7 +171,7 @@ __device__ __forceinline__ Scatter sample_bsdf(const Surface& s, uint32_t& rng) 
     const float cos_theta = sqrtf((1.0f - u) / (1.0f + (a2 - 1.0f) * u));
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
     const v3 h = basis_mul(s.n, v3{sin_theta * cs, sin_theta * sn, cos_theta});
     const v3 e = -wo;  // reflect(e, h) = e - 2 dot(h, e) h
@@ -240,20 +219,20 @@ __global__ __launch_bounds__(256) void k_generate_vpls(TriScene sc, irt_frame F,
   v3 direction;
   float omega = 4.0f * IRT_PI;
   if (l.color_type[3] > 0.5f) {
-    const float ch = vpl_cos(l.direction_angle[3] * 0.5f);
+    const float ch = shared_cos(l.direction_angle[3] * 0.5f);
     omega = 2.0f * IRT_PI * (1.0f - ch);
     const float r = rnd(rng);
     const float z = ch * (1.0f - r) + 1.0f * r;  // mix(cos(angle/2), 1, r)
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     const float sz = sqrtf(1.0f - z * z);
     direction = basis_mul(normalize(ld3(l.direction_angle)), v3{sz * cs, sz * sn, z});
   } else {
     const float z = 2.0f * rnd(rng) - 1.0f;
     const float phi = 2.0f * IRT_PI * rnd(rng);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     const float sz = sqrtf(1.0f - z * z);
     direction = v3{sz * cs, sz * sn, z};
   }

@@ -14,6 +14,8 @@
 
 namespace {
 
+void shared_sincos(float x, float* sn, float* cs);  // defined with the VPL pass below
+
 template <class R>
 struct Vec {
   R x, y, z;
@@ -467,8 +469,15 @@ struct Oracle {
     const R two_pi = (R)(2.0f * 3.14159265359f);  // 2.*PI folded in f32
     const R angle = two_pi * (R)random_f32(rng);
     const R radius = std::sqrt((R)random_f32(rng)) * (R)(float)spatial_radius;
-    *dx = to_i32(std::nearbyint(std::cos(angle) * radius));
-    *dy = to_i32(std::nearbyint(std::sin(angle) * radius));
+    R cs, sn;
+    if constexpr (sizeof(R) == sizeof(float)) {
+      shared_sincos(angle, &sn, &cs);  // the device's float sin/cos
+    } else {
+      cs = std::cos(angle);
+      sn = std::sin(angle);
+    }
+    *dx = to_i32(std::nearbyint(cs * radius));
+    *dy = to_i32(std::nearbyint(sn * radius));
   }
   void neighbor(int64_t x, int64_t y, uint32_t* rng, int64_t* qx, int64_t* qy) {  // lighting.wgsl:86-89
     int64_t dx, dy;
@@ -1020,10 +1029,10 @@ Hit trace_all(const oracle_scene* sc, const oracle_geometry* g, Vf o, Vf d) {
   return h;
 }
 
-// sin / cos of the VPL pass's angles: quarter-turn reduction in two
+// sin / cos of the spatial neighbour's and the VPL pass's angles: quarter-turn reduction in two
 // constants (after Cody & Waite 1980), Taylor polynomials to degree 11 / 12
 // (truncation below 2e-9 on |r| <= pi/4), the device's float operations.
-void vpl_sincos(float x, float* sn, float* cs) {
+void shared_sincos(float x, float* sn, float* cs) {
   const float k = std::nearbyint(x * 0.63661977236758134f);
   const float r = (x - k * 1.57079637050628662f) - k * -4.37113900018624283e-8f;
   const float r2 = r * r;
@@ -1033,9 +1042,9 @@ void vpl_sincos(float x, float* sn, float* cs) {
   *sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
   *cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
 }
-float vpl_cos(float x) {
+float shared_cos(float x) {
   float s, c;
-  vpl_sincos(x, &s, &c);
+  shared_sincos(x, &s, &c);
   return c;
 }
 
@@ -1055,7 +1064,7 @@ Vf cosine_direction(Vf n, uint32_t* rng) {
   const float u = random_f32(rng);
   const float phi = 2.0f * PI32 * random_f32(rng);
   float sn, cs;
-  vpl_sincos(phi, &sn, &cs);
+  shared_sincos(phi, &sn, &cs);
   return basis_mul(n, V<float>(std::sqrt(u) * cs, std::sqrt(u) * sn, std::sqrt(1.0f - u)));
 }
 
@@ -1071,7 +1080,7 @@ void sample_bsdf(Oracle<float>& o, const Surface<float>& s, Vf wo, uint32_t* rng
     const float phi = 2.0f * PI32 * random_f32(rng);
     const float st = std::sqrt(1.0f - cos_theta * cos_theta);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     const Vf h = basis_mul(s.n, V<float>(st * cs, st * sn, cos_theta));
     const Vf e = scale3(wo, -1.0f);
     wi = sub3(e, scale3(h, 2.0f * dot3(h, e)));  // reflect(-wo, h)
@@ -1110,20 +1119,20 @@ irt_vpl generate_one(Oracle<float>& o, const oracle_scene* sc, const oracle_geom
   Vf origin = F3<float>(l.position_strength), direction;
   float omega = 4.0f * PI32;
   if (l.color_type[3] > 0.5f) {  // vpl.wgsl:8-10
-    const float ch = vpl_cos(l.direction_angle[3] * 0.5f);
+    const float ch = shared_cos(l.direction_angle[3] * 0.5f);
     omega = 2.0f * PI32 * (1.0f - ch);
     const float r = random_f32(&rng);
     const float z = ch * (1.0f - r) + 1.0f * r;
     const float phi = 2.0f * PI32 * random_f32(&rng);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     direction = basis_mul(normalize3(F3<float>(l.direction_angle)),
                           V<float>(std::sqrt(1.0f - z * z) * cs, std::sqrt(1.0f - z * z) * sn, z));
   } else {  // vpl.wgsl:11
     const float z = 2.0f * random_f32(&rng) - 1.0f;
     const float phi = 2.0f * PI32 * random_f32(&rng);
     float sn, cs;
-    vpl_sincos(phi, &sn, &cs);
+    shared_sincos(phi, &sn, &cs);
     direction = V<float>(std::sqrt(1.0f - z * z) * cs, std::sqrt(1.0f - z * z) * sn, z);
   }
   const float Lf = (float)L;  // vpl.wgsl:14

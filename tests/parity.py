@@ -28,15 +28,19 @@ def compare_reservoirs(gpu: np.ndarray, ref: np.ndarray) -> dict:
     same = np.ones(g.size, bool)
     for k in RES_INT:
         same &= g[k] == ref[k]
-    rel = 0.0
+    rel, worst = 0.0, None
+    idx = np.nonzero(same)[0]
     for k in RES_FLT:
         a = g[k][same].astype(np.float64)
         b = ref[k][same]
         fin = np.isfinite(a) & np.isfinite(b)
         d = np.abs(a[fin] - b[fin]) / np.maximum(np.abs(b[fin]), 1e-30)
-        if d.size:
-            rel = max(rel, float(d.max()))
-    return {"match": float(same.mean()), "max_rel": rel, "mismatches": int((~same).sum())}
+        if d.size and float(d.max()) > rel:
+            rel = float(d.max())
+            j = int(idx[fin][np.argmax(d)])
+            worst = {"pixel": j, "field": k, "gpu": {f: float(g[f][j]) for f in RES_FLT + RES_INT},
+                     "ref": {f: float(ref[f][j]) for f in RES_FLT + RES_INT}}
+    return {"match": float(same.mean()), "max_rel": rel, "mismatches": int((~same).sum()), "worst": worst}
 
 
 def compare_images(gpu: np.ndarray, ref: np.ndarray, tol: float = 1e-3, floor: float = 1e-6) -> dict:
