@@ -83,13 +83,16 @@ class SceneStream:
     inputs are staged: each frame's G-buffer and VPL pool are produced on
     the device by the server (null request pointers)."""
 
-    def __init__(self, cfg: SceneConfig, precision: str, frames: int, log=None, device_inputs: bool = False):
+    def __init__(self, cfg: SceneConfig, precision: str, frames: int, log=None, device_inputs: bool = False,
+                 cs: ConfigScene | None = None):
         from . import hip
         self.cfg = cfg
-        self.cs = ConfigScene(cfg)
+        self.cs = cs if cs is not None else ConfigScene(cfg)
         self.settings = cfg.settings()
         self.pixels = cfg.width * cfg.height
         self.device_inputs = device_inputs
+        self.generation = self.cs.scene.generation
+        self.light_count = len(self.cs.scene.lights)
         # the template only holds the scene: an 8x8 view keeps its state small
         self.template = Renderer(self.cs.scene, cfg.settings(render_width=8, render_height=8), precision=precision,
                                  schedule="fused")
@@ -112,6 +115,14 @@ class SceneStream:
     @property
     def scene_ctx(self):
         return self.template.ctx
+
+    def release_host(self):
+        """Drops the host copy of the scene (SDF atlas, meshes) once the
+        template context holds it; only streams whose inputs are produced on
+        the device can do without it."""
+        if not self.device_inputs:
+            raise ValueError("a stream with staged host inputs keeps its scene")
+        self.cs = None
 
     def with_device_inputs(self) -> "SceneStream":
         """The same scene and camera path with every frame's inputs produced on
@@ -246,6 +257,20 @@ class Arrival:
     offset: int
 
 
+def session_arrivals(rate: float, duration: float, class_probs: list, lifetimes: tuple,
+                     rng: np.random.Generator) -> tuple:
+    """Poisson arrivals where every session brings its own scene: arrival i
+    uses stream i.  Returns (arrivals, resolution class of each arrival)."""
+    out, classes = [], []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            return out, classes
+        classes.append(int(rng.choice(len(class_probs), p=class_probs)))
+        out.append(Arrival(t, len(out), int(rng.integers(lifetimes[0], lifetimes[1] + 1)), 0))
+
+
 def poisson_arrivals(rate: float, duration: float, n_streams_by_class: list, class_probs: list,
                      lifetimes: tuple, frames_per_stream: int, rng: np.random.Generator) -> list:
     """Arrival times of a Poisson process (exponential gaps), each with a
@@ -348,7 +373,7 @@ def simulate(server, streams: list, arrivals: list, duration_s: float, warmup_s:
                 fr = st.frames[(c.arrival.offset + c.next_frame) % len(st.frames)]
                 tag += 1
                 tags[tag] = c.sid
-                server.submit(c.sid, fr, st.cs.scene.generation, len(st.cs.scene.lights), st.cfg.vpl_count, tag)
+                server.submit(c.sid, fr, st.generation, st.light_count, st.cfg.vpl_count, tag)
                 c.outstanding = True
                 c.next_frame += 1
                 progressed = True
