@@ -152,8 +152,10 @@ typedef enum irt_precision {
    * Rgba16F target are rounded to binary16 at the same point, so results
    * follow the reference's storage rounding. */
   IRT_PRECISION_FP32 = 0,
-  /* FP16 G-buffer / radiance / SVGF radiance, FP16 first moment with FP32
-   * second moment, 16-bit SDF samples; FP32 march, FP32 reservoir W/M. */
+  /* binary16 G-buffer / radiance / SVGF radiance / direct lighting; SVGF
+   * moments as binary16 first moment and age with an FP32 second moment
+   * (8 B); binary16 SDF samples rounded toward -inf; FP32 arithmetic,
+   * reservoir W/M sums and march. */
   IRT_PRECISION_MIXED = 1
 } irt_precision;
 

@@ -34,6 +34,7 @@ enum Fmt {
   FMT_F32X2_AS_F16X2, // internal float2 <-> Rg16F
   FMT_F16X2,          // internal binary16x2 <-> Rg16F
   FMT_F16X4_AS_F32X4, // internal binary16x4 <-> Rgba32F
+  FMT_MOM8,           // internal Mom8 (binary16 m1 and age, FP32 m2) <-> Rgba32F
   FMT_VPL             // irt_vpl
 };
 
@@ -42,7 +43,7 @@ size_t internal_bytes(Fmt f) {
     case FMT_RESERVOIR: return 32;
     case FMT_F32: return 4;
     case FMT_F32X4: case FMT_F32X4_AS_F16X4: return 16;
-    case FMT_F16X4: case FMT_F16X4_AS_F32X4: return 8;
+    case FMT_F16X4: case FMT_F16X4_AS_F32X4: case FMT_MOM8: return 8;
     case FMT_F32X2_AS_F16X2: return 8;
     case FMT_F16X2: return 4;
     case FMT_VPL: return 48;
@@ -53,7 +54,7 @@ size_t external_bytes(Fmt f) {
   switch (f) {
     case FMT_RESERVOIR: return 32;
     case FMT_F32: return 4;
-    case FMT_F32X4: case FMT_F16X4_AS_F32X4: return 16;
+    case FMT_F32X4: case FMT_F16X4_AS_F32X4: case FMT_MOM8: return 16;
     case FMT_F32X4_AS_F16X4: case FMT_F16X4: return 8;
     case FMT_F32X2_AS_F16X2: case FMT_F16X2: return 4;
     case FMT_VPL: return 48;
@@ -176,7 +177,7 @@ struct irt_ctx {
   void* scratch[2] = {nullptr, nullptr};
   void* feedback[2] = {nullptr, nullptr};
   void* history = nullptr;
-  float4* moments[2] = {nullptr, nullptr};
+  void* moments[2] = {nullptr, nullptr};  // SVGF moments: float4 (FP32 profile) or Mom8 (mixed)
   GuidePx* guide_slot[2] = {nullptr, nullptr};  // alternating per rebuild
   int guide_cur = 0;
   GuidePx* guide = nullptr;  // per-pixel guide records of the current G-buffer (fused schedule)
@@ -393,8 +394,8 @@ std::vector<std::pair<void**, size_t>> view_alloc_plan(irt_ctx* c, bool m) {
   A(&c->feedback[0], n * r4);
   A(&c->feedback[1], n * r4);
   A(&c->history, n * r4);
-  A(&c->moments[0], n * 16);
-  A(&c->moments[1], n * 16);
+  A(&c->moments[0], n * (m ? sizeof(Mom8) : 16));
+  A(&c->moments[1], n * (m ? sizeof(Mom8) : 16));
   A(&c->counters, 8 * sizeof(unsigned long long));
   A(&c->guide_slot[0], n * sizeof(GuidePx));
   A(&c->guide_slot[1], n * sizeof(GuidePx));
@@ -426,8 +427,8 @@ BufDesc describe(irt_ctx* c, uint32_t b) {
     case IRT_BUF_DIRECT_LIGHTING: return {c->direct_lighting, m ? FMT_F16X4_AS_F32X4 : FMT_F32X4, n};
     case IRT_BUF_LIGHTING: return {c->lighting, r4, n};
     case IRT_BUF_HISTORY: return {c->history, r4, n};
-    case IRT_BUF_MOMENTS: return {c->moments[cur], FMT_F32X4, n};
-    case IRT_BUF_MOMENTS_OLD: return {c->moments[old], FMT_F32X4, n};
+    case IRT_BUF_MOMENTS: return {c->moments[cur], m ? FMT_MOM8 : FMT_F32X4, n};
+    case IRT_BUF_MOMENTS_OLD: return {c->moments[old], m ? FMT_MOM8 : FMT_F32X4, n};
     case IRT_BUF_FEEDBACK: return {c->feedback[cur], r4, n};
     case IRT_BUF_FEEDBACK_OLD: return {c->feedback[old], r4, n};
     case IRT_BUF_TEMPORAL: return {c->temporal, r4, n};
@@ -466,6 +467,20 @@ void to_external(Fmt f, const uint8_t* in, uint8_t* out, size_t count) {
       for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f16_to_f32(s[i]);
       break;
     }
+    case FMT_MOM8: {  // (m1 f16, age f16, m2 f32) -> (m1, m2, age, 0)
+      float* d = (float*)out;
+      for (size_t i = 0; i < count; ++i) {
+        uint16_t h[2];
+        float m2;
+        std::memcpy(h, in + 8 * i, 4);
+        std::memcpy(&m2, in + 8 * i + 4, 4);
+        d[4 * i] = irt_host::f16_to_f32(h[0]);
+        d[4 * i + 1] = m2;
+        d[4 * i + 2] = irt_host::f16_to_f32(h[1]);
+        d[4 * i + 3] = 0.0f;
+      }
+      break;
+    }
     default:
       std::memcpy(out, in, count * internal_bytes(f));
   }
@@ -488,6 +503,15 @@ void to_internal(Fmt f, const uint8_t* in, uint8_t* out, size_t count) {
       const float* s = (const float*)in;
       uint16_t* d = (uint16_t*)out;
       for (size_t i = 0; i < count * 4; ++i) d[i] = irt_host::f32_to_f16(s[i]);
+      break;
+    }
+    case FMT_MOM8: {
+      const float* s = (const float*)in;
+      for (size_t i = 0; i < count; ++i) {
+        const uint16_t h[2] = {irt_host::f32_to_f16(s[4 * i]), irt_host::f32_to_f16(s[4 * i + 2])};
+        std::memcpy(out + 8 * i, h, 4);
+        std::memcpy(out + 8 * i + 4, &s[4 * i + 1], 4);
+      }
       break;
     }
     default:
@@ -748,11 +772,11 @@ int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.old_radiance = (const typename P::R4*)c->feedback[old];
-  a.old_moments = c->moments[old];
+  a.old_moments = (const typename P::M4*)c->moments[old];
   a.input = (const typename P::R4*)c->feedback[old];
-  a.moments = c->moments[old];
+  a.moments = (const typename P::M4*)c->moments[old];
   a.out = (typename P::R4*)c->temporal;
-  a.moments_out = c->moments[cur];
+  a.moments_out = (typename P::M4*)c->moments[cur];
   if (guide) {
     const dim3 grid((c->W + TMP_TILE - 1) / TMP_TILE, (c->H + TMP_TILE - 1) / TMP_TILE);
     if (c->counting)
@@ -771,7 +795,7 @@ int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.input = (const typename P::R4*)c->temporal;
-  a.moments = c->moments[cur];
+  a.moments = (const typename P::M4*)c->moments[cur];
   a.out = (typename P::R4*)c->scratch[0];
   if (guide) {
     const dim3 grid((c->W + VAR_TILE - 1) / VAR_TILE, (c->H + VAR_TILE - 1) / VAR_TILE);
@@ -791,7 +815,7 @@ int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, co
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.input = (const typename P::R4*)in;
-  a.moments = c->moments[cur];
+  a.moments = (const typename P::M4*)c->moments[cur];
   a.out = (typename P::R4*)out;
   a.out2 = (typename P::R4*)out2;
   a.stride = 1u << i;

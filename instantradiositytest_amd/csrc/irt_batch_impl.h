@@ -237,14 +237,14 @@ void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
   base.guide = c->guide;
   bv.st = base;
   bv.st.old_radiance = (const typename P::R4*)c->feedback[old];
-  bv.st.old_moments = c->moments[old];
+  bv.st.old_moments = (const typename P::M4*)c->moments[old];
   bv.st.input = (const typename P::R4*)c->feedback[old];
-  bv.st.moments = c->moments[old];
+  bv.st.moments = (const typename P::M4*)c->moments[old];
   bv.st.out = (typename P::R4*)c->temporal;
-  bv.st.moments_out = c->moments[cur];
+  bv.st.moments_out = (typename P::M4*)c->moments[cur];
   bv.sv = base;
   bv.sv.input = (const typename P::R4*)c->temporal;
-  bv.sv.moments = c->moments[cur];
+  bv.sv.moments = (const typename P::M4*)c->moments[cur];
   bv.sv.out = (typename P::R4*)c->scratch[0];
   // a-trous with the feedback / final copies folded into the stores (run_svgf, fused)
   void* in = c->scratch[0];
@@ -263,7 +263,7 @@ void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
     }
     bv.sa[i] = base;
     bv.sa[i].input = (const typename P::R4*)in;
-    bv.sa[i].moments = c->moments[cur];
+    bv.sa[i].moments = (const typename P::M4*)c->moments[cur];
     bv.sa[i].out = (typename P::R4*)out;
     bv.sa[i].out2 = (typename P::R4*)out2;
     bv.sa[i].stride = 1u << i;

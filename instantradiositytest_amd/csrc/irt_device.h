@@ -186,18 +186,45 @@ __device__ __forceinline__ GuidePx ld_guide(const GuidePx* g, uint32_t i) {
   return unpack_guide(p[0], p[1]);
 }
 
+// SVGF moments record of the mixed profile, 8 B: the first moment and the age
+// in binary16 (ages are integers <= 32, exact), the second moment in FP32.
+// The second moment keeps FP32 because the variance m2 - m1^2 cancels: a
+// 16-bit m2 would move the variance of a smooth region by more than the
+// variance itself.  The first moment is a luminance mean, at most 65000 as
+// the radiance is.  The FP32 profile keeps the reference's Rgba32F record.
+struct __align__(8) Mom8 {
+  _Float16 m0, age;
+  float m1;
+};
+static_assert(sizeof(Mom8) == 8, "mixed moments record is 8 bytes");
+__device__ __forceinline__ float4 ld_mom(const float4* p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float4 ld_mom(const Mom8* p, uint32_t i) {
+  const Mom8 r = p[i];
+  return make_float4((float)r.m0, r.m1, (float)r.age, 0.0f);
+}
+__device__ __forceinline__ void st_mom(float4* p, uint32_t i, float4 v) { p[i] = v; }
+__device__ __forceinline__ void st_mom(Mom8* p, uint32_t i, float4 v) {
+  Mom8 r;
+  r.m0 = (_Float16)v.x;
+  r.age = (_Float16)v.z;
+  r.m1 = v.y;
+  p[i] = r;
+}
+
 // Storage profiles.
 struct ProfileF32 {
   typedef float4 N4;  // normal, colour (xyz/rgb, w)
   typedef float2 P2;  // params (rough, metal), motion
   typedef float4 R4;  // radiance targets (values binary16-rounded as the reference stores them)
   typedef float4 L4;  // direct lighting (Rgba32F in the reference)
+  typedef float4 M4;  // SVGF moments (Rgba32F in the reference)
 };
 struct ProfileMixed {
   typedef h4 N4;
   typedef h2 P2;
   typedef h4 R4;
   typedef h4 L4;
+  typedef Mom8 M4;
 };
 
 // ------------------------------------------------------------- counters

@@ -343,7 +343,7 @@ __device__ __forceinline__ void body_svgf_temporal(const SvgfArgs<P>& a, const P
   float4 rad, mom;
   svgf_temporal<P, C, FAST>(a, p.x, p.y, &rad, &mom, cnt);
   st_r16(a.out, p.id, rad);
-  a.moments_out[p.id] = mom;
+  st_mom(a.moments_out, p.id, mom);
 }
 template <class P, class C, bool FAST>
 __device__ __forceinline__ void body_svgf_variance(const SvgfArgs<P>& a, const Px& p, C& cnt) {
@@ -411,7 +411,7 @@ __device__ __forceinline__ void temporal_tile(const SvgfArgs<P>& a, int32_t x0, 
     float4 rad, mom;
     svgf_temporal_staged<P>(a, x, y, s_g, s_r, i + TMP_HALO, j + TMP_HALO, TMP_SIDE, &rad, &mom, cnt);
     st_r16(a.out, id, rad);
-    a.moments_out[id] = mom;
+    st_mom(a.moments_out, id, mom);
   }
 }
 template <class P, bool CNT>
@@ -434,7 +434,7 @@ __device__ __forceinline__ void variance_tile(const SvgfArgs<P>& a, int32_t x0, 
   const int32_t x = x0 + i, y = y0 + j;
   const bool in = x < W && y < H;
   const uint32_t id = in ? (uint32_t)y * (uint32_t)W + (uint32_t)x : 0u;
-  const bool young = in && a.moments[id].z < 4.0f;
+  const bool young = in && ld_mom(a.moments, id).z < 4.0f;
   if (__syncthreads_or(young)) {
     for (int32_t e = (int32_t)threadIdx.x; e < VAR_SIDE * VAR_SIDE; e += 256) {
       const int32_t gx = x0 + e % VAR_SIDE - VAR_HALO, gy = y0 + e / VAR_SIDE - VAR_HALO;
@@ -443,7 +443,7 @@ __device__ __forceinline__ void variance_tile(const SvgfArgs<P>& a, int32_t x0, 
         const uint4* gp = reinterpret_cast<const uint4*>(a.guide + q);
         s_g[e][0] = gp[0];
         s_g[e][1] = gp[1];
-        const float4 mo = a.moments[q];
+        const float4 mo = ld_mom(a.moments, q);
         s_m[e] = make_float2(mo.x, mo.y);
       } else {
         s_g[e][0] = make_uint4(0u, 0u, 0u, 0u);  // valid = 0: outside the image

@@ -363,12 +363,12 @@ struct SvgfArgs {
   const typename P::N4* old_normal;
   const typename P::R4* noisy;
   const typename P::R4* old_radiance;
-  const float4* old_moments;
+  const typename P::M4* old_moments;
   const typename P::R4* input;
-  const float4* moments;
+  const typename P::M4* moments;
   typename P::R4* out;
   typename P::R4* out2;  // optional second destination (feedback / final copy)
-  float4* moments_out;
+  typename P::M4* moments_out;
   uint32_t stride;
   unsigned long long* counters;
   const GuidePx* guide;  // when set, positions and G-buffer channels come from the guide records
@@ -521,7 +521,7 @@ __device__ __forceinline__ void svgf_temporal(const SvgfArgs<P>& a, int32_t x, i
         int32_t qx = bx + i, qy = by + j;
         if (!history_ok(a, x, y, qx, qy)) continue;
         const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
-        float4 mo = a.old_moments[q];
+        float4 mo = ld_mom(a.old_moments, q);
         if (mo.z <= 0.0f) continue;
         float wx = i == 1 ? fx : 1.0f - fx;
         float wy = j == 1 ? fy : 1.0f - fy;
@@ -575,7 +575,7 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
   const uint32_t W = a.F.size[0];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   float4 c = ld4(a.input, id);
-  float age = a.moments[id].z;
+  float age = ld_mom(a.moments, id).z;
   float sigma = c.w;
   if (age < 4.0f && valid_px(a, id)) {
     float s0 = 0.f, s1 = 0.f, total = 0.f;
@@ -586,7 +586,7 @@ __device__ __forceinline__ float4 svgf_variance(const SvgfArgs<P>& a, int32_t x,
         int32_t qx = x + i, qy = y + j;
         float w = geometry_weight<P, C, FAST>(a, g, qx, qy, 1.0f, inv_tol, cnt);
         if (w <= 0.0f) continue;
-        float4 mo = a.moments[(uint32_t)qy * W + (uint32_t)qx];
+        float4 mo = ld_mom(a.moments, (uint32_t)qy * W + (uint32_t)qx);
         s0 += mo.x * w;
         s1 += mo.y * w;
         total += w;
@@ -646,7 +646,7 @@ __device__ __forceinline__ void svgf_temporal_staged(const SvgfArgs<P>& a, int32
         int32_t qx = bx + i, qy = by + j;
         if (!history_ok_at(a, pa, na, qx, qy)) continue;
         const uint32_t q = (uint32_t)qy * W + (uint32_t)qx;
-        float4 mo = a.old_moments[q];
+        float4 mo = ld_mom(a.old_moments, q);
         if (mo.z <= 0.0f) continue;
         float wx = i == 1 ? fx : 1.0f - fx;
         float wy = j == 1 ? fy : 1.0f - fy;
@@ -707,7 +707,7 @@ __device__ __forceinline__ float4 svgf_variance_staged(const SvgfArgs<P>& a, int
   const uint32_t W = a.F.size[0];
   const uint32_t id = (uint32_t)y * W + (uint32_t)x;
   float4 c = ld4(a.input, id);
-  float age = a.moments[id].z;
+  float age = ld_mom(a.moments, id).z;
   float sigma = c.w;
   const int32_t e0 = lj * n + li;
   const GuidePx self = unpack_guide(sg[e0][0], sg[e0][1]);

@@ -92,8 +92,7 @@ def fake_streams(n=2, frames=4):
     out = []
     for i in range(n):
         out.append(SimpleNamespace(settings=None, scene_ctx=f"scene{i}", frames=[f"s{i}f{k}" for k in range(frames)],
-                                   cs=SimpleNamespace(scene=SimpleNamespace(generation=0, lights=[1, 2])),
-                                   cfg=SimpleNamespace(vpl_count=12)))
+                                   generation=0, light_count=2, cfg=SimpleNamespace(vpl_count=12)))
     return out
 
 
@@ -181,12 +180,12 @@ def test_state_estimate_counts_every_per_view_buffer():
     fp32 = L.irt_estimate_bytes(ctypes.byref(s), 0)
     ring = (hist + 1) * V * 48
     # per pixel, both profiles: reservoirs 6 x 32, guide records 2 slots x 32, resolve record 32, ray
-    # requests 2 x 64, visibility 2 x 4, moments 2 x 16, depth 3 G-buffer slots x 4
-    common = 6 * 32 + 2 * 32 + 32 + 128 + 8 + 32 + 3 * 4
+    # requests 2 x 64, visibility 2 x 4, depth 3 G-buffer slots x 4
+    common = 6 * 32 + 2 * 32 + 32 + 128 + 8 + 3 * 4
     # mixed: normal + colour 3 slots x 2 x 8, params + motion 3 slots x 2 x 4, direct lighting +
-    # noisy radiance (2 slots) + temporal + 2 scratch + 2 feedback + history = 9 x 8
-    assert mixed == n * (common + 48 + 24 + 72) + ring + 64 + 64
-    # fp32: the same planes at twice the width plus 24 B/px of binary16 staging for ingest
-    assert fp32 == n * (common + 96 + 48 + 144 + 24) + ring + 64 + 64
+    # noisy radiance (2 slots) + temporal + 2 scratch + 2 feedback + history = 9 x 8, moments 2 x 8
+    assert mixed == n * (common + 48 + 24 + 72 + 16) + ring + 64 + 64
+    # fp32: the same planes at twice the width, moments 2 x 16, plus 24 B/px of binary16 staging
+    assert fp32 == n * (common + 96 + 48 + 144 + 32 + 24) + ring + 64 + 64
     s2 = make_settings(render_width=2 * W, render_height=H, vpl_count=V, restir_max_history=hist)
-    assert L.irt_estimate_bytes(ctypes.byref(s2), 1) - mixed == n * (common + 48 + 24 + 72)
+    assert L.irt_estimate_bytes(ctypes.byref(s2), 1) - mixed == n * (common + 48 + 24 + 72 + 16)
