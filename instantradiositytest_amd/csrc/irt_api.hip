@@ -19,7 +19,6 @@
 #include "irt_present.h"
 #include "irt_bake.h"
 
-#include <hipcub/device/device_scan.hpp>
 
 using namespace irt;
 
@@ -136,8 +135,8 @@ struct irt_ctx {
   uint32_t cap_gb_tiles = 0;
   uint32_t* gb_scan = nullptr;
   uint32_t cap_gb_scan = 0;
-  uint8_t* gb_scan_tmp = nullptr;
-  uint32_t cap_gb_scan_tmp = 0;
+  uint32_t* gb_scan_sums = nullptr;  // block sums, then block offsets, of the tile-count scan
+  uint32_t cap_gb_scan_sums = 0;
   unsigned long long* gb_vis = nullptr;
   uint32_t cap_gb_vis = 0;
   bool gb_vis_clear = false;  // gb_vis holds the clear key (each resolve leaves it so)
@@ -1810,10 +1809,8 @@ int irt_render_gbuffer(irt_ctx* c) {
   if (!rc) rc = ensure_array(c, &c->gb_vis, &c->cap_gb_vis, (uint32_t)c->npx);
   if (rc) return rc;
   if (c->gb_vis != vis_before) c->gb_vis_clear = false;
-  size_t tmp_bytes = 0;
-  HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, c->gb_tiles, c->gb_scan, (int)std::max(n, 1u),
-                                                c->stream));
-  rc = ensure_array(c, &c->gb_scan_tmp, &c->cap_gb_scan_tmp, (uint32_t)tmp_bytes);
+  const uint32_t scan_blocks = (n + SCAN_BLOCK_ITEMS - 1) / SCAN_BLOCK_ITEMS;
+  rc = ensure_array(c, &c->gb_scan_sums, &c->cap_gb_scan_sums, scan_blocks);
   if (rc) return rc;
   if (base != c->host_tri_base || c->cap_obj_tri_base == 0) {
     HIP_CHECK(c, hipMemcpyAsync(c->obj_tri_base, base.data(), sizeof(uint32_t) * base.size(), hipMemcpyHostToDevice,
@@ -1837,8 +1834,10 @@ int irt_render_gbuffer(irt_ctx* c) {
   if (n) {
     hipLaunchKernelGGL(k_gb_setup, dim3((n + 255) / 256), dim3(256), 0, c->stream, sc, c->gb_objs, n, W, H,
                        c->gb_tris, c->gb_attrs, c->gb_tiles);
-    HIP_CHECK(c, hipcub::DeviceScan::InclusiveSum(c->gb_scan_tmp, tmp_bytes, c->gb_tiles, c->gb_scan, (int)n,
-                                                  c->stream));
+    hipLaunchKernelGGL(k_scan_sums, dim3(scan_blocks), dim3(256), 0, c->stream, c->gb_tiles, n, c->gb_scan_sums);
+    hipLaunchKernelGGL(k_scan_carries, dim3(1), dim3(256), 0, c->stream, c->gb_scan_sums, scan_blocks);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(256), 0, c->stream, c->gb_tiles, n, c->gb_scan_sums,
+                       c->gb_scan);
     hipLaunchKernelGGL(k_gb_raster, dim3(c->march_blocks), dim3(256), 0, c->stream, c->gb_tris, c->gb_scan, n, W, H,
                        c->gb_vis);
   }

@@ -242,6 +242,78 @@ __global__ __launch_bounds__(256) void k_gb_raster(const GbTri* __restrict__ tri
   }
 }
 
+// Inclusive scan of the per-triangle tile counts (the raster's work list):
+// k_scan_sums writes the sum of each block of SCAN_BLOCK_ITEMS counts,
+// k_scan_carries turns those sums into exclusive block offsets in place (one
+// workgroup, 256 sums per step with a running carry), and k_scan_blocks scans
+// each block and adds its offset.  Block scans are Hillis-Steele over 256
+// per-thread totals in LDS; each thread owns SCAN_THREAD_ITEMS consecutive
+// counts.
+constexpr uint32_t SCAN_THREAD_ITEMS = 2, SCAN_BLOCK_ITEMS = 256 * SCAN_THREAD_ITEMS;
+
+// Inclusive scan of v over the 256 threads of the block (s: 256 words of
+// LDS); *total gets the block's sum.  Both are read before the last barrier,
+// so a following call may overwrite s.
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* s, uint32_t* total) {
+  const uint32_t t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint32_t add = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  const uint32_t r = s[t];
+  *total = s[255];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                   uint32_t* __restrict__ sums) {
+  __shared__ uint32_t s[256];
+  const uint32_t base = blockIdx.x * SCAN_BLOCK_ITEMS + threadIdx.x * SCAN_THREAD_ITEMS;
+  uint32_t v = 0;
+  for (uint32_t k = 0; k < SCAN_THREAD_ITEMS; ++k)
+    if (base + k < n) v += in[base + k];
+  uint32_t total;
+  block_scan_256(v, s, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_carries(uint32_t* sums, uint32_t m) {
+  __shared__ uint32_t s[256];
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < m; c0 += 256) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < m ? sums[i] : 0u;
+    uint32_t total;
+    const uint32_t incl = block_scan_256(v, s, &total);
+    if (i < m) sums[i] = carry + incl - v;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict__ in, uint32_t n,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     uint32_t* __restrict__ out) {
+  __shared__ uint32_t s[256];
+  const uint32_t base = blockIdx.x * SCAN_BLOCK_ITEMS + threadIdx.x * SCAN_THREAD_ITEMS;
+  uint32_t v[SCAN_THREAD_ITEMS];
+  uint32_t total = 0;
+  for (uint32_t k = 0; k < SCAN_THREAD_ITEMS; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    total += v[k];
+  }
+  uint32_t block_total;
+  uint32_t run = offsets[blockIdx.x] + block_scan_256(total, s, &block_total) - total;
+  for (uint32_t k = 0; k < SCAN_THREAD_ITEMS; ++k) {
+    run += v[k];
+    if (base + k < n) out[base + k] = run;
+  }
+}
+
 __global__ void k_gb_clear(unsigned long long* vis, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) vis[i] = GB_CLEAR_KEY;
