@@ -530,6 +530,23 @@ int irt_server_stats(const irt_server* server, irt_server_status* out);
  * (the scene is kept). */
 int irt_reset(irt_ctx* ctx);
 
+/* Stream-ordering audit (debug runs).  No counterpart in the reference, whose
+ * single wgpu queue orders every pass; it is for checking this library's two
+ * streams per context (lighting, and SVGF of the fused schedule).  With
+ * IRT_STREAM_AUDIT=1 in the environment when a context is created, every
+ * operation the context enqueues declares the buffers it reads and writes,
+ * and a read after a write, write after a read or write after a write on the
+ * other stream with no join, event wait or host synchronisation between them
+ * is recorded.  irt_audit_report returns the number of such violations and
+ * writes their text (one per line, then a line with the number of audited
+ * operations on each stream) into out; IRT_ERR_STATE when the audit is
+ * off or the context has joined a batch (batched steps run on the batch's
+ * streams, which the audit does not model).  irt_host_audit_selftest runs
+ * scripted schedules with known answers through the audit, without a GPU:
+ * 0 when every case is judged right, else the number of the first wrong one. */
+int irt_audit_report(irt_ctx* ctx, char* out, size_t capacity);
+int irt_host_audit_selftest(void);
+
 /* ------------------------------------------------ host precompute (a23)
  * The scene-side helpers renderer.rs runs on the CPU before upload. */
 

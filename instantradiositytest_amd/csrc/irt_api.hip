@@ -6,8 +6,12 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
+#include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -64,6 +68,95 @@ size_t external_bytes(Fmt f) {
 struct Timer {
   std::string name;
   hipEvent_t a, b;
+};
+
+// ------------------------------------------------------ stream audit
+// A host-side model of the ordering between a context's two streams (0: the
+// lighting stream, 1: the SVGF stream), for checking the schedule in debug
+// runs.  Every enqueued operation declares the device buffers it reads and
+// writes; joins, event waits and host synchronisations record which of the
+// other stream's operations a stream is ordered after.  A read after a write,
+// a write after a read or a write after a write on different streams with no
+// such ordering is a violation.  Operations on one stream are ordered by the
+// stream itself.
+class StreamAudit {
+ public:
+  struct Snapshot {  // a stream's knowledge when an event is recorded on it
+    uint64_t epoch[2] = {0, 0};
+    uint64_t seen[2] = {0, 0};
+  };
+  void op(int s, std::initializer_list<const void*> reads, std::initializer_list<const void*> writes,
+          const char* what) {
+    const uint64_t e = ++epoch_[s];
+    for (const void* p : reads) {
+      if (!p) continue;
+      const Buf& b = bufs_[p];
+      if (b.w >= 0 && !ordered(s, b.w, b.we)) report("read", what, s, p, b.wname, b.w);
+    }
+    for (const void* p : writes) {
+      if (!p) continue;
+      const Buf& b = bufs_[p];
+      if (b.w >= 0 && !ordered(s, b.w, b.we)) report("write after write", what, s, p, b.wname, b.w);
+      for (int t = 0; t < 2; ++t)
+        if (t != s && b.r[t] && !ordered(s, t, b.r[t])) report("write after read", what, s, p, b.rname[t], t);
+    }
+    for (const void* p : reads)
+      if (p) {
+        bufs_[p].r[s] = e;
+        bufs_[p].rname[s] = what;
+      }
+    for (const void* p : writes)
+      if (p) {
+        Buf& b = bufs_[p];
+        b.w = s;
+        b.we = e;
+        b.wname = what;
+        b.r[0] = b.r[1] = 0;
+      }
+  }
+  // `to` waits for everything enqueued on `from` so far.
+  void join(int from, int to) { wait(to, snapshot(from), from); }
+  Snapshot snapshot(int s) const {
+    Snapshot sn;
+    sn.epoch[0] = epoch_[0];
+    sn.epoch[1] = epoch_[1];
+    sn.seen[0] = seen_[s][0];
+    sn.seen[1] = seen_[s][1];
+    return sn;
+  }
+  void wait(int to, const Snapshot& sn, int from) {
+    seen_[to][from] = std::max(seen_[to][from], sn.epoch[from]);
+    for (int x = 0; x < 2; ++x) seen_[to][x] = std::max(seen_[to][x], sn.seen[x]);
+  }
+  // The host waited for stream s: later operations on either stream follow
+  // everything s had enqueued and everything s was ordered after.
+  void host_sync(int s) {
+    for (int x = 0; x < 2; ++x) {
+      seen_[x][s] = std::max(seen_[x][s], epoch_[s]);
+      for (int y = 0; y < 2; ++y) seen_[x][y] = std::max(seen_[x][y], seen_[s][y]);
+    }
+  }
+  const std::vector<std::string>& violations() const { return log_; }
+  uint64_t operations(int s) const { return epoch_[s]; }
+
+ private:
+  struct Buf {
+    int w = -1;
+    uint64_t we = 0;
+    uint64_t r[2] = {0, 0};
+    std::string wname, rname[2];
+  };
+  bool ordered(int s, int t, uint64_t e) const { return t == s || seen_[s][t] >= e; }
+  void report(const char* kind, const char* what, int s, const void* p, const std::string& other, int t) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s: %s on stream %d touches buffer %p last used by %s on stream %d without "
+                  "an ordering", kind, what, s, p, other.c_str(), t);
+    log_.emplace_back(buf);
+  }
+  uint64_t epoch_[2] = {0, 0};
+  uint64_t seen_[2][2] = {{0, 0}, {0, 0}};  // seen_[s][t]: later ops on s follow t's ops up to this epoch
+  std::unordered_map<const void*, Buf> bufs_;
+  std::vector<std::string> log_;
 };
 
 }  // namespace
@@ -205,6 +298,9 @@ struct irt_ctx {
   std::vector<std::pair<void*, size_t>> allocs;  // device allocations and their sizes
   size_t device_bytes = 0;
   std::string err;
+  std::unique_ptr<StreamAudit> audit;  // IRT_STREAM_AUDIT=1 at irt_create
+  StreamAudit::Snapshot ev_snap[2];    // the SVGF stream at the ev_frame records
+  bool audit_dropped = false;          // the context joined a batch, whose streams the audit does not model
 
   int fail(int code, const std::string& m) {
     err = m;
@@ -221,6 +317,18 @@ struct irt_ctx {
 
 namespace {
 
+// Stream audit hooks (no-ops unless the audit is on).
+int sid(const irt_ctx* c, hipStream_t h) { return h == c->svgf_stream ? 1 : 0; }
+void aud(irt_ctx* c, hipStream_t h, std::initializer_list<const void*> reads,
+         std::initializer_list<const void*> writes, const char* what) {
+  if (c->audit) c->audit->op(sid(c, h), reads, writes, what);
+}
+hipError_t sync_stream(irt_ctx* c, hipStream_t h) {
+  const hipError_t e = hipStreamSynchronize(h);
+  if (e == hipSuccess && c->audit) c->audit->host_sync(sid(c, h));
+  return e;
+}
+
 // Bytes dev_alloc allocates for a request (never zero).
 size_t alloc_size(size_t bytes) { return bytes ? bytes : 256; }
 
@@ -228,8 +336,8 @@ size_t alloc_size(size_t bytes) { return bytes ? bytes : 256; }
 // used the context's buffers is included: after every step the context's
 // streams wait for the step's events.
 int sync_all(irt_ctx* c) {
-  HIP_CHECK(c, hipStreamSynchronize(c->svgf_stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->svgf_stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   return IRT_OK;
 }
 
@@ -247,7 +355,8 @@ int dev_alloc(irt_ctx* c, void** p, size_t bytes) {
   c->device_bytes += bytes;
   e = hipMemsetAsync(*p, 0, bytes, c->stream);
   if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
-  e = hipStreamSynchronize(c->stream);
+  aud(c, c->stream, {}, {*p}, "allocation zero-fill");
+  e = sync_stream(c, c->stream);
   if (e != hipSuccess) return c->fail(IRT_ERR_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
   return IRT_OK;
 }
@@ -301,7 +410,10 @@ void bind_slots(irt_ctx* c) {
 // (the G-buffer slot of two frames back, the other guide and noisy-radiance
 // slots): wait for the SVGF work of the frame before the previous one.
 int wait_for_slots(irt_ctx* c) {
-  if (c->frames_ended >= 2) HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_frame[c->frames_ended % 2], 0));
+  if (c->frames_ended >= 2) {
+    HIP_CHECK(c, hipStreamWaitEvent(c->stream, c->ev_frame[c->frames_ended % 2], 0));
+    if (c->audit) c->audit->wait(sid(c, c->stream), c->ev_snap[c->frames_ended % 2], 1);
+  }
   return IRT_OK;
 }
 
@@ -321,6 +433,8 @@ int make_current_writable(irt_ctx* c, bool keep) {
     HIP_CHECK(c, hipMemcpyAsync(c->gb_color[w], c->gb_color[f], n * n4, hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(c->gb_params[w], c->gb_params[f], n * p2, hipMemcpyDeviceToDevice, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(c->gb_motion[w], c->gb_motion[f], n * p2, hipMemcpyDeviceToDevice, c->stream));
+    aud(c, c->stream, {c->gb_depth[f], c->gb_normal[f], c->gb_color[f], c->gb_params[f], c->gb_motion[f]},
+        {c->gb_depth[w], c->gb_normal[w], c->gb_color[w], c->gb_params[w], c->gb_motion[w]}, "G-buffer slot copy");
   }
   c->gb_cur = w;
   bind_slots(c);
@@ -331,6 +445,7 @@ int make_current_writable(irt_ctx* c, bool keep) {
 // previous-frame textures): the current slot becomes the previous one.
 int end_frame_slots(irt_ctx* c) {
   HIP_CHECK(c, hipEventRecord(c->ev_frame[c->frames_ended % 2], c->svgf_stream));
+  if (c->audit) c->ev_snap[c->frames_ended % 2] = c->audit->snapshot(1);
   c->frames_ended++;
   c->gb_pp = c->gb_prev;
   c->gb_prev = c->gb_cur;
@@ -342,6 +457,7 @@ int end_frame_slots(irt_ctx* c) {
 int join(irt_ctx* c, hipStream_t from, hipStream_t to) {
   HIP_CHECK(c, hipEventRecord(c->ev_lit, from));
   HIP_CHECK(c, hipStreamWaitEvent(to, c->ev_lit, 0));
+  if (c->audit) c->audit->join(sid(c, from), sid(c, to));
   return IRT_OK;
 }
 
@@ -637,6 +753,7 @@ int ensure_guide(irt_ctx* c) {
                      (const typename P::N4*)c->normal, (const typename P::N4*)c->color,
                      (const typename P::P2*)c->params, c->guide);
   HIP_CHECK(c, hipGetLastError());
+  aud(c, c->stream, {c->depth, c->normal, c->color, c->params}, {c->guide}, "guide records");
   c->guide_dirty = false;
   return IRT_OK;
 }
@@ -663,7 +780,7 @@ int ensure_grid(irt_ctx* c) {
   }
   const size_t total = (size_t)c->grid.nx * c->grid.ny * c->grid.nz;
   if (c->cells_cap < total) {
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
     int rc = dev_free(c, c->cells);
     if (rc) return rc;
     c->cells = nullptr;
@@ -719,6 +836,8 @@ int run_lighting(irt_ctx* c) {
     a.guide = c->guide;
     a.sc.grid = c->grid;
     LAUNCH(c, k_reservoirs_a, P, a, c->direct[0], c->indirect[0]);
+    aud(c, c->stream, {c->guide, c->motion, c->prev_depth, c->prev_normal, c->direct[2], c->indirect[2], c->ring},
+        {c->direct[0], c->direct[1], c->indirect[0], c->indirect[1]}, "candidates + temporal reuse");
     timer_end(c);
     LightArgs<P> b = light_args<P>(c, 1, 2, 0);
     b.guide = c->guide;
@@ -727,12 +846,16 @@ int run_lighting(irt_ctx* c) {
       // scene beyond the LDS staging limits: one kernel per pixel
       timer_begin(c, "reservoirs_b_resolve");
       LAUNCH(c, k_reservoirs_b, P, b);
+      aud(c, c->stream, {c->direct[1], c->indirect[1], c->guide, c->motion, c->ring},
+          {c->direct[2], c->indirect[2], c->direct_lighting, c->lighting}, "spatial reuse + resolves");
       timer_end(c);
       return IRT_OK;
     }
     timer_begin(c, "spatial_setup");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->queue, 16u);
     LAUNCH(c, k_spatial_setup, P, b, c->resolve_rec, c->rays);
+    aud(c, c->stream, {c->direct[1], c->indirect[1], c->guide, c->ring},
+        {c->direct[2], c->indirect[2], c->resolve_rec, c->rays, c->queue}, "spatial reuse + ray set-up");
     timer_end(c);
     timer_begin(c, "march");
     if (mixed(c))
@@ -745,11 +868,14 @@ int run_lighting(irt_ctx* c) {
       hipLaunchKernelGGL((k_march<false, IRT_MARCH_EXACT>), dim3(c->march_blocks), dim3(256), 0, c->stream, b.sc,
                          c->frame, c->n_meshes, c->rays, (uint32_t)(2 * c->npx), c->queue, c->vis, c->counters);
     HIP_CHECK(c, hipGetLastError());
+    aud(c, c->stream, {c->rays, c->queue}, {c->vis, c->queue}, "march");
     timer_end(c);
     timer_begin(c, "resolve_combine");
     hipLaunchKernelGGL((k_resolve_combine<P>), grid_of(c), dim3(TILE, TILE, 1), 0, c->stream, b, c->resolve_rec,
                        c->vis);
     HIP_CHECK(c, hipGetLastError());
+    aud(c, c->stream, {c->resolve_rec, c->vis, c->guide, c->motion}, {c->direct_lighting, c->lighting},
+        "resolve combine");
     timer_end(c);
     return IRT_OK;
   }
@@ -760,6 +886,10 @@ int run_lighting(irt_ctx* c) {
     timer_begin(c, names[p]);
     rc = lighting_pass<P>(c, p);
     if (rc) return rc;
+    aud(c, c->stream, {c->depth, c->normal, c->color, c->params, c->motion, c->prev_depth, c->prev_normal,
+                       c->direct[0], c->direct[1], c->direct[2], c->indirect[0], c->indirect[1], c->indirect[2], c->ring},
+        {c->direct[0], c->direct[1], c->direct[2], c->indirect[0], c->indirect[1], c->indirect[2], c->direct_lighting,
+         c->lighting}, names[p]);
     timer_end(c);
   }
   return IRT_OK;
@@ -768,6 +898,8 @@ int run_lighting(irt_ctx* c) {
 template <class P>
 int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u, old = 1u - cur;
+  aud(c, c->stream, {c->lighting, c->feedback[old], c->moments[old], guide, c->depth, c->normal, c->color, c->params,
+                     c->motion, c->prev_depth, c->prev_normal}, {c->temporal, c->moments[cur]}, "SVGF temporal");
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.old_radiance = (const typename P::R4*)c->feedback[old];
@@ -791,6 +923,8 @@ int svgf_temporal_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
 template <class P>
 int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u;
+  aud(c, c->stream, {c->temporal, c->moments[cur], guide, c->depth, c->normal, c->color, c->params},
+      {c->scratch[0]}, "SVGF variance");
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.input = (const typename P::R4*)c->temporal;
@@ -811,6 +945,7 @@ int svgf_variance_pass(irt_ctx* c, const GuidePx* guide = nullptr) {
 template <class P>
 int svgf_atrous_pass(irt_ctx* c, uint32_t i, void* in, void* out, void* out2, const GuidePx* guide = nullptr) {
   const uint32_t cur = c->frame.size[2] % 2u;
+  aud(c, c->stream, {in, guide, c->depth, c->normal, c->color, c->params}, {out, out2}, "SVGF a-trous");
   SvgfArgs<P> a = svgf_args<P>(c);
   a.guide = guide;
   a.input = (const typename P::R4*)in;
@@ -874,6 +1009,7 @@ int run_svgf_on(irt_ctx* c, uint32_t iterations, uint32_t debug) {
   if (debug >= 3) {
     timer_begin(c, "svgf");
     HIP_CHECK(c, hipMemcpyAsync(c->history, c->lighting, rb, hipMemcpyDeviceToDevice, c->stream));
+    aud(c, c->stream, {c->lighting}, {c->history}, "SVGF bypass copy");
     timer_end(c);
     return IRT_OK;
   }
@@ -920,10 +1056,13 @@ int run_svgf_on(irt_ctx* c, uint32_t iterations, uint32_t debug) {
   for (uint32_t i = 0; i < iterations; ++i) {
     rc = svgf_atrous_pass<P>(c, i, c->scratch[i % 2], c->scratch[1 - i % 2], nullptr);
     if (rc) return rc;
-    if (i == 0)
+    if (i == 0) {
       HIP_CHECK(c, hipMemcpyAsync(c->feedback[cur], c->scratch[1], rb, hipMemcpyDeviceToDevice, c->stream));
+      aud(c, c->stream, {c->scratch[1]}, {c->feedback[cur]}, "SVGF feedback copy");
+    }
   }
   HIP_CHECK(c, hipMemcpyAsync(c->history, c->scratch[iterations % 2], rb, hipMemcpyDeviceToDevice, c->stream));
+  aud(c, c->stream, {c->scratch[iterations % 2]}, {c->history}, "SVGF history copy");
   timer_end(c);
   return IRT_OK;
 }
@@ -962,6 +1101,7 @@ int ingest(irt_ctx* c, const float* depth, const uint16_t* n, const uint16_t* co
                      prm, mot, c->depth, (typename P::N4*)c->normal, (typename P::N4*)c->color,
                      (typename P::P2*)c->params, (typename P::P2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
+  aud(c, c->stream, {n, col, prm, mot}, {c->depth, c->normal, c->color, c->params, c->motion}, "G-buffer ingest");
   return IRT_OK;
 }
 
@@ -1166,6 +1306,8 @@ int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   irt_ctx* c = new irt_ctx();
   c->s = *s;
   c->precision = precision;
+  const char* audit_env = std::getenv("IRT_STREAM_AUDIT");
+  if (audit_env && audit_env[0] == '1') c->audit.reset(new StreamAudit());
   (void)hipGetDevice(&c->device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->svgf_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1233,7 +1375,7 @@ int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, u
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   if (!texels || w < 2 || h < 2 || d < 2) return c->fail(IRT_ERR_INVALID_ARGUMENT, "invalid SDF atlas");
   const size_t count = (size_t)w * h * d;
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   for (void** slot : {(void**)&c->atlas, (void**)&c->atlas16}) {
     if (*slot) {
       int rc = dev_free(c, *slot);
@@ -1249,12 +1391,12 @@ int irt_set_sdf_atlas(irt_ctx* c, const float* texels, uint32_t w, uint32_t h, u
     int rc = dev_alloc(c, (void**)&c->atlas16, count * 2);
     if (rc) return rc;
     HIP_CHECK(c, hipMemcpyAsync(c->atlas16, h16.data(), count * 2, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
   } else {
     int rc = dev_alloc(c, (void**)&c->atlas, count * 4);
     if (rc) return rc;
     HIP_CHECK(c, hipMemcpyAsync(c->atlas, texels, count * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
   }
   c->atlas_dims[0] = w;
   c->atlas_dims[1] = h;
@@ -1278,7 +1420,7 @@ int irt_set_meshes(irt_ctx* c, const irt_mesh* meshes, uint32_t count) {
   int rc = ensure_array(c, &c->meshes, &c->cap_meshes, count);
   if (rc) return rc;
   HIP_CHECK(c, hipMemcpyAsync(c->meshes, meshes, sizeof(irt_mesh) * count, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   c->n_meshes = count;
   c->host_meshes.assign(meshes, meshes + count);
   c->have_geometry = false;  // the triangle hierarchies follow the mesh records
@@ -1320,7 +1462,7 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   c->host_nodes.clear();
   HIP_CHECK(c, hipMemcpyAsync(c->objects, objects, sizeof(irt_object) * count, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(c, hipMemcpyAsync(c->nodes, nodes, sizeof(irt_node) * node_count, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   c->n_objects = count;
   c->n_nodes = node_count;
   c->max_mesh_ref = max_ref;
@@ -1339,7 +1481,7 @@ int irt_set_objects(irt_ctx* c, const irt_object* objects, uint32_t count, const
   if (rc) return rc;
   HIP_CHECK(c, hipMemcpyAsync(c->obj_boxes, boxes.data(), sizeof(float4) * boxes.size(), hipMemcpyHostToDevice,
                               c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   {
     const float margin = 1e-3f;
     float lo[3], ext = 0.f, e[3];
@@ -1370,7 +1512,7 @@ int irt_set_lights(irt_ctx* c, const irt_light* lights, uint32_t count) {
   if (rc) return rc;
   if (count) {
     HIP_CHECK(c, hipMemcpyAsync(c->lights, lights, sizeof(irt_light) * count, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
   }
   c->n_lights = count;
   return IRT_OK;
@@ -1384,8 +1526,8 @@ int irt_copy_scene(irt_ctx* dst, irt_ctx* src) {
     return dst->fail(IRT_ERR_STATE, "source context has no complete scene");
   int rc = ensure_grid(src);
   if (rc) return dst->fail(rc, std::string("source scene: ") + src->err);
-  HIP_CHECK(dst, hipStreamSynchronize(src->stream));
-  HIP_CHECK(dst, hipStreamSynchronize(dst->stream));
+  HIP_CHECK(dst, sync_stream(src, src->stream));
+  HIP_CHECK(dst, sync_stream(dst, dst->stream));
   const size_t texels = (size_t)src->atlas_dims[0] * src->atlas_dims[1] * src->atlas_dims[2];
   const size_t tb = src->atlas16 ? 2 : 4;
   void** slot = src->atlas16 ? (void**)&dst->atlas16 : (void**)&dst->atlas;
@@ -1500,8 +1642,10 @@ int irt_reset(irt_ctx* c) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   int rc = sync_all(c);
   if (rc) return rc;
-  for (const auto& a : view_alloc_plan(c, mixed(c)))
+  for (const auto& a : view_alloc_plan(c, mixed(c))) {
     HIP_CHECK(c, hipMemsetAsync(*a.first, 0, alloc_size(a.second), c->stream));
+    aud(c, c->stream, {}, {*a.first}, "irt_reset clear");
+  }
   const uint32_t n = (uint32_t)c->npx, one = 0x3f800000u;
   for (int k = 0; k < 3; ++k)
     hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, (uint32_t*)c->gb_depth[k], one, n);
@@ -1561,12 +1705,14 @@ int irt_set_gbuffer(irt_ctx* c, const float* depth, const uint16_t* normal, cons
   const size_t n = c->npx;
   c->guide_dirty = true;
   HIP_CHECK(c, hipMemcpyAsync(c->depth, depth, n * 4, hipMemcpyHostToDevice, c->stream));
+  aud(c, c->stream, {}, {c->depth, c->normal, c->color, c->params, c->motion, c->stage[0], c->stage[1], c->stage[2],
+                         c->stage[3]}, "G-buffer upload");
   if (mixed(c)) {
     HIP_CHECK(c, hipMemcpyAsync(c->normal, normal, n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(c->color, color, n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(c->params, params, n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(c, hipMemcpyAsync(c->motion, motion, n * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
     return IRT_OK;
   }
   HIP_CHECK(c, hipMemcpyAsync(c->stage[0], normal, n * 8, hipMemcpyHostToDevice, c->stream));
@@ -1575,7 +1721,7 @@ int irt_set_gbuffer(irt_ctx* c, const float* depth, const uint16_t* normal, cons
   HIP_CHECK(c, hipMemcpyAsync(c->stage[3], motion, n * 4, hipMemcpyHostToDevice, c->stream));
   int rc = ingest<ProfileF32>(c, c->depth, c->stage[0], c->stage[1], c->stage[2], c->stage[3]);
   if (rc) return rc;
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   return IRT_OK;
 }
 
@@ -1613,6 +1759,8 @@ int irt_set_gbuffer_device(irt_ctx* c, const float* depth, const uint16_t* norma
       for (int k = 0; k < 5; ++k)
         HIP_CHECK(c, hipMemcpyAsync(dst[k], src[k], bytes[k], hipMemcpyDeviceToDevice, c->stream));
     }
+    aud(c, c->stream, {depth, normal, color, params, motion}, {c->depth, c->normal, c->color, c->params, c->motion},
+        "G-buffer ingest");
     return IRT_OK;
   }
   return ingest<ProfileF32>(c, depth, normal, color, params, motion);
@@ -1625,7 +1773,8 @@ static int set_vpls_impl(irt_ctx* c, const irt_vpl* v, uint32_t count, hipMemcpy
   if (!c->have_frame) return c->fail(IRT_ERR_STATE, "set the frame before its VPL pool");
   const uint32_t slot = c->frame.size[2] % c->slots;
   HIP_CHECK(c, hipMemcpyAsync(c->ring + (size_t)slot * c->V, v, sizeof(irt_vpl) * c->V, kind, c->stream));
-  if (kind == hipMemcpyHostToDevice) HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  aud(c, c->stream, {kind == hipMemcpyDeviceToDevice ? (const void*)v : nullptr}, {c->ring}, "VPL pool upload");
+  if (kind == hipMemcpyHostToDevice) HIP_CHECK(c, sync_stream(c, c->stream));
   return IRT_OK;
 }
 int irt_set_vpls(irt_ctx* c, const irt_vpl* v, uint32_t count) { return set_vpls_impl(c, v, count, hipMemcpyHostToDevice); }
@@ -1686,7 +1835,7 @@ int irt_set_geometry(irt_ctx* c, const irt_vertex* vertices, uint32_t vertex_cou
                                 c->stream));
   HIP_CHECK(c, hipMemcpyAsync(c->mesh_nodes, ranges.data(), sizeof(uint2) * ranges.size(), hipMemcpyHostToDevice,
                               c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   c->n_vertices = vertex_count;
   c->n_indices = index_count;
   c->n_materials = material_count;
@@ -1715,7 +1864,7 @@ int irt_set_textures(irt_ctx* c, const uint8_t* rgba8, uint32_t size, uint32_t l
     return c->fail(IRT_ERR_LIMIT, "texture array too large");
   const size_t texels = (size_t)size * size * layers;
   if (c->cap_tex < texels) {
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
     int rc = dev_free(c, c->tex);
     c->tex = nullptr;
     c->cap_tex = 0;
@@ -1732,7 +1881,7 @@ int irt_set_textures(irt_ctx* c, const uint8_t* rgba8, uint32_t size, uint32_t l
   }
   HIP_CHECK(c, hipMemcpyAsync(c->tex, rgba8, texels * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(c, hipMemcpyAsync(c->srgb_lut, lut, sizeof(lut), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   c->tex_size = size;
   c->tex_layers = layers;
   return IRT_OK;
@@ -1779,6 +1928,7 @@ int irt_generate_vpls(irt_ctx* c) {
   }
   HIP_CHECK(c, hipGetLastError());
   timer_end(c);
+  aud(c, c->stream, {}, {c->ring}, "VPL pass");
   return IRT_OK;
 }
 
@@ -1815,7 +1965,7 @@ int irt_render_gbuffer(irt_ctx* c) {
   if (base != c->host_tri_base || c->cap_obj_tri_base == 0) {
     HIP_CHECK(c, hipMemcpyAsync(c->obj_tri_base, base.data(), sizeof(uint32_t) * base.size(), hipMemcpyHostToDevice,
                                 c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, sync_stream(c, c->stream));
     c->host_tri_base = base;
   }
   rc = make_current_writable(c, false);
@@ -1851,6 +2001,7 @@ int irt_render_gbuffer(irt_ctx* c) {
                        c->materials, texture_view(c), c->gb_vis, W, H, c->depth, (float4*)c->normal, (float4*)c->color,
                        (float2*)c->params, (float2*)c->motion);
   HIP_CHECK(c, hipGetLastError());
+  aud(c, c->stream, {}, {c->depth, c->normal, c->color, c->params, c->motion}, "G-buffer pass");
   timer_end(c);
   return IRT_OK;
 }
@@ -1865,10 +2016,12 @@ int irt_present_device(irt_ctx* c, void* rgba8) {
     float t[SRGB_THRESHOLDS];
     irt_host::srgb_thresholds(t);
     HIP_CHECK(c, hipMemcpyAsync(c->srgb_t, t, sizeof(t), hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    aud(c, c->stream, {}, {c->srgb_t}, "sRGB threshold upload");
+    HIP_CHECK(c, sync_stream(c, c->stream));
   }
   // after the frame's SVGF, on the stream that runs it
   hipStream_t s = c->schedule == IRT_SCHEDULE_FUSED ? c->svgf_stream : c->stream;
+  aud(c, s, {c->history, c->srgb_t}, {rgba8}, "presentation");
   const uint32_t n = (uint32_t)c->npx;
   if (mixed(c))
     hipLaunchKernelGGL(k_present<ProfileMixed>, dim3((n + 255) / 256), dim3(256), 0, s, (const h4*)c->history, n,
@@ -1937,6 +2090,104 @@ int irt_bake_sdf_noalloc(const float* positions, uint32_t stride_floats, uint32_
   return bake_finish(e1, flags, lipschitz);
 }
 
+int irt_audit_report(irt_ctx* c, char* out, size_t capacity) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  if (!c->audit)
+    return c->fail(IRT_ERR_STATE, c->audit_dropped ? "stream audit stopped: the context joined a batch"
+                                                   : "stream audit is off (set IRT_STREAM_AUDIT=1 before irt_create)");
+  const auto& v = c->audit->violations();
+  if (out && capacity) {
+    std::string text;
+    for (const auto& line : v) text += line + "\n";
+    text += "audited " + std::to_string(c->audit->operations(0)) + " operations on the lighting stream and " +
+            std::to_string(c->audit->operations(1)) + " on the SVGF stream\n";
+    const size_t n = std::min(text.size(), capacity - 1);
+    std::memcpy(out, text.data(), n);
+    out[n] = '\0';
+  }
+  return (int)std::min<size_t>(v.size(), 0x7fffffff);
+}
+
+int irt_host_audit_selftest(void) {
+  // Scripted schedules with known answers; returns 0, or the number of the
+  // first case the audit judges wrongly.
+  int x[8];
+  const void* A = &x[0];
+  const void* B = &x[1];
+  const void* C = &x[2];
+  const void* D = &x[3];
+  const void* E = &x[4];
+  const void* F = &x[5];
+  const void* G = &x[6];
+  struct Case {
+    int expect;
+    void (*run)(StreamAudit&, const void* const*);
+  };
+  const void* bufs[7] = {A, B, C, D, E, F, G};
+  const Case cases[] = {
+      // 1: a write on stream 0 read on stream 1 without a join
+      {1, [](StreamAudit& a, const void* const* b) { a.op(0, {}, {b[0]}, "w"); a.op(1, {b[0]}, {}, "r"); }},
+      // 2: the same with a join
+      {0, [](StreamAudit& a, const void* const* b) {
+         a.op(0, {}, {b[0]}, "w");
+         a.join(0, 1);
+         a.op(1, {b[0]}, {}, "r");
+       }},
+      // 3: a read on stream 1 overwritten on stream 0 without waiting for it
+      {1, [](StreamAudit& a, const void* const* b) { a.op(1, {b[2]}, {}, "r"); a.op(0, {}, {b[2]}, "w"); }},
+      // 4: the same after an event recorded behind the read and waited for
+      {0, [](StreamAudit& a, const void* const* b) {
+         a.op(1, {b[3]}, {}, "r");
+         const StreamAudit::Snapshot s = a.snapshot(1);
+         a.wait(0, s, 1);
+         a.op(0, {}, {b[1]}, "w other");
+         a.op(0, {}, {b[3]}, "w");
+       }},
+      // 4b: a second read enqueued after the record is not covered by the wait
+      {1, [](StreamAudit& a, const void* const* b) {
+         a.op(1, {b[3]}, {}, "r");
+         const StreamAudit::Snapshot s = a.snapshot(1);
+         a.op(1, {b[3]}, {}, "r2");
+         a.wait(0, s, 1);
+         a.op(0, {}, {b[3]}, "w");
+       }},
+      // 5: an event recorded before the read does not order the overwrite
+      {1, [](StreamAudit& a, const void* const* b) {
+         const StreamAudit::Snapshot s = a.snapshot(1);
+         a.op(1, {b[3]}, {}, "r");
+         a.wait(0, s, 1);
+         a.op(0, {}, {b[3]}, "w");
+       }},
+      // 6: a host wait for stream 0 orders its write before stream 1
+      {0, [](StreamAudit& a, const void* const* b) {
+         a.op(0, {}, {b[4]}, "w");
+         a.host_sync(0);
+         a.op(1, {b[4]}, {}, "r");
+       }},
+      // 7: a zero-fill on stream 0 and a store on stream 1 into the same buffer
+      {1, [](StreamAudit& a, const void* const* b) {
+         a.op(0, {}, {b[5]}, "fill");
+         a.op(1, {b[6]}, {b[5]}, "store");
+       }},
+      // 8: ordering carried through a third party: 0 -> 1 joined, 1 -> 0 joined
+      {0, [](StreamAudit& a, const void* const* b) {
+         a.op(0, {}, {b[0]}, "w0");
+         a.join(0, 1);
+         a.op(1, {b[0]}, {b[1]}, "rw1");
+         a.join(1, 0);
+         a.op(0, {b[1]}, {b[0]}, "rw0");
+       }},
+  };
+  int k = 0;
+  for (const Case& cs : cases) {
+    ++k;
+    StreamAudit a;
+    cs.run(a, bufs);
+    if ((int)a.violations().size() != cs.expect) return k;
+  }
+  return 0;
+}
+
 int irt_lighting(irt_ctx* c) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   return mixed(c) ? run_lighting<ProfileMixed>(c) : run_lighting<ProfileF32>(c);
@@ -1981,7 +2232,7 @@ int irt_read(irt_ctx* c, uint32_t b, void* dst, size_t bytes) {
   if (bytes != d.count * external_bytes(d.fmt)) return c->fail(IRT_ERR_INVALID_ARGUMENT, "read size mismatch");
   std::vector<uint8_t> tmp(d.count * internal_bytes(d.fmt));
   HIP_CHECK(c, hipMemcpyAsync(tmp.data(), d.ptr, tmp.size(), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   to_external(d.fmt, tmp.data(), (uint8_t*)dst, d.count);
   return IRT_OK;
 }
@@ -2000,7 +2251,8 @@ int irt_write(irt_ctx* c, uint32_t b, const void* src, size_t bytes) {
   to_internal(d.fmt, (const uint8_t*)src, tmp.data(), d.count);
   if (b == IRT_BUF_DEPTH || b == IRT_BUF_NORMAL || b == IRT_BUF_COLOR || b == IRT_BUF_PARAMS) c->guide_dirty = true;
   HIP_CHECK(c, hipMemcpyAsync(d.ptr, tmp.data(), tmp.size(), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  aud(c, c->stream, {}, {d.ptr}, "irt_write");
+  HIP_CHECK(c, sync_stream(c, c->stream));
   return IRT_OK;
 }
 
@@ -2053,7 +2305,7 @@ int irt_read_counters(irt_ctx* c, uint64_t counts[IRT_COUNTER_COUNT]) {
   int rc = join(c, c->svgf_stream, c->stream);
   if (rc) return rc;
   HIP_CHECK(c, hipMemcpyAsync(tmp, c->counters, sizeof(tmp), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  HIP_CHECK(c, sync_stream(c, c->stream));
   for (int i = 0; i < IRT_COUNTER_COUNT; ++i) counts[i] = tmp[i];
   return IRT_OK;
 }

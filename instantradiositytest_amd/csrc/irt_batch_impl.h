@@ -587,6 +587,13 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
   b->tiles_used[f] = true;
   b->tflip = 1 - f;
   b->views.assign(views, views + count);
+  // The stream audit models a context's own two streams; batched steps run
+  // on the batch's streams, so a context in a batch leaves the audit.
+  for (irt_ctx* v : b->views)
+    if (v->audit) {
+      v->audit.reset();
+      v->audit_dropped = true;
+    }
   b->view_shape.resize(count);
   for (uint32_t v = 0; v < count; ++v)
     b->view_shape[v] = {views[v]->W, views[v]->H, views[v]->s.svgf_iterations};

@@ -250,3 +250,10 @@ def test_binary16_conversion_round_to_nearest_even():
     assert np.array_equal(ref, got)
     back = np.array([L.oracle_f16_to_f32(int(b)) for b in ref], np.float32)
     assert np.array_equal(back, ref.view(np.float16).astype(np.float32))
+
+
+def test_stream_audit_judges_scripted_schedules():
+    """The stream-ordering audit on scripted two-stream schedules (no GPU):
+    unordered read-after-write, write-after-read and write-after-write are
+    flagged, joins, event waits and host synchronisations order them."""
+    assert _lib.irt().irt_host_audit_selftest() == 0
