@@ -656,6 +656,7 @@ void timer_end(irt_ctx* c) {
 
 // ------------------------------------------------------------- launches
 dim3 grid_of(const irt_ctx* c) { return dim3((c->W + TILE - 1) / TILE, (c->H + TILE - 1) / TILE, 1); }
+dim3 grid_banded(const irt_ctx* c) { return dim3(XCDS * band_width(c->W) * ((c->H + TILE - 1) / TILE), 1, 1); }
 
 template <class P>
 LightArgs<P> light_args(irt_ctx* c, int in, int out, int hist) {
@@ -713,14 +714,15 @@ SvgfArgs<P> svgf_args(irt_ctx* c) {
     HIP_CHECK(c, hipGetLastError());                                                                     \
   } while (0)
 
-#define LAUNCH(c, kern, P, ...)                                                                          \
+#define LAUNCH_GRID(c, grid, kern, P, ...)                                                               \
   do {                                                                                                   \
     if ((c)->counting)                                                                                   \
-      hipLaunchKernelGGL((kern<P, true>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+      hipLaunchKernelGGL((kern<P, true>), grid, dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__);       \
     else                                                                                                 \
-      hipLaunchKernelGGL((kern<P, false>), grid_of(c), dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__); \
+      hipLaunchKernelGGL((kern<P, false>), grid, dim3(TILE, TILE, 1), 0, (c)->stream, __VA_ARGS__);      \
     HIP_CHECK(c, hipGetLastError());                                                                     \
   } while (0)
+#define LAUNCH(c, kern, P, ...) LAUNCH_GRID(c, grid_of(c), kern, P, __VA_ARGS__)
 
 int check_ready(irt_ctx* c) {
   if (!c->have_frame) return c->fail(IRT_ERR_STATE, "no frame set (irt_set_frame)");
@@ -853,7 +855,7 @@ int run_lighting(irt_ctx* c) {
     }
     timer_begin(c, "spatial_setup");
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->queue, 16u);
-    LAUNCH(c, k_spatial_setup, P, b, c->resolve_rec, c->rays);
+    LAUNCH_GRID(c, grid_banded(c), k_spatial_setup, P, b, c->resolve_rec, c->rays);
     aud(c, c->stream, {c->direct[1], c->indirect[1], c->guide, c->ring},
         {c->direct[2], c->indirect[2], c->resolve_rec, c->rays, c->queue}, "spatial reuse + ray set-up");
     timer_end(c);

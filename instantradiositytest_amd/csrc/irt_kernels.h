@@ -29,6 +29,27 @@ __device__ __forceinline__ Px pixel(uint32_t W, uint32_t H) {
   return p;
 }
 
+// Tiles in column bands, one band per XCD.  The dispatcher deals the
+// workgroups of a launch to the 8 XCDs in turn (workgroup L runs on XCD
+// L % 8), so workgroup L takes a tile of band L % 8: XCD k walks the tile
+// columns [k*bw, (k+1)*bw) row by row.  The neighbours a pixel gathers (up
+// to 12 px away) then sit mostly in tiles of the same XCD, whose L2 already
+// holds their lines.  The launch is one-dimensional: XCDS * bw * tiles_y
+// workgroups of TILE x TILE threads.
+constexpr uint32_t XCDS = 8;
+__host__ __device__ __forceinline__ uint32_t band_width(uint32_t W) { return ((W + TILE - 1) / TILE + XCDS - 1) / XCDS; }
+__device__ __forceinline__ Px pixel_banded(uint32_t W, uint32_t H) {
+  const uint32_t bw = band_width(W);
+  const uint32_t j = blockIdx.x / XCDS;
+  const uint32_t tx = (blockIdx.x % XCDS) * bw + j % bw, ty = j / bw;
+  Px p;
+  p.x = (int32_t)(tx * TILE + threadIdx.x);
+  p.y = (int32_t)(ty * TILE + threadIdx.y);
+  p.active = (uint32_t)p.x < W && (uint32_t)p.y < H;
+  p.id = p.active ? (uint32_t)p.y * W + (uint32_t)p.x : 0u;
+  return p;
+}
+
 template <class P>
 __device__ __forceinline__ bool valid_surface(const LightArgs<P>& a, const Px& p, Surface* s) {
   bool valid = p.active && (a.guide ? a.guide[p.id].valid != 0u : a.depth[p.id] < 1.0f);
@@ -265,7 +286,7 @@ __device__ __forceinline__ void body_spatial_setup(const LightArgs<P>& a, const 
 }
 template <class P, bool CNT>
 __global__ __launch_bounds__(64) void k_spatial_setup(LightArgs<P> a, ResolveRec* rec, RayReq* reqs) {
-  Px p = pixel(a.F.size[0], a.F.size[1]);
+  Px p = pixel_banded(a.F.size[0], a.F.size[1]);
   Count<CNT> cnt;
   body_spatial_setup(a, p, rec, reqs, cnt);
   cnt.flush(a.counters);
