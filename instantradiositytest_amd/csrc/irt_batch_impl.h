@@ -183,8 +183,10 @@ struct irt_batch {
   hipEvent_t tiles_staged[2] = {nullptr, nullptr};
   bool tiles_used[2] = {false, false};
   int tflip = 0;
-  uint32_t tile_off[3] = {0, 0, 0};
-  uint32_t n_tiles[3] = {0, 0, 0};
+  // tables 0-2: row-major tiles of shape 0-2; tables 3-5: the same tiles in
+  // column bands per XCD (the spatial set-up's order, tile_bands)
+  uint32_t tile_off[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t n_tiles[6] = {0, 0, 0, 0, 0, 0};
   uint32_t at_off[5] = {0, 0, 0, 0, 0};  // a-trous patch tables, iteration i: 16*2^i pixel patches
   uint32_t at_n[5] = {0, 0, 0, 0, 0};
   uint32_t shape[BK_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -281,9 +283,10 @@ void fill_view(irt_ctx* c, BatchView<P>& bv, MarchView& mv) {
 template <int K, class P>
 int launch_kind(irt_batch* b, int q, hipStream_t st, uint32_t iter) {
   const uint32_t sh = b->shape[K];
+  const uint32_t table = K == BK_SETUP ? 3 + sh : sh;  // the set-up gathers neighbours: banded tiles
   const BatchView<P>* views = (const BatchView<P>*)b->d_views[q];
-  const uint32_t n = b->n_tiles[sh];
-  const uint32_t* tiles = b->d_tiles + b->tile_off[sh];
+  const uint32_t n = b->n_tiles[table];
+  const uint32_t* tiles = b->d_tiles + b->tile_off[table];
   if (n == 0) return IRT_OK;
   if (sh == 0)
     hipLaunchKernelGGL((k_batch<K, P, 8, 8>), dim3(n), dim3(64), 0, st, views, tiles, iter);
@@ -521,16 +524,21 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
     }
     b->cap_views = cap;
   }
-  // tile tables: shape s occupies [tile_off[s], tile_off[s] + n_tiles[s]),
+  // tile tables: table t occupies [tile_off[t], tile_off[t] + n_tiles[t]),
   // then the a-trous patch tables of iterations 0..4
   size_t total = 0;
-  uint32_t n[3], na[5];
-  for (int s = 0; s < 3; ++s) {
-    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
+  uint32_t n[6], na[5];
+  for (int t = 0; t < 6; ++t) {
+    const uint32_t tw = kShapes[t % 3][0], th = kShapes[t % 3][1];
     size_t k = 0;
-    for (uint32_t v = 0; v < count; ++v) k += (size_t)((views[v]->W + tw - 1) / tw) * ((views[v]->H + th - 1) / th);
+    for (uint32_t v = 0; v < count; ++v) {
+      const uint32_t nx = (views[v]->W + tw - 1) / tw, ny = (views[v]->H + th - 1) / th;
+      if (t >= 3 && XCDS * band_width_tiles(nx) > kMaxTileCoord)
+        return b->fail(IRT_ERR_LIMIT, "view too wide for the banded tile table");
+      k += (size_t)(t < 3 ? nx : XCDS * band_width_tiles(nx)) * ny;
+    }
     if (k > 0x7fffffffull) return b->fail(IRT_ERR_LIMIT, "too many tiles in one batch");
-    n[s] = (uint32_t)k;
+    n[t] = (uint32_t)k;
     total += k;
   }
   for (uint32_t i = 0; i < 5; ++i) {
@@ -561,15 +569,25 @@ int set_views(irt_batch* b, irt_ctx* const* views, uint32_t count) {
   // the a-trous launches of the last step (SVGF stream) still read the tables
   BHIP(b, hipStreamWaitEvent(b->stream, b->done, 0));
   size_t k = 0;
-  for (int s = 0; s < 3; ++s) {
-    b->tile_off[s] = (uint32_t)k;
-    const uint32_t tw = kShapes[s][0], th = kShapes[s][1];
+  for (int t = 0; t < 6; ++t) {
+    b->tile_off[t] = (uint32_t)k;
+    const uint32_t tw = kShapes[t % 3][0], th = kShapes[t % 3][1];
     for (uint32_t v = 0; v < count; ++v) {
       const uint32_t nx = (views[v]->W + tw - 1) / tw, ny = (views[v]->H + th - 1) / th;
-      for (uint32_t ty = 0; ty < ny; ++ty)
-        for (uint32_t tx = 0; tx < nx; ++tx) h[k++] = (v << 24) | (tx << 12) | ty;
+      if (t < 3) {
+        for (uint32_t ty = 0; ty < ny; ++ty)
+          for (uint32_t tx = 0; tx < nx; ++tx) h[k++] = (v << 24) | (tx << 12) | ty;
+      } else {
+        // entry 8j + x of the view's block (the launch deals it to XCD x, as
+        // every view's block starts at a multiple of 8): tile j of band x, the
+        // band's columns [x*bw, (x+1)*bw) walked row by row; columns past the
+        // view's last are empty tiles whose pixels all fall outside it
+        const uint32_t bw = band_width_tiles(nx);
+        for (uint32_t j = 0; j < bw * ny; ++j)
+          for (uint32_t x = 0; x < XCDS; ++x) h[k++] = (v << 24) | ((x * bw + j % bw) << 12) | (j / bw);
+      }
     }
-    b->n_tiles[s] = n[s];
+    b->n_tiles[t] = n[t];
   }
   for (uint32_t i = 0; i < 5; ++i) {
     b->at_off[i] = (uint32_t)k;

@@ -37,7 +37,8 @@ __device__ __forceinline__ Px pixel(uint32_t W, uint32_t H) {
 // holds their lines.  The launch is one-dimensional: XCDS * bw * tiles_y
 // workgroups of TILE x TILE threads.
 constexpr uint32_t XCDS = 8;
-__host__ __device__ __forceinline__ uint32_t band_width(uint32_t W) { return ((W + TILE - 1) / TILE + XCDS - 1) / XCDS; }
+__host__ __device__ __forceinline__ uint32_t band_width_tiles(uint32_t tiles_x) { return (tiles_x + XCDS - 1) / XCDS; }
+__host__ __device__ __forceinline__ uint32_t band_width(uint32_t W) { return band_width_tiles((W + TILE - 1) / TILE); }
 __device__ __forceinline__ Px pixel_banded(uint32_t W, uint32_t H) {
   const uint32_t bw = band_width(W);
   const uint32_t j = blockIdx.x / XCDS;
