@@ -383,6 +383,12 @@ void* irt_stream_svgf(irt_ctx* ctx);
  * number of entries written. */
 int irt_enable_timing(irt_ctx* ctx, int enable);
 int irt_pass_times(irt_ctx* ctx, char names[][32], float* ms, int capacity);
+/* Timing of one pass only: like irt_enable_timing(ctx, 1), but records just
+ * the launches reported under `pass` (for example "march"), two events per
+ * launch, so a timed run carries almost no timing work; NULL or "" records
+ * every pass.  No counterpart in the reference (its timestamp queries cover
+ * every pass); for callers that time one kernel inside a throughput run. */
+int irt_enable_timing_for(irt_ctx* ctx, const char* pass);
 
 /* Event counters of the next irt_encode / irt_lighting (instrumented
  * kernels; slower).  counts[] order: brdf, node tests, box object

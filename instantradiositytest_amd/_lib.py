@@ -80,6 +80,7 @@ def irt() -> ctypes.CDLL:
         "irt_stream": (c_void_p, [P]),
         "irt_stream_svgf": (c_void_p, [P]),
         "irt_enable_timing": (c_int, [P, c_int]),
+        "irt_enable_timing_for": (c_int, [P, c_char_p]),
         "irt_pass_times": (c_int, [P, P, POINTER(c_float), c_int]),
         "irt_enable_counters": (c_int, [P, c_int]),
         "irt_read_counters": (c_int, [P, POINTER(c_uint64)]),

@@ -293,6 +293,8 @@ struct irt_ctx {
   unsigned long long* counters = nullptr;
   bool counting = false;
   bool timing = false;
+  bool timer_open = false;  // timer_begin recorded a start event for the pass in flight
+  std::string timing_only;  // non-empty: time only the passes reported under this name
   std::vector<Timer> timers;
   size_t n_timers_used = 0;
   std::vector<std::pair<void*, size_t>> allocs;  // device allocations and their sizes
@@ -636,7 +638,9 @@ void to_internal(Fmt f, const uint8_t* in, uint8_t* out, size_t count) {
 
 // ------------------------------------------------------------- timing
 void timer_begin(irt_ctx* c, const char* name) {
-  if (!c->timing) return;
+  c->timer_open = false;
+  if (!c->timing || (!c->timing_only.empty() && c->timing_only != name)) return;
+  c->timer_open = true;
   if (c->n_timers_used >= c->timers.size()) {
     Timer t;
     t.name = name;
@@ -649,7 +653,8 @@ void timer_begin(irt_ctx* c, const char* name) {
   (void)hipEventRecord(t.a, c->stream);
 }
 void timer_end(irt_ctx* c) {
-  if (!c->timing) return;
+  if (!c->timer_open) return;
+  c->timer_open = false;
   (void)hipEventRecord(c->timers[c->n_timers_used].b, c->stream);
   c->n_timers_used++;
 }
@@ -2271,6 +2276,15 @@ void* irt_stream_svgf(irt_ctx* c) { return c ? (void*)c->svgf_stream : nullptr; 
 int irt_enable_timing(irt_ctx* c, int enable) {
   if (!c) return IRT_ERR_INVALID_ARGUMENT;
   c->timing = enable != 0;
+  c->timing_only.clear();
+  c->n_timers_used = 0;
+  return IRT_OK;
+}
+
+int irt_enable_timing_for(irt_ctx* c, const char* pass) {
+  if (!c) return IRT_ERR_INVALID_ARGUMENT;
+  c->timing = true;
+  c->timing_only = pass ? pass : "";
   c->n_timers_used = 0;
   return IRT_OK;
 }

@@ -244,8 +244,13 @@ class Renderer:
         check(self.L.irt_write(self.ctx, BUFFERS[name], ptr(data), nbytes), self.ctx)
 
     # -- timing / counters ---------------------------------------------------
-    def enable_timing(self, on: bool = True):
-        check(self.L.irt_enable_timing(self.ctx, int(on)), self.ctx)
+    def enable_timing(self, on: bool = True, only: str | None = None):
+        """Record pass GPU times (irt_pass_times); `only` names the single pass
+        to time (irt_enable_timing_for)."""
+        if on and only:
+            check(self.L.irt_enable_timing_for(self.ctx, only.encode()), self.ctx)
+        else:
+            check(self.L.irt_enable_timing(self.ctx, int(on)), self.ctx)
         self.timing = on
 
     def pass_times(self) -> dict:
