@@ -427,8 +427,9 @@ struct GridView {
 };
 
 // Scene records as the kernels see them.
-struct SceneView {
-  const irt_object* objects;
+template <class O>
+struct SceneViewOf {
+  const O* objects;
   const irt_mesh* meshes;
   const irt_node* nodes;
   const irt_light* lights;
@@ -437,6 +438,14 @@ struct SceneView {
   GridView grid;
   const _Float16* atlas16;  // mixed profile: the same layout, samples rounded toward -inf (else null)
 };
+using SceneView = SceneViewOf<irt_object>;
+// The parts of an object record that the scene queries read (inverse
+// transform and info: 80 of its 208 bytes), as the march stages them in LDS.
+struct QueryObject {
+  float inverse[16];
+  uint32_t info[4];
+};
+using QuerySceneView = SceneViewOf<QueryObject>;
 
 // signed_box_distance — sdf.wgsl:1-4.
 __device__ __forceinline__ float signed_box_distance(v3 p, v3 lo, v3 hi) {
@@ -445,7 +454,8 @@ __device__ __forceinline__ float signed_box_distance(v3 p, v3 lo, v3 hi) {
 }
 
 // sample_field (manual 8-tap trilinear) — common.wgsl:31-38.
-__device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_t* origin, v3 q, uint32_t res) {
+template <class SV>
+__device__ __forceinline__ float sample_field(const SV& sc, const uint32_t* origin, v3 q, uint32_t res) {
   // cell = min(floor(q), res - 2) with the clamp taken on the float (a
   // median of three: also 0 below, the saturation of the u32 conversion);
   // res - 2 and the floors are exact integers, so cell and f = q - cell are
@@ -484,8 +494,8 @@ __device__ __forceinline__ float sample_field(const SceneView& sc, const uint32_
 }
 
 // mesh_distance — sdf.wgsl:5-33.
-template <class C>
-__device__ __forceinline__ float mesh_distance(const SceneView& sc, v3 p, const irt_mesh& mesh, C& cnt) {
+template <class SV, class C>
+__device__ __forceinline__ float mesh_distance(const SV& sc, v3 p, const irt_mesh& mesh, C& cnt) {
   v3 glo = ld3(mesh.geometry_lo), ghi = ld3(mesh.geometry_hi);
   float geometry_bound = signed_box_distance(p, glo, ghi);
   if (mesh.geometry_lo[3] > 0.5f) {
@@ -511,10 +521,10 @@ __device__ __forceinline__ float mesh_distance(const SceneView& sc, v3 p, const 
 }
 
 // object_distance — sdf.wgsl:35-41.  Returns the distance; *voxel gets the spacing term.
-template <class C>
-__device__ __forceinline__ float object_distance(const SceneView& sc, v3 p, uint32_t id, bool need_spacing,
+template <class SV, class C>
+__device__ __forceinline__ float object_distance(const SV& sc, v3 p, uint32_t id, bool need_spacing,
                                                  float hit_eps, float* voxel, C& cnt) {
-  const irt_object& obj = sc.objects[id];
+  const auto& obj = sc.objects[id];
   const irt_mesh& mesh = sc.meshes[obj.info[0]];
   float4 l = mat_mul(obj.inverse, p.x, p.y, p.z, 1.0f);
   float d = mesh_distance(sc, v3{l.x, l.y, l.z}, mesh, cnt) * __uint_as_float(obj.info[2]);
@@ -533,8 +543,8 @@ struct Distance {
 // objects whose conservative lower bound exceeds the running minimum).  When
 // the candidate grid is present, the same minimum is taken over the cell's
 // candidate list with the same per-object bound test.
-template <class C>
-__device__ __forceinline__ bool scene_distance_grid(const SceneView& sc, float hit_eps, v3 p, bool need_spacing,
+template <class SV, class C>
+__device__ __forceinline__ bool scene_distance_grid(const SV& sc, float hit_eps, v3 p, bool need_spacing,
                                                     uint32_t hint, C& cnt, Distance* out) {
   const GridView& g = sc.grid;
   const float fx = (p.x - g.x0) * g.inv_cell, fy = (p.y - g.y0) * g.inv_cell, fz = (p.z - g.z0) * g.inv_cell;
@@ -569,8 +579,8 @@ __device__ __forceinline__ bool scene_distance_grid(const SceneView& sc, float h
   return true;
 }
 
-template <class C>
-__device__ __forceinline__ Distance scene_distance(const SceneView& sc, uint32_t n_objects, float hit_eps, v3 p,
+template <class SV, class C>
+__device__ __forceinline__ Distance scene_distance(const SV& sc, uint32_t n_objects, float hit_eps, v3 p,
                                                    bool need_spacing, uint32_t hint, C& cnt) {
   if (sc.grid.cells) {
     Distance d;

@@ -40,21 +40,25 @@ constexpr uint32_t MARCH_REFILL = 12;   // idle lanes that trigger a refill
 
 // The persistent march loop over one view's request slots (queue[1] = slots
 // taken).  Every loop iteration runs one scene query per busy lane.
-template <class C>
-__device__ __forceinline__ void march_queue(const SceneView& lsc, const irt_frame& F, const RayReq* reqs,
+//
+// A lane's ray lives in 15 registers: the request's twelve floats are reused
+// as the ray proceeds.  Phase 0 (clearance at the surface point) reads the
+// point (r[0..2]) and normal (r[3..5]) and leaves the ray origin in r[0..2];
+// phase 1 (clearance at a VPL endpoint) reads the target (r[6..8]) and the
+// VPL normal (r[9..11]); the set-up leaves the direction in r[3..5] and the
+// march scalars in the target's and VPL normal's registers (r[6] limit, r[7]
+// t, r[8] result, r[9] previous distance, r[10] step count), which the march
+// steps no longer need.  The arithmetic is sdf.wgsl:61-81's, unchanged.
+template <class SV, class C>
+__device__ __forceinline__ void march_queue(const SV& lsc, const irt_frame& F, const RayReq* reqs,
                                             uint32_t n_rays, uint32_t* queue, float* vis, C& cnt) {
-  // Phase 0: the clearance query at the surface point; phase 1: the
-  // clearance query at a VPL endpoint; phase 2: the march steps
-  // (sdf.wgsl:61-81 in that order).
   const uint32_t lane = __lane_id();
   const float eps0 = F.sdf[1];
   const uint32_t max_steps = f2u_sat(F.sdf[0]);
   bool active = false, exhausted = false;
   uint32_t chunk_next = 0, chunk_end = 0;  // wave-uniform
-  RayReq rq{};
-  MarchRay ray{};
-  MarchState st{};
-  uint32_t phase = 0;
+  float r[12] = {};
+  uint32_t slot = 0u, closest = 0u, phase = 0u;
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -74,7 +78,10 @@ __device__ __forceinline__ void march_queue(const SceneView& lsc, const irt_fram
       if (!active) {
         const uint32_t rank = (uint32_t)__popcll(idle & lanes_below());
         if (rank < take) {
-          rq = reqs[chunk_next + rank];
+          const RayReq rq = reqs[chunk_next + rank];
+          r[0] = rq.px, r[1] = rq.py, r[2] = rq.pz, r[3] = rq.nx, r[4] = rq.ny, r[5] = rq.nz;
+          r[6] = rq.tx, r[7] = rq.ty, r[8] = rq.tz, r[9] = rq.ex, r[10] = rq.ey, r[11] = rq.ez;
+          slot = rq.slot;
           phase = 0;
           active = rq.valid != 0u;
           if (active) cnt.add(C_MARCH, 1);
@@ -91,91 +98,118 @@ __device__ __forceinline__ void march_queue(const SceneView& lsc, const irt_fram
       bool spacing = true;
       uint32_t hint = 0xffffffffu;
       if (phase == 0u) {
-        q = v3{rq.px, rq.py, rq.pz};
+        q = v3{r[0], r[1], r[2]};
       } else if (phase == 1u) {
-        q = v3{rq.tx, rq.ty, rq.tz};
+        q = v3{r[6], r[7], r[8]};
       } else {
-        q = v3{ray.ox, ray.oy, ray.oz} + v3{ray.dx, ray.dy, ray.dz} * st.t;
+        q = v3{r[0], r[1], r[2]} + v3{r[3], r[4], r[5]} * r[7];
         spacing = false;
-        hint = st.closest;
+        hint = closest;
         cnt.add(C_STEP, 1);
       }
       const Distance d = scene_distance(lsc, F.counts[3], eps0, q, spacing, hint, cnt);
       bool setup_done = false;
-      v3 endpoint = v3{rq.tx, rq.ty, rq.tz};
+      v3 endpoint = v3{r[6], r[7], r[8]};
       float v = 0.0f;
       bool done = false;
       if (phase == 0u) {
         const float epsilon = fmaxf(eps0, d.voxel * 0.65f);
-        const v3 origin = v3{rq.px, rq.py, rq.pz} + v3{rq.nx, rq.ny, rq.nz} * epsilon;
-        ray.ox = origin.x;
-        ray.oy = origin.y;
-        ray.oz = origin.z;
-        st.closest = d.object_id;
-        const v3 e = v3{rq.ex, rq.ey, rq.ez};
+        const v3 origin = v3{r[0], r[1], r[2]} + v3{r[3], r[4], r[5]} * epsilon;
+        r[0] = origin.x;
+        r[1] = origin.y;
+        r[2] = origin.z;
+        closest = d.object_id;
+        const v3 e = v3{r[9], r[10], r[11]};
         if (dot(e, e) > 0.5f) {
           phase = 1u;
         } else {
           setup_done = true;
         }
       } else if (phase == 1u) {
-        endpoint = endpoint + v3{rq.ex, rq.ey, rq.ez} * fmaxf(eps0, d.voxel * 0.65f);
+        endpoint = endpoint + v3{r[9], r[10], r[11]} * fmaxf(eps0, d.voxel * 0.65f);
         setup_done = true;
       } else {
         const float h = d.distance;
-        st.closest = d.object_id;
+        closest = d.object_id;
         if (h < eps0) {
           v = 0.0f;
           done = true;
         } else {
-          const float y = h * h / (2.0f * st.previous);
+          const float y = h * h / (2.0f * r[9]);
           const float dd = fsqrt(fmaxf(h * h - y * y, 0.0f));
-          if (st.t > y) st.result = fminf(st.result, F.sdf[2] * dd / (st.t - y));
-          if (st.result < 0.001f) {
+          if (r[7] > y) r[8] = fminf(r[8], F.sdf[2] * dd / (r[7] - y));
+          if (r[8] < 0.001f) {
             v = 0.0f;
             done = true;
           } else {
-            st.previous = h;
-            st.t += h;
-            st.i += 1u;
+            r[9] = h;
+            r[7] += h;
+            r[10] = __uint_as_float(__float_as_uint(r[10]) + 1u);
           }
         }
       }
       if (setup_done) {
-        const v3 delta = endpoint - v3{ray.ox, ray.oy, ray.oz};
+        const v3 delta = endpoint - v3{r[0], r[1], r[2]};
         const float limit = length(delta) - eps0;
         if (limit <= 0.0f) {
           v = 1.0f;
           done = true;
         } else {
           const v3 dir = normalize(delta);
-          ray.dx = dir.x;
-          ray.dy = dir.y;
-          ray.dz = dir.z;
-          ray.limit = limit;
-          st.t = eps0;
-          st.result = 1.0f;
-          st.previous = 1e20f;
-          st.i = 0u;
+          r[3] = dir.x;
+          r[4] = dir.y;
+          r[5] = dir.z;
+          r[6] = limit;
+          r[7] = eps0;                 // t
+          r[8] = 1.0f;                 // result
+          r[9] = 1e20f;                // previous distance
+          r[10] = __uint_as_float(0u);  // steps taken
           phase = 2u;
         }
       }
       // loop-top checks of the march for the next iteration
       if (!done && phase == 2u) {
-        if (st.i >= max_steps) {
+        if (__float_as_uint(r[10]) >= max_steps) {
           v = 0.0f;
           done = true;
-        } else if (st.t >= ray.limit) {
-          v = clampf(st.result, 0.0f, 1.0f);
+        } else if (r[7] >= r[6]) {
+          v = clampf(r[8], 0.0f, 1.0f);
           done = true;
         }
       }
       if (done) {
-        vis[rq.slot] = v;
+        vis[slot] = v;
         active = false;
       }
     }
   }
+}
+
+// Stages the query part of n object records (inverse and info: words 32-51
+// of each 52-word record) into LDS, one word per thread and step.
+__device__ __forceinline__ void stage_query_objects(QueryObject* dst, const irt_object* src, uint32_t n) {
+  constexpr uint32_t QW = sizeof(QueryObject) / 4, OW = sizeof(irt_object) / 4;
+  constexpr uint32_t OFF = offsetof(irt_object, inverse) / 4;
+  static_assert(offsetof(irt_object, info) == offsetof(irt_object, inverse) + 64, "inverse, then info");
+  for (uint32_t i = threadIdx.x; i < n * QW; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(dst)[i] = reinterpret_cast<const uint32_t*>(src)[(i / QW) * OW + OFF + i % QW];
+}
+// The scene view of the march: the global records except objects, meshes
+// and object boxes, which it reads from LDS.
+__device__ __forceinline__ QuerySceneView query_view(const SceneView& sc, const QueryObject* objects,
+                                                     const irt_mesh* meshes, const float4* boxes) {
+  QuerySceneView v;
+  v.objects = objects;
+  v.meshes = meshes;
+  v.nodes = sc.nodes;
+  v.lights = sc.lights;
+  v.atlas = sc.atlas;
+  v.atlas_w = sc.atlas_w;
+  v.atlas_h = sc.atlas_h;
+  v.grid = sc.grid;
+  v.grid.boxes = boxes;
+  v.atlas16 = sc.atlas16;
+  return v;
 }
 
 // Persistent march over the request slots (queue[1] = slots taken).
@@ -187,25 +221,20 @@ __global__ __launch_bounds__(256) void k_march(SceneView sc, irt_frame F, uint32
                                                uint32_t n_rays, uint32_t* queue, float* vis,
                                                unsigned long long* counters) {
   static_assert(M == IRT_MARCH_MATH, "each translation unit instantiates the march of its own arithmetic");
-  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
+  __shared__ QueryObject s_obj[LDS_MAX_OBJECTS];
   __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
   __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
   // The host launches this kernel only when the scene fits the LDS arrays
   // (objects and meshes <= LDS_MAX_OBJECTS) and the candidate grid exists.
   const uint32_t n_obj = F.counts[3];
-  const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
-  for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
-    reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(sc.objects)[i];
+  stage_query_objects(s_obj, sc.objects, n_obj);
   const uint32_t words_mesh = n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
   for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
     reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(sc.meshes)[i];
   for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = sc.grid.boxes[i];
   __syncthreads();
   Count<CNT> cnt;
-  SceneView lsc = sc;
-  lsc.objects = s_obj;
-  lsc.meshes = s_mesh;
-  lsc.grid.boxes = s_box;
+  const QuerySceneView lsc = query_view(sc, s_obj, s_mesh, s_box);
   // Each wave claims chunks of MARCH_CHUNK rays with one atomic and hands
   // them to its idle lanes; lanes refill as soon as MARCH_REFILL of them are
   // idle, so a wave is not held by its longest march.
@@ -228,7 +257,7 @@ struct MarchView {
 template <int M>
 __global__ __launch_bounds__(256) void k_march_views(const MarchView* __restrict__ views, uint32_t n_views) {
   static_assert(M == IRT_MARCH_MATH, "each translation unit instantiates the march of its own arithmetic");
-  __shared__ irt_object s_obj[LDS_MAX_OBJECTS];
+  __shared__ QueryObject s_obj[LDS_MAX_OBJECTS];
   __shared__ irt_mesh s_mesh[LDS_MAX_OBJECTS];
   __shared__ float4 s_box[2 * LDS_MAX_OBJECTS];
   Count<false> cnt;
@@ -236,18 +265,13 @@ __global__ __launch_bounds__(256) void k_march_views(const MarchView* __restrict
     const MarchView& mv = views[v];
     __syncthreads();  // the previous view's readers are done with the LDS records
     const uint32_t n_obj = mv.F.counts[3];
-    const uint32_t words_obj = n_obj * (uint32_t)(sizeof(irt_object) / 4);
-    for (uint32_t i = threadIdx.x; i < words_obj; i += blockDim.x)
-      reinterpret_cast<uint32_t*>(s_obj)[i] = reinterpret_cast<const uint32_t*>(mv.sc.objects)[i];
+    stage_query_objects(s_obj, mv.sc.objects, n_obj);
     const uint32_t words_mesh = mv.n_meshes * (uint32_t)(sizeof(irt_mesh) / 4);
     for (uint32_t i = threadIdx.x; i < words_mesh; i += blockDim.x)
       reinterpret_cast<uint32_t*>(s_mesh)[i] = reinterpret_cast<const uint32_t*>(mv.sc.meshes)[i];
     for (uint32_t i = threadIdx.x; i < 2 * n_obj; i += blockDim.x) s_box[i] = mv.sc.grid.boxes[i];
     __syncthreads();
-    SceneView lsc = mv.sc;
-    lsc.objects = s_obj;
-    lsc.meshes = s_mesh;
-    lsc.grid.boxes = s_box;
+    const QuerySceneView lsc = query_view(mv.sc, s_obj, s_mesh, s_box);
     march_queue(lsc, mv.F, mv.reqs, mv.n_rays, mv.queue, mv.vis, cnt);
   }
 }
