@@ -3,6 +3,8 @@
 // reference's pass schedule) or several per launch (fused schedule).
 #pragma once
 
+#include <type_traits>
+
 #include "irt_device.h"
 
 namespace irt {
@@ -484,6 +486,43 @@ __device__ __forceinline__ float guide_tap_weight(const Guide& g, const GuidePx&
                           v3{(float)r.cr, (float)r.cg, (float)r.cb}, make_float2((float)r.m0, (float)r.m1), stride,
                           inv_tol);
 }
+// The mixed profile's fused SVGF taps.  Its radiance and moments are stored
+// in 16 bits, so the tap arithmetic fuses each product and sum into one
+// multiply-add (one rounding), lets the records' binary16 channels enter the
+// multiply-adds as they are, and evaluates the product of the tap's
+// exponential weights (svgf.wgsl:30-44 plane and colour terms, and the
+// a-trous luminance term) as one exponential of the summed exponents.  A tap
+// whose weight is zero adds zero instead of being skipped: staged radiance
+// and moments are finite.  The FP32 profile keeps the reference's
+// evaluation order (tap_weight).
+template <class P>
+constexpr bool fused_svgf() {
+  return std::is_same<P, ProfileMixed>::value;
+}
+struct TapTerms {
+  float wn;   // normal weight, 0 for records without geometry
+  float arg;  // exponent of the plane and colour weights
+};
+__device__ __forceinline__ float lum_fused(float r, float g, float b) {
+  return fmaf(b, 0.0722f, fmaf(g, 0.7152f, r * 0.2126f));
+}
+__device__ __forceinline__ TapTerms tap_terms_fused(const Guide& g, const GuidePx& r, float inv_tol) {
+  const float nd = fmaf(g.normal.z, (float)r.nz, fmaf(g.normal.y, (float)r.ny, g.normal.x * (float)r.nx));
+  const float plane = fabsf(fmaf(r.pz - g.position.z, g.normal.z,
+                                 fmaf(r.py - g.position.y, g.normal.y, (r.px - g.position.x) * g.normal.x)));
+  const float dr = fmaf(-1.0f, (float)r.cr, g.color.x), dg = fmaf(-1.0f, (float)r.cg, g.color.y);
+  const float db = fmaf(-1.0f, (float)r.cb, g.color.z);
+  const float d0 = fmaf(-1.0f, (float)r.m0, g.m0), d1 = fmaf(-1.0f, (float)r.m1, g.m1);
+  const float lc = __builtin_amdgcn_sqrtf(fmaf(db, db, fmaf(dg, dg, dr * dr)));
+  const float lm = __builtin_amdgcn_sqrtf(fmaf(d1, d1, d0 * d0));
+  return TapTerms{r.valid ? pow64(fmaxf(nd, 0.0f)) : 0.0f, fmaf(-plane, inv_tol, -8.0f * (lc + lm))};
+}
+// wn * exp(arg), 0 for records without geometry (whose terms may not be finite).
+__device__ __forceinline__ float tap_weight_fused(const TapTerms& t, bool valid, float extra_arg) {
+  const float w = t.wn * __expf(t.arg + extra_arg);
+  return valid ? w : 0.0f;
+}
+
 template <class P, class C, bool FAST = false>
 __device__ __forceinline__ float geometry_weight(const SvgfArgs<P>& a, const Guide& g, int32_t qx, int32_t qy,
                                                  float stride, float inv_tol, C& cnt) {
@@ -670,16 +709,37 @@ __device__ __forceinline__ void svgf_temporal_staged(const SvgfArgs<P>& a, int32
     float lo = 1e20f, hi = 0.0f;
     const Guide g = guide_of(F, self);
     const float inv_tol = 1.0f / tap_tolerance(g, 1.0f);
-    for (int32_t j = -1; j <= 1; ++j) {
-      for (int32_t i = -1; i <= 1; ++i) {
-        int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
-        int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
-        cnt.add(C_TAP, 1);
-        const int32_t e = (lj + qy - y) * n + li + qx - x;
-        if (guide_tap_weight<true>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol) > 0.01f) {
-          float v = luminance(xyz(ld4(sr, (uint32_t)e)));
-          lo = fminf(lo, v);
-          hi = fmaxf(hi, v);
+    if constexpr (fused_svgf<P>()) {
+      // one tap in flight at a time (see svgf_atrous_staged)
+#pragma unroll 1
+      for (int32_t j = -1; j <= 1; ++j) {
+#pragma unroll 1
+        for (int32_t i = -1; i <= 1; ++i) {
+          int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+          int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+          cnt.add(C_TAP, 1);
+          const int32_t e = (lj + qy - y) * n + li + qx - x;
+          const GuidePx r = unpack_guide(sg[e][0], sg[e][1]);
+          const float4 smp = ld4(sr, (uint32_t)e);
+          if (tap_weight_fused(tap_terms_fused(g, r, inv_tol), r.valid != 0u, 0.0f) > 0.01f) {
+            const float v = lum_fused(smp.x, smp.y, smp.z);
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+          }
+        }
+      }
+    } else {
+      for (int32_t j = -1; j <= 1; ++j) {
+        for (int32_t i = -1; i <= 1; ++i) {
+          int32_t qx = min(max(x + i, 0), (int32_t)W - 1);
+          int32_t qy = min(max(y + j, 0), (int32_t)H - 1);
+          cnt.add(C_TAP, 1);
+          const int32_t e = (lj + qy - y) * n + li + qx - x;
+          if (guide_tap_weight<true>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol) > 0.01f) {
+            float v = luminance(xyz(ld4(sr, (uint32_t)e)));
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+          }
         }
       }
     }
@@ -715,16 +775,34 @@ __device__ __forceinline__ float4 svgf_variance_staged(const SvgfArgs<P>& a, int
     float s0 = 0.f, s1 = 0.f, total = 0.f;
     const Guide g = guide_of(a.F, self);
     const float inv_tol = FAST ? 1.0f / tap_tolerance(g, 1.0f) : 0.0f;
-    for (int32_t j = -3; j <= 3; ++j) {
-      for (int32_t i = -3; i <= 3; ++i) {
-        cnt.add(C_TAP, 1);
-        const int32_t e = (lj + j) * n + li + i;
-        float w = guide_tap_weight<FAST>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol);
-        if (w <= 0.0f) continue;
-        const float2 mo = sm[e];
-        s0 += mo.x * w;
-        s1 += mo.y * w;
-        total += w;
+    if constexpr (fused_svgf<P>() && FAST) {
+      // one tap in flight at a time (see svgf_atrous_staged)
+#pragma unroll 1
+      for (int32_t j = -3; j <= 3; ++j) {
+#pragma unroll 1
+        for (int32_t i = -3; i <= 3; ++i) {
+          cnt.add(C_TAP, 1);
+          const int32_t e = (lj + j) * n + li + i;
+          const GuidePx r = unpack_guide(sg[e][0], sg[e][1]);
+          const float w = tap_weight_fused(tap_terms_fused(g, r, inv_tol), r.valid != 0u, 0.0f);
+          const float2 mo = sm[e];
+          s0 = fmaf(mo.x, w, s0);
+          s1 = fmaf(mo.y, w, s1);
+          total += w;
+        }
+      }
+    } else {
+      for (int32_t j = -3; j <= 3; ++j) {
+        for (int32_t i = -3; i <= 3; ++i) {
+          cnt.add(C_TAP, 1);
+          const int32_t e = (lj + j) * n + li + i;
+          float w = guide_tap_weight<FAST>(g, unpack_guide(sg[e][0], sg[e][1]), 1.0f, inv_tol);
+          if (w <= 0.0f) continue;
+          const float2 mo = sm[e];
+          s0 += mo.x * w;
+          s1 += mo.y * w;
+          total += w;
+        }
       }
     }
     float t = fmaxf(total, 0.00001f);
@@ -815,6 +893,30 @@ __device__ __forceinline__ float4 svgf_atrous_staged(const SvgfArgs<P>& a, int32
   const int32_t step = (int32_t)a.stride;
   const float inv_tol = 1.0f / tap_tolerance(g, (float)step);
   const float inv_phi = 1.0f / phi;
+  if constexpr (fused_svgf<P>()) {
+    const float lcf = lum_fused(c.x, c.y, c.z);
+    // one tap in flight at a time: unrolled, the branch-free taps would be
+    // scheduled with every tap's loads at once (registers for about 250
+    // values, one wave per SIMD)
+#pragma unroll 1
+    for (int32_t j = -2; j <= 2; ++j) {
+#pragma unroll 1
+      for (int32_t i = -2; i <= 2; ++i) {
+        cnt.add(C_TAP, 1);
+        const int32_t e = (lj + j) * n + li + i;
+        const GuidePx r = unpack_guide(sg[e][0], sg[e][1]);
+        const float4 smp = ld4(sr, (uint32_t)e);
+        const float dl = fabsf(lum_fused(smp.x, smp.y, smp.z) - lcf);
+        const float w = (bspline(i) * bspline(j)) *
+                        tap_weight_fused(tap_terms_fused(g, r, inv_tol), r.valid != 0u, -dl * inv_phi);
+        sum = v3{fmaf(smp.x, w, sum.x), fmaf(smp.y, w, sum.y), fmaf(smp.z, w, sum.z)};
+        v = fmaf(smp.w * smp.w, w * w, v);
+        total += w;
+      }
+    }
+    const float t = fmaxf(total, 0.00001f);
+    return make_float4(sum.x / t, sum.y / t, sum.z / t, fminf(sqrtf(v) / t, 65000.0f));
+  }
   for (int32_t j = -2; j <= 2; ++j) {
     for (int32_t i = -2; i <= 2; ++i) {
       cnt.add(C_TAP, 1);

@@ -64,6 +64,13 @@ class DeviceBuffer:
         _check(lib().hipMemcpy(b.ptr, c_void_p(a.ctypes.data), a.nbytes, H2D), "hipMemcpy H2D")
         return b
 
+    def upload(self, a: np.ndarray):
+        """Blocking copy of a host array into the buffer's first bytes."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().hipMemcpy(self.ptr, c_void_p(a.ctypes.data), a.nbytes, H2D), "hipMemcpy H2D")
+
     def to_array(self, dtype, shape) -> np.ndarray:
         """Blocking copy of the buffer's first bytes into a new host array."""
         out = np.empty(shape, dtype)
