@@ -155,7 +155,10 @@ typedef enum irt_precision {
   /* binary16 G-buffer / radiance / SVGF radiance / direct lighting; SVGF
    * moments as binary16 first moment and age with an FP32 second moment
    * (8 B); binary16 SDF samples rounded toward -inf; FP32 arithmetic,
-   * reservoir W/M sums and march. */
+   * reservoir W/M sums and march.  The march contracts multiply-adds and
+   * uses the hardware division and square root; the fused schedule's SVGF
+   * taps are fused multiply-adds with one exponential per tap.  The
+   * reservoir passes keep the FP32 profile's exact arithmetic. */
   IRT_PRECISION_MIXED = 1
 } irt_precision;
 
