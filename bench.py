@@ -475,7 +475,7 @@ def build_session_scenes(classes: list, frames: int, seed: int) -> list:
 
 
 def run_serving(streams, arrivals, seconds: float, warmup_s: float, sessions: int, tile_shapes=None,
-                max_in_flight: int = 2) -> dict:
+                max_in_flight: int = 2, cadence_hz: float = 60.0) -> dict:
     """The serving simulation; arrival i uses scene i, and arrivals beyond the built scenes reuse them
     cyclically (counted in scenes_reused)."""
     from instantradiositytest_amd.serving import Arrival, Server, simulate
@@ -483,11 +483,11 @@ def run_serving(streams, arrivals, seconds: float, warmup_s: float, sessions: in
     arrivals = [Arrival(a.t, i % n, a.lifetime, a.offset) for i, a in enumerate(arrivals)]
     srv = Server(max_sessions=sessions, precision="mixed", budget_bytes=200e9, max_in_flight=max_in_flight,
                  tile_shapes=tile_shapes)
-    rep = simulate(srv, streams, arrivals, seconds, warmup_s, 60.0, log=log)
+    rep = simulate(srv, streams, arrivals, seconds, warmup_s, cadence_hz, log=log)
     d = rep.as_dict()
     d["workload"] = (f"Poisson arrivals 40/s over {seconds:.0f} s, sessions 640x360/1280x720/1920x1200 at "
                      f"0.4/0.4/0.2, lifetimes U[120,600] frames, <= {sessions} resident sessions, 200 GB budget, "
-                     f"60 Hz clients with one request outstanding, continuous batching ({max_in_flight} steps in "
+                     f"{cadence_hz:g} Hz clients with one request outstanding, continuous batching ({max_in_flight} steps in "
                      f"flight); every session its own seeded scene (8-48 meshes at 64^3-128^3, 500-2000 VPLs, "
                      f"3-5 bounces) with its G-buffer and VPL pool rendered on the device each frame; latency = "
                      f"submit to completion; window after {warmup_s:.0f} s warm-up")
@@ -528,6 +528,8 @@ def main() -> int:
     ap.add_argument("--serve-sweep", type=lambda v: [int(x) for x in v.split(",") if x], default=[4, 8, 16],
                     help="resident-session caps of the short serving runs after the main one")
     ap.add_argument("--sweep-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-cadences", type=lambda v: [float(x) for x in v.split(",") if x], default=[30.0, 24.0, 20.0],
+                    help="client frame rates (Hz) of short serving runs at the full session count")
     ap.add_argument("--only", default="",
                     help="comma list of legs: headline,plain,fused_fp32,vpls,gbuffer,full,bake,batched,serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -672,6 +674,14 @@ def main() -> int:
             sweep[str(n)] = {k: d[k] for k in ("mpix_s", "p50_ms", "p99_ms", "mean_batch", "frames")}
         if sweep:
             line["serving"]["by_sessions"] = sweep
+        # the full session count with slower clients: the rate at which the latency target holds
+        by_rate = {}
+        for hz in args.serve_cadences:
+            arr, _ = session_arrivals(40.0, args.sweep_seconds, probs, (120, 600), rng)
+            d = run_serving(streams, arr, args.sweep_seconds, 3.0, args.sessions, shapes, cadence_hz=hz)
+            by_rate[f"{hz:g}"] = {k: d[k] for k in ("mpix_s", "p50_ms", "p99_ms", "mean_batch", "frames")}
+        if by_rate:
+            line["serving"][f"by_client_hz_at_{args.sessions}_sessions"] = by_rate
         del streams
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
