@@ -68,6 +68,7 @@ def irt() -> ctypes.CDLL:
         "irt_bake_sdf_noalloc": (c_int, [P, u32, u32, P, u32, u32, P, P, P, POINTER(c_float)]),
         "irt_audit_report": (c_int, [P, c_char_p, c_size_t]),
         "irt_host_audit_selftest": (c_int, []),
+        "irt_check_fast_division": (c_int, [P]),
         "irt_lighting": (c_int, [P]),
         "irt_svgf": (c_int, [P, u32, u32]),
         "irt_end_frame": (c_int, [P]),

@@ -1278,6 +1278,46 @@ extern "C" {
 uint32_t irt_abi_version(void) { return IRT_ABI_VERSION; }
 const char* irt_global_error(void) { return irt_host::global_error(); }
 
+// Every binary32 input whose top byte is `hi` (one launch per top byte):
+// rcp_rn and div_pi_rn against the division sequence on their domains.
+__global__ __launch_bounds__(256) void k_check_fast_division(uint32_t hi, unsigned long long* mismatches) {
+  const uint32_t bits = (hi << 24) | (blockIdx.x * 256u + threadIdx.x);
+  const float x = __uint_as_float(bits);
+  const float ax = fabsf(x);
+  const bool special = !(ax > 0.0f) || ax == __builtin_inff();  // zero, NaN or infinite
+  bool bad_rcp = false, bad_pi = false;
+  if (special || (ax >= 0x1p-100f && ax <= 0x1p100f))
+    bad_rcp = __float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x);
+  if (special || (ax >= 0x1p-79f && ax <= 0x1p126f))
+    bad_pi = __float_as_uint(div_pi_rn(x)) != __float_as_uint(x / IRT_PI);
+  if (bad_rcp) atomicAdd(&mismatches[0], 1ull);
+  if (bad_pi) atomicAdd(&mismatches[1], 1ull);
+}
+
+int irt_check_fast_division(uint64_t mismatches[2]) {
+  if (!mismatches) {
+    irt_host::set_global_error("null argument");
+    return IRT_ERR_INVALID_ARGUMENT;
+  }
+  unsigned long long* d = nullptr;
+  unsigned long long h[2] = {0ull, 0ull};
+  hipError_t e = hipMalloc(&d, sizeof(h));
+  if (e == hipSuccess) e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice);
+  for (uint32_t hi = 0; e == hipSuccess && hi < 256u; ++hi) {
+    hipLaunchKernelGGL(k_check_fast_division, dim3(1u << 16), dim3(256), 0, nullptr, hi, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  if (d) (void)hipFree(d);
+  if (e != hipSuccess) {
+    irt_host::set_global_error(std::string("irt_check_fast_division: ") + hipGetErrorString(e));
+    return IRT_ERR_DEVICE;
+  }
+  mismatches[0] = h[0];
+  mismatches[1] = h[1];
+  return IRT_OK;
+}
+
 int irt_create(const irt_settings* s, uint32_t precision, irt_ctx** out) {
   if (!s || !out) {
     irt_host::set_global_error("null argument");

@@ -66,6 +66,34 @@ __device__ __forceinline__ float fsqrt(float x) {
 #endif
 }
 
+// ------------------------------------- correctly rounded division shortcuts
+// The exact contract's a / b is an 11-instruction sequence (operand scaling,
+// reciprocal, Newton step, two remainder corrections, special-case fixup).
+// Where the operands are known to stay in range, fewer steps give the same
+// bits:
+// - the hardware reciprocal refined by one Newton step is RN(1/b) for every
+//   normal |b| in [2^-100, 2^100];
+// - with y = RN(1/b), q = RN(a y) and one remainder step RN(q + (a - b q) y)
+//   is RN(a/b) (Markstein's correction) while a, a/b and the remainder stay
+//   normal;
+// - v_div_fixup gives zero, infinite and NaN operands the IEEE result, and
+//   -0 numerators their sign.
+// irt_check_fast_division compares both helpers with the division sequence
+// on the GPU for every binary32 input of their domains.
+
+// 1.0f / b for normal |b| in [2^-100, 2^100] and for +-0, +-inf and NaN.
+__device__ __forceinline__ float rcp_rn(float b) {
+  const float y = __builtin_amdgcn_rcpf(b);
+  return __builtin_amdgcn_div_fixupf(fmaf(fmaf(-b, y, 1.0f), y, y), b, 1.0f);
+}
+// a / IRT_PI for |a| in [2^-79, 2^126] and for +-0, +-inf and NaN.
+__device__ __forceinline__ float div_pi_rn(float a) {
+  constexpr float y = 1.0f / IRT_PI;  // RN(1/pi), folded at compile time
+  const float q = a * y;
+  const float r = fmaf(-IRT_PI, q, a);
+  return __builtin_amdgcn_div_fixupf(fmaf(r, y, q), IRT_PI, a);
+}
+
 // ------------------------------------------------------------------ vectors
 struct v3 {
   float x, y, z;
@@ -325,8 +353,11 @@ __device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {
   v3 f = v3{f0.x + (1.0f - f0.x) * p5, f0.y + (1.0f - f0.y) * p5, f0.z + (1.0f - f0.z) * p5};
   // (1-f)*(1-metal)*color/PI + f*d*g1v*g1l/max(4*nv*nl, 1e-5), evaluated left to right.
   float den = fmaxf(4.0f * nv * nl, 0.00001f);
-  v3 diff = v3{(1.0f - f.x) * im * s.color.x / IRT_PI, (1.0f - f.y) * im * s.color.y / IRT_PI,
-               (1.0f - f.z) * im * s.color.z / IRT_PI};
+  // The numerators are 0, infinite, NaN or in [2^-59, 2^65]: factors 1 - f
+  // (0 or at least 2^-24: exact below 2 by Sterbenz's lemma), 1 - metal and
+  // the colour (binary16 G-buffer values: 0 or at least 2^-24, at most 2^16).
+  v3 diff = v3{div_pi_rn((1.0f - f.x) * im * s.color.x), div_pi_rn((1.0f - f.y) * im * s.color.y),
+               div_pi_rn((1.0f - f.z) * im * s.color.z)};
   v3 spec = v3{f.x * d * g1v * g1l / den, f.y * d * g1v * g1l / den, f.z * d * g1v * g1l / den};
   return diff + spec;
 }
@@ -337,7 +368,7 @@ __device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, C& cn
   v3 lp = v3{l.position_strength[0], l.position_strength[1], l.position_strength[2]};
   v3 delta = lp - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0001f);
-  v3 wi = delta * (1.0f / sqrtf(r2));  // inverseSqrt(r2)
+  v3 wi = delta * rcp_rn(sqrtf(r2));  // inverseSqrt(r2); sqrt(r2) is in [0.01, 2^64] or +inf
   if (l.color_type[3] > 0.5f) {
     v3 axis = normalize(v3{l.direction_angle[0], l.direction_angle[1], l.direction_angle[2]});
     if (dot(-wi, axis) < cosf(l.direction_angle[3] * 0.5f)) return v3{0.f, 0.f, 0.f};
@@ -355,7 +386,7 @@ template <class C>
 __device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, C& cnt) {
   v3 delta = ld3(v.position) - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0025f);
-  v3 wi = delta * (1.0f / sqrtf(r2));  // inverseSqrt(r2)
+  v3 wi = delta * rcp_rn(sqrtf(r2));  // inverseSqrt(r2); sqrt(r2) is in [0.05, 2^64] or +inf
   cnt.add(C_BRDF, 1);
   v3 b = brdf(s, wi);  // view terms of the frame's camera (surface_view)
   float k1 = fmaxf(dot(s.n, wi), 0.0f);
