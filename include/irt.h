@@ -556,16 +556,23 @@ int irt_reset(irt_ctx* ctx);
 int irt_audit_report(irt_ctx* ctx, char* out, size_t capacity);
 int irt_host_audit_selftest(void);
 
-/* Self-check of the kernels' shortcuts for correctly rounded division (no
- * counterpart in the reference: it guards the exact FP32 contract of this
- * library on the device it runs on).  On the current device, compares the
- * reciprocal shortcut with 1.0f / b for every binary32 b that is 0, infinite,
- * NaN or normal with |b| in [2^-100, 2^100], and the division by pi with
- * a / pi for every a that is 0, infinite, NaN or normal with |a| in
- * [2^-79, 2^126]; writes the number of inputs whose result bits differ to
- * mismatches[0] and mismatches[1] (both 0 on a correct device).  About a
- * second of GPU time.  IRT_ERR_DEVICE (irt_global_error) on a HIP failure. */
-int irt_check_fast_division(uint64_t mismatches[2]);
+/* Self-check of the kernels' shortcuts for correctly rounded division and
+ * square root (no counterpart in the reference: it guards the exact FP32
+ * contract of this library on the device it runs on).  On the current
+ * device, compares for every binary32 input of each domain, and writes the
+ * number of inputs whose result bits differ (all 0 on a correct device):
+ *   mismatches[0]: the reciprocal shortcut against 1.0f / b, for b 0,
+ *                  infinite, NaN or normal with |b| in [2^-100, 2^100];
+ *   mismatches[1]: the division by pi against a / pi, for a 0, infinite,
+ *                  NaN or normal with |a| in [2^-79, 2^126];
+ *   mismatches[2]: the square root shortcut against sqrtf(x), for x NaN or
+ *                  in [2^-96, +inf];
+ *   mismatches[3]: the ranged quotient against a / b, for every a that is 0,
+ *                  infinite, NaN or in [2^-79, 2^110), each over one divisor
+ *                  b drawn from a's hash with |b| in [2^-17, 2^40).
+ * About a second of GPU time.  IRT_ERR_DEVICE (irt_global_error) on a HIP
+ * failure. */
+int irt_check_fast_division(uint64_t mismatches[4]);
 
 /* ------------------------------------------------ host precompute (a23)
  * The scene-side helpers renderer.rs runs on the CPU before upload. */

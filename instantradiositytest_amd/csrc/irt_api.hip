@@ -1279,28 +1279,45 @@ uint32_t irt_abi_version(void) { return IRT_ABI_VERSION; }
 const char* irt_global_error(void) { return irt_host::global_error(); }
 
 // Every binary32 input whose top byte is `hi` (one launch per top byte):
-// rcp_rn and div_pi_rn against the division sequence on their domains.
+// rcp_rn, div_pi_rn and sqrt_rn against the correctly rounded sequences on
+// their domains, and div_ranged for that numerator over one divisor of its
+// domain.
 __global__ __launch_bounds__(256) void k_check_fast_division(uint32_t hi, unsigned long long* mismatches) {
   const uint32_t bits = (hi << 24) | (blockIdx.x * 256u + threadIdx.x);
   const float x = __uint_as_float(bits);
   const float ax = fabsf(x);
   const bool special = !(ax > 0.0f) || ax == __builtin_inff();  // zero, NaN or infinite
-  bool bad_rcp = false, bad_pi = false;
+  bool bad_rcp = false, bad_pi = false, bad_sqrt = false;
   if (special || (ax >= 0x1p-100f && ax <= 0x1p100f))
     bad_rcp = __float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x);
   if (special || (ax >= 0x1p-79f && ax <= 0x1p126f))
     bad_pi = __float_as_uint(div_pi_rn(x)) != __float_as_uint(x / IRT_PI);
+  if (x != x || x >= 0x1p-96f)  // NaN, [2^-96, +inf]
+    bad_sqrt = __float_as_uint(sqrt_rn(x)) != __float_as_uint(sqrtf(x));
+  // div_ranged: numerator x (every binary32 value) over a divisor drawn from
+  // the input's hash, exponent uniform in [-17, 39], any significand, either sign
+  uint32_t hsh = bits * 0x9e3779b1u;
+  hsh ^= hsh >> 15;
+  hsh *= 0x85ebca77u;
+  hsh ^= hsh >> 13;
+  const uint32_t eb = 110u + (hsh >> 26) % 57u;  // biased exponent 110 .. 166
+  const float b = __uint_as_float((hsh & 0x80000000u) | (eb << 23) | (hsh & 0x007fffffu));
+  bool bad_div = false;
+  if (!outside_div_ranged(x, x, x))
+    bad_div = __float_as_uint(div_ranged(x, b, rcp_refined(b))) != __float_as_uint(x / b);
   if (bad_rcp) atomicAdd(&mismatches[0], 1ull);
   if (bad_pi) atomicAdd(&mismatches[1], 1ull);
+  if (bad_sqrt) atomicAdd(&mismatches[2], 1ull);
+  if (bad_div) atomicAdd(&mismatches[3], 1ull);
 }
 
-int irt_check_fast_division(uint64_t mismatches[2]) {
+int irt_check_fast_division(uint64_t mismatches[4]) {
   if (!mismatches) {
     irt_host::set_global_error("null argument");
     return IRT_ERR_INVALID_ARGUMENT;
   }
   unsigned long long* d = nullptr;
-  unsigned long long h[2] = {0ull, 0ull};
+  unsigned long long h[4] = {0ull, 0ull, 0ull, 0ull};
   hipError_t e = hipMalloc(&d, sizeof(h));
   if (e == hipSuccess) e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice);
   for (uint32_t hi = 0; e == hipSuccess && hi < 256u; ++hi) {
@@ -1313,8 +1330,7 @@ int irt_check_fast_division(uint64_t mismatches[2]) {
     irt_host::set_global_error(std::string("irt_check_fast_division: ") + hipGetErrorString(e));
     return IRT_ERR_DEVICE;
   }
-  mismatches[0] = h[0];
-  mismatches[1] = h[1];
+  for (int k = 0; k < 4; ++k) mismatches[k] = h[k];
   return IRT_OK;
 }
 

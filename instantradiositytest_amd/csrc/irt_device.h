@@ -78,9 +78,21 @@ __device__ __forceinline__ float fsqrt(float x) {
 //   normal;
 // - v_div_fixup gives zero, infinite and NaN operands the IEEE result, and
 //   -0 numerators their sign.
-// irt_check_fast_division compares both helpers with the division sequence
-// on the GPU for every binary32 input of their domains.
+// irt_check_fast_division compares these helpers with the correctly rounded
+// sequences on the GPU for every binary32 input of their domains.
 
+// sqrtf(x) for x >= 2^-96, +inf and NaN: the hardware root and the two
+// neighbour residual tests of the correctly rounded sequence, without its
+// rescaling of inputs below 2^-96 and its zero/infinity select (9
+// instructions for 16).
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float down = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = fmaf(-down, s, x) <= 0.0f ? down : s;
+  r = fmaf(-up, s, x) > 0.0f ? up : r;
+  return r;
+}
 // 1.0f / b for normal |b| in [2^-100, 2^100] and for +-0, +-inf and NaN.
 __device__ __forceinline__ float rcp_rn(float b) {
   const float y = __builtin_amdgcn_rcpf(b);
@@ -92,6 +104,26 @@ __device__ __forceinline__ float div_pi_rn(float a) {
   const float q = a * y;
   const float r = fmaf(-IRT_PI, q, a);
   return __builtin_amdgcn_div_fixupf(fmaf(r, y, q), IRT_PI, a);
+}
+// RN(1/b) for normal |b| in [2^-100, 2^100] (no special-case fixup).
+__device__ __forceinline__ float rcp_refined(float b) {
+  const float y = __builtin_amdgcn_rcpf(b);
+  return fmaf(fmaf(-b, y, 1.0f), y, y);
+}
+// a / b given y = rcp_refined(b) for normal |b| in [2^-17, 2^40]: the
+// correctly rounded quotient when a is 0, infinite, NaN or
+// 2^-79 <= |a| < 2^110 (then a, the quotient and the remainder stay normal,
+// so the significand-level equality scales to them exactly).
+__device__ __forceinline__ float div_ranged(float a, float b, float y) {
+  const float q = a * y;
+  return __builtin_amdgcn_div_fixupf(fmaf(fmaf(-b, q, a), y, q), b, a);
+}
+// True when one of the three numerators lies outside div_ranged's domain
+// (frexp's exponent is 0 for 0, infinities and NaN, which are inside).
+__device__ __forceinline__ bool outside_div_ranged(float a0, float a1, float a2) {
+  const int e0 = __builtin_amdgcn_frexp_expf(a0), e1 = __builtin_amdgcn_frexp_expf(a1),
+            e2 = __builtin_amdgcn_frexp_expf(a2);
+  return min(e0, min(e1, e2)) < -78 || max(e0, max(e1, e2)) > 110;
 }
 
 // ------------------------------------------------------------------ vectors
@@ -337,13 +369,29 @@ __device__ __forceinline__ void surface_toward(Surface& s, v3 wo) {
 __device__ __forceinline__ void surface_view(Surface& s, v3 camera) { surface_toward(s, normalize(camera - s.p)); }
 
 // brdf toward wi for a surface whose view terms surface_view() filled.
-__device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {
+// FAST (G-buffer surfaces, whose colour and metalness are binary16 values)
+// divides by pi with div_pi_rn, takes the half vector's normalisation and
+// the specular division with div_ranged and sets `unsafe` when an operand
+// leaves div_ranged's domain (the caller then evaluates the exact form); the
+// results are the exact form's bits.
+template <bool FAST>
+__device__ __forceinline__ v3 brdf_of(const Surface& s, v3 wi, bool& unsafe) {
   const v3 wo = s.wo;
   const float nv = s.nv, a2 = s.a2, g1v = s.g1v, im = s.im;
   const v3 f0 = s.f0;
   float nl = fmaxf(dot(s.n, wi), 0.0f);
   if (nl <= 0.0f) return v3{0.f, 0.f, 0.f};
-  v3 h = normalize(wo + wi);
+  v3 h;
+  if constexpr (FAST) {
+    // |wo + wi| <= 2 for unit wo, wi; below 2^-17, 0 or NaN leaves the domain
+    const v3 hs = wo + wi;
+    const float l = fsqrt(dot(hs, hs));
+    unsafe = unsafe || !(l >= 0x1p-17f) || outside_div_ranged(hs.x, hs.y, hs.z);
+    const float y = rcp_refined(l);
+    h = v3{div_ranged(hs.x, l, y), div_ranged(hs.y, l, y), div_ranged(hs.z, l, y)};
+  } else {
+    h = normalize(wo + wi);
+  }
   float nh = fmaxf(dot(s.n, h), 0.0f);
   float vh = fmaxf(dot(wo, h), 0.0f);
   float q = nh * nh * (a2 - 1.0f) + 1.0f;
@@ -353,46 +401,106 @@ __device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {
   v3 f = v3{f0.x + (1.0f - f0.x) * p5, f0.y + (1.0f - f0.y) * p5, f0.z + (1.0f - f0.z) * p5};
   // (1-f)*(1-metal)*color/PI + f*d*g1v*g1l/max(4*nv*nl, 1e-5), evaluated left to right.
   float den = fmaxf(4.0f * nv * nl, 0.00001f);
-  // The numerators are 0, infinite, NaN or in [2^-59, 2^65]: factors 1 - f
-  // (0 or at least 2^-24: exact below 2 by Sterbenz's lemma), 1 - metal and
-  // the colour (binary16 G-buffer values: 0 or at least 2^-24, at most 2^16).
-  v3 diff = v3{div_pi_rn((1.0f - f.x) * im * s.color.x), div_pi_rn((1.0f - f.y) * im * s.color.y),
-               div_pi_rn((1.0f - f.z) * im * s.color.z)};
-  v3 spec = v3{f.x * d * g1v * g1l / den, f.y * d * g1v * g1l / den, f.z * d * g1v * g1l / den};
+  v3 diff;
+  if constexpr (FAST) {
+    // G-buffer surfaces: the numerators are 0, infinite, NaN or in
+    // [2^-59, 2^65]: factors 1 - f (0 or at least 2^-24: exact below 2 by
+    // Sterbenz's lemma), 1 - metal and the colour (binary16 values: 0 or at
+    // least 2^-24, at most 2^16).
+    diff = v3{div_pi_rn((1.0f - f.x) * im * s.color.x), div_pi_rn((1.0f - f.y) * im * s.color.y),
+              div_pi_rn((1.0f - f.z) * im * s.color.z)};
+  } else {
+    diff = v3{(1.0f - f.x) * im * s.color.x / IRT_PI, (1.0f - f.y) * im * s.color.y / IRT_PI,
+              (1.0f - f.z) * im * s.color.z / IRT_PI};
+  }
+  v3 spec;
+  if constexpr (FAST) {
+    // den in [1e-5, 4.01]: n and wo, wi are unit vectors
+    const float s0 = f.x * d * g1v * g1l, s1 = f.y * d * g1v * g1l, s2 = f.z * d * g1v * g1l;
+    unsafe = unsafe || outside_div_ranged(s0, s1, s2);
+    const float y = rcp_refined(den);
+    spec = v3{div_ranged(s0, den, y), div_ranged(s1, den, y), div_ranged(s2, den, y)};
+  } else {
+    spec = v3{f.x * d * g1v * g1l / den, f.y * d * g1v * g1l / den, f.z * d * g1v * g1l / den};
+  }
   return diff + spec;
 }
 
-// direct — common.wgsl:75-79.
-template <class C>
-__device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, C& cnt) {
+// The exact form, for surfaces of any material values (the VPL pass).
+__device__ __forceinline__ v3 brdf(const Surface& s, v3 wi) {
+  bool unused = false;
+  return brdf_of<false>(s, wi, unused);
+}
+
+// direct — common.wgsl:75-79.  FAST as in brdf_of.
+template <bool FAST, class C>
+__device__ __forceinline__ v3 direct_of(const Surface& s, const irt_light& l, C& cnt, bool& unsafe) {
   v3 lp = v3{l.position_strength[0], l.position_strength[1], l.position_strength[2]};
   v3 delta = lp - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0001f);
-  v3 wi = delta * rcp_rn(sqrtf(r2));  // inverseSqrt(r2); sqrt(r2) is in [0.01, 2^64] or +inf
+  v3 wi = delta * rcp_rn(sqrt_rn(r2));  // inverseSqrt(r2); r2 >= 1e-4, sqrt(r2) in [0.01, 2^64] or +inf
   if (l.color_type[3] > 0.5f) {
     v3 axis = normalize(v3{l.direction_angle[0], l.direction_angle[1], l.direction_angle[2]});
     if (dot(-wi, axis) < cosf(l.direction_angle[3] * 0.5f)) return v3{0.f, 0.f, 0.f};
   }
   cnt.add(C_BRDF, 1);
-  v3 b = brdf(s, wi);  // view terms of the frame's camera (surface_view)
+  v3 b = brdf_of<FAST>(s, wi, unsafe);  // view terms of the frame's camera (surface_view)
   float k = fmaxf(dot(s.n, wi), 0.0f);
   float strength = l.position_strength[3];
-  return v3{b.x * k * l.color_type[0] * strength / r2, b.y * k * l.color_type[1] * strength / r2,
-            b.z * k * l.color_type[2] * strength / r2};
+  const float n0 = b.x * k * l.color_type[0] * strength, n1 = b.y * k * l.color_type[1] * strength,
+              n2 = b.z * k * l.color_type[2] * strength;
+  if constexpr (FAST) {
+    // r2 >= 1e-4; above 2^40 (or +inf) it leaves div_ranged's domain
+    unsafe = unsafe || !(r2 <= 0x1p40f) || outside_div_ranged(n0, n1, n2);
+    const float y = rcp_refined(r2);
+    return v3{div_ranged(n0, r2, y), div_ranged(n1, r2, y), div_ranged(n2, r2, y)};
+  } else {
+    return v3{n0 / r2, n1 / r2, n2 / r2};
+  }
+}
+// The shortcut form, and the exact form for a wave with a lane whose
+// operands leave the shortcuts' domains (counted once).
+template <class C>
+__device__ __forceinline__ v3 direct(const Surface& s, const irt_light& l, C& cnt) {
+  bool unsafe = false;
+  v3 v = direct_of<true>(s, l, cnt, unsafe);
+  if (unsafe) {
+    Count<false> none;
+    v = direct_of<false>(s, l, none, unsafe);
+  }
+  return v;
 }
 
-// indirect — common.wgsl:80-83.
-template <class C>
-__device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, C& cnt) {
+// indirect — common.wgsl:80-83.  FAST as in brdf_of.
+template <bool FAST, class C>
+__device__ __forceinline__ v3 indirect_of(const Surface& s, const irt_vpl& v, C& cnt, bool& unsafe) {
   v3 delta = ld3(v.position) - s.p;
   float r2 = fmaxf(dot(delta, delta), 0.0025f);
-  v3 wi = delta * rcp_rn(sqrtf(r2));  // inverseSqrt(r2); sqrt(r2) is in [0.05, 2^64] or +inf
+  v3 wi = delta * rcp_rn(sqrt_rn(r2));  // inverseSqrt(r2); r2 >= 0.0025, sqrt(r2) in [0.05, 2^64] or +inf
   cnt.add(C_BRDF, 1);
-  v3 b = brdf(s, wi);  // view terms of the frame's camera (surface_view)
+  v3 b = brdf_of<FAST>(s, wi, unsafe);  // view terms of the frame's camera (surface_view)
   float k1 = fmaxf(dot(s.n, wi), 0.0f);
   float k2 = fmaxf(dot(ld3(v.normal), -wi), 0.0f);
   float den = 4.0f * IRT_PI * r2;
-  return v3{b.x * k1 * k2 * v.flux[0] / den, b.y * k1 * k2 * v.flux[1] / den, b.z * k1 * k2 * v.flux[2] / den};
+  const float n0 = b.x * k1 * k2 * v.flux[0], n1 = b.y * k1 * k2 * v.flux[1], n2 = b.z * k1 * k2 * v.flux[2];
+  if constexpr (FAST) {
+    // den >= 0.0314; above 2^40 (or +inf) it leaves div_ranged's domain
+    unsafe = unsafe || !(den <= 0x1p40f) || outside_div_ranged(n0, n1, n2);
+    const float y = rcp_refined(den);
+    return v3{div_ranged(n0, den, y), div_ranged(n1, den, y), div_ranged(n2, den, y)};
+  } else {
+    return v3{n0 / den, n1 / den, n2 / den};
+  }
+}
+template <class C>
+__device__ __forceinline__ v3 indirect(const Surface& s, const irt_vpl& v, C& cnt) {
+  bool unsafe = false;
+  v3 r = indirect_of<true>(s, v, cnt, unsafe);
+  if (unsafe) {
+    Count<false> none;
+    r = indirect_of<false>(s, v, none, unsafe);
+  }
+  return r;
 }
 
 // ------------------------------------------------------------- reservoirs
