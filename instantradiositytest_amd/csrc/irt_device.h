@@ -125,6 +125,19 @@ __device__ __forceinline__ bool outside_div_ranged(float a0, float a1, float a2)
             e2 = __builtin_amdgcn_frexp_expf(a2);
   return min(e0, min(e1, e2)) < -78 || max(e0, max(e1, e2)) > 110;
 }
+// a / pdf for a candidate weight, pdf = 1 / count (|pdf| in [2^-32, 1]) and
+// y = rcp_refined(pdf): div_ranged's argument holds for 2^-79 <= |a| < 2^95
+// (the quotient stays below 2^127), and 0, infinite or NaN a; other a take
+// the division sequence.  The branch is skipped by waves whose lanes all fit.
+__device__ __forceinline__ float div_weight(float a, float pdf, float y) {
+  float w = div_ranged(a, pdf, y);
+  const int e = __builtin_amdgcn_frexp_expf(a);
+  if (e < -78 || e > 95) {
+    asm volatile("" : "+v"(a));  // keeps the division in the branch (not speculated)
+    w = a / pdf;
+  }
+  return w;
+}
 
 // ------------------------------------------------------------------ vectors
 struct v3 {
@@ -385,8 +398,10 @@ __device__ __forceinline__ v3 brdf_of(const Surface& s, v3 wi, bool& unsafe) {
   if constexpr (FAST) {
     // |wo + wi| <= 2 for unit wo, wi; below 2^-17, 0 or NaN leaves the domain
     const v3 hs = wo + wi;
-    const float l = fsqrt(dot(hs, hs));
-    unsafe = unsafe || !(l >= 0x1p-17f) || outside_div_ranged(hs.x, hs.y, hs.z);
+    const float l2 = dot(hs, hs);
+    // l2 >= 2^-34: sqrt_rn's domain and a length of at least 2^-17
+    unsafe = unsafe || !(l2 >= 0x1p-34f) || outside_div_ranged(hs.x, hs.y, hs.z);
+    const float l = sqrt_rn(l2);
     const float y = rcp_refined(l);
     h = v3{div_ranged(hs.x, l, y), div_ranged(hs.y, l, y), div_ranged(hs.z, l, y)};
   } else {
@@ -396,7 +411,17 @@ __device__ __forceinline__ v3 brdf_of(const Surface& s, v3 wi, bool& unsafe) {
   float vh = fmaxf(dot(wo, h), 0.0f);
   float q = nh * nh * (a2 - 1.0f) + 1.0f;
   float d = a2 / (IRT_PI * q * q);
-  float g1l = 2.0f * nl / (nl + sqrtf(a2 + (1.0f - a2) * nl * nl));
+  float g1l;
+  if constexpr (FAST) {
+    // nl >= 2^-40 (and nl <= 1.01) puts 2 nl and its divisor in div_ranged's
+    // argument (both in [2^-40, 4)); the root's argument must reach 2^-96
+    const float g = a2 + (1.0f - a2) * nl * nl;
+    unsafe = unsafe || !(nl >= 0x1p-40f) || !(g >= 0x1p-96f);
+    const float gb = nl + sqrt_rn(g);
+    g1l = div_ranged(2.0f * nl, gb, rcp_refined(gb));
+  } else {
+    g1l = 2.0f * nl / (nl + sqrtf(a2 + (1.0f - a2) * nl * nl));
+  }
   float p5 = pow5(1.0f - vh);
   v3 f = v3{f0.x + (1.0f - f0.x) * p5, f0.y + (1.0f - f0.y) * p5, f0.z + (1.0f - f0.z) * p5};
   // (1-f)*(1-metal)*color/PI + f*d*g1v*g1l/max(4*nv*nl, 1e-5), evaluated left to right.

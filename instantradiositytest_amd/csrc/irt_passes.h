@@ -130,10 +130,11 @@ __device__ __forceinline__ irt_reservoir candidates_direct(const LightArgs<P>& a
   uint32_t rng = seed(id, 11u, F.size[2]);
   if (valid && F.counts[0] > 0u) {
     const float pdf = 1.0f / (float)F.counts[0];
+    const float ypdf = rcp_refined(pdf);
     for (uint32_t i = 0; i < F.restir[0]; ++i) {
       uint32_t light_id = min(f2u_sat(rnd(rng) * (float)F.counts[0]), F.counts[0] - 1u);
       float target = luminance(direct(s, a.sc.lights[light_id], cnt));
-      add(r, light_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
+      add(r, light_id, F.size[2], target, pdf, div_weight(target, pdf, ypdf), 1u, 0u, rng);
     }
   }
   return r;
@@ -148,12 +149,13 @@ __device__ __forceinline__ irt_reservoir candidates_indirect(const LightArgs<P>&
   uint32_t rng = seed(id, 13u, F.size[2]);
   if (valid) {
     const float pdf = 1.0f / (float)F.counts[1];
+    const float ypdf = rcp_refined(pdf);
     const uint32_t slots = f2u_sat(F.rr[2]) + 1u;
     const irt_vpl* current = a.vpl_ring + (size_t)(F.size[2] % slots) * F.counts[1];
     for (uint32_t i = 0; i < F.restir[1]; ++i) {
       uint32_t vpl_id = min(f2u_sat(rnd(rng) * (float)F.counts[1]), F.counts[1] - 1u);
       float target = luminance(indirect(s, current[vpl_id], cnt));
-      add(r, vpl_id, F.size[2], target, pdf, target / pdf, 1u, 0u, rng);
+      add(r, vpl_id, F.size[2], target, pdf, div_weight(target, pdf, ypdf), 1u, 0u, rng);
     }
   }
   return r;
