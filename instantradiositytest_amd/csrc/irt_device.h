@@ -66,6 +66,29 @@ __device__ __forceinline__ float fsqrt(float x) {
 #endif
 }
 
+// min(max(x, lo), hi) for lo <= hi.  IEEE-mode minimum and maximum quiet a
+// signalling NaN, so the compiler puts a canonicalising copy in front of
+// every operand it cannot prove is an arithmetic result (values read from
+// LDS or memory); the contracted unit takes the median of three, which is
+// the same value for every non-NaN x in one instruction.  The exact contract
+// keeps the minimum and maximum (a NaN x gives lo there, as in the FP32
+// restatement).
+__device__ __forceinline__ float fclamp(float x, float lo, float hi) {
+#if IRT_MARCH_MATH == IRT_MARCH_CONTRACTED
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+  return fminf(fmaxf(x, lo), hi);
+#endif
+}
+// max(x, 0) the same way.
+__device__ __forceinline__ float fmax0(float x) {
+#if IRT_MARCH_MATH == IRT_MARCH_CONTRACTED
+  return __builtin_amdgcn_fmed3f(x, 0.0f, __builtin_inff());
+#else
+  return fmaxf(x, 0.0f);
+#endif
+}
+
 // ------------------------------------- correctly rounded division shortcuts
 // The exact contract's a / b is an 11-instruction sequence (operand scaling,
 // reciprocal, Newton step, two remainder corrections, special-case fixup).
@@ -669,7 +692,7 @@ __device__ __forceinline__ float mesh_distance(const SV& sc, v3 p, const irt_mes
   cnt.add(C_OBJ_GRID, 1);
   uint32_t res = mesh.field[3];
   v3 lo = ld3(mesh.lo), hi = ld3(mesh.hi);
-  v3 clamped = vmin(vmax(p, lo), hi);
+  v3 clamped = v3{fclamp(p.x, lo.x, hi.x), fclamp(p.y, lo.y, hi.y), fclamp(p.z, lo.z, hi.z)};
   v3 q = (clamped - lo) * ld3(mesh.grid);
   float value = sample_field(sc, mesh.field, q, res);
   float bound = value - mesh.grid[3] * (0.5f + 1.0f / 128.0f);
@@ -729,7 +752,7 @@ __device__ __forceinline__ bool scene_distance_grid(const SV& sc, float hit_eps,
     const float4 lo = g.boxes[2 * j], hi = g.boxes[2 * j + 1];
     v3 delta = vmaxs(vmax(v3{lo.x - p.x, lo.y - p.y, lo.z - p.z}, v3{p.x - hi.x, p.y - hi.y, p.z - hi.z}), 0.0f);
     v3 lb = delta * 0.9998f;
-    float bp = fmaxf(best, 0.0f);
+    float bp = fmax0(best);
     if (dot(lb, lb) > bp * bp) continue;
     float vx;
     float d = object_distance(sc, p, j, need_spacing, hit_eps, &vx, cnt);
@@ -760,7 +783,7 @@ __device__ __forceinline__ Distance scene_distance(const SV& sc, uint32_t n_obje
     cnt.add(C_NODE, 1);
     v3 delta = vmaxs(vmax(ld3(node.lo) - p, p - ld3(node.hi)), 0.0f);
     v3 lb = delta * 0.9998f;
-    float bp = fmaxf(best, 0.0f);
+    float bp = fmax0(best);
     if (dot(lb, lb) > bp * bp) {
       cursor = node.info[3];
       continue;
