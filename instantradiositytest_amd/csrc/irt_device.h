@@ -666,6 +666,15 @@ __device__ __forceinline__ float sample_field(const SV& sc, const uint32_t* orig
     t000 = b[0], t100 = b[1], t010 = b[sy], t110 = b[sy + 1];
     t001 = b[sz], t101 = b[sz + 1], t011 = b[sz + sy], t111 = b[sz + sy + 1];
   }
+#if IRT_MARCH_MATH == IRT_MARCH_CONTRACTED
+  // The contracted unit blends with seven lerps, x then y then z: the same
+  // trilinear value up to rounding, in 14 instructions for the weighted
+  // sum's 27.
+  const float x00 = fmaf(fx, t100 - t000, t000), x10 = fmaf(fx, t110 - t010, t010);
+  const float x01 = fmaf(fx, t101 - t001, t001), x11 = fmaf(fx, t111 - t011, t011);
+  const float y0 = fmaf(fy, x10 - x00, x00), y1 = fmaf(fy, x11 - x01, x01);
+  return fmaf(fz, y1 - y0, y0);
+#else
   float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
   // value += texel * w.x * w.y * w.z, corners in z, y, x order (common.wgsl:33-36).
   float value = 0.0f;
@@ -678,6 +687,7 @@ __device__ __forceinline__ float sample_field(const SV& sc, const uint32_t* orig
   value += t011 * gx * fy * fz;
   value += t111 * fx * fy * fz;
   return value;
+#endif
 }
 
 // mesh_distance — sdf.wgsl:5-33.
