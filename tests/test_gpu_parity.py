@@ -83,8 +83,10 @@ def atlas_as_stored(scene, precision: str) -> np.ndarray:
 
 
 def run_open_loop(precision: str, frames: int = 5, w: int = 160, h: int = 100, debug: int = 0,
-                  name: str = "plain-640x400"):
+                  name: str = "plain-640x400", scene_hook=None, inputs_hook=None):
     cfg, cs = small_config(w, h, frames, name)
+    if scene_hook:
+        scene_hook(cs)
     settings = cfg.settings()
     r = Renderer(cs.scene, settings, precision=precision, schedule="passes")
     r.debug_mode = debug
@@ -94,6 +96,8 @@ def run_open_loop(precision: str, frames: int = 5, w: int = 160, h: int = 100, d
     stats = {}
     for fi in range(frames):
         inputs = cs.frame_inputs(fi)
+        if inputs_hook:
+            inputs_hook(inputs)
         frame = r.begin_frame(cs.scene, inputs)
         gb = gbuffer_planes(r, inputs)
         cur, prev, ring = vpl_pools(r, fi, V, slots)
@@ -184,6 +188,41 @@ def test_open_loop_every_pass_matches_oracle(require_gpu, precision, debug):
     check_stats(stats)
     if debug >= 3:
         assert any(k[1] == "svgf_bypass" for k in stats)
+
+
+def far_light(cs: ConfigScene):
+    """Light 0 moved about 5e6 units away (r2 > 2^40) with its strength raised
+    to match: the inverse-square division leaves the ranged quotient's domain."""
+    lights = cs.scene.lights.copy()
+    lights[0]["position_strength"] = (3e6, 3e6, 3e6, 4e13)
+    cs.scene.set_lights(lights)
+
+
+def extreme_materials(inputs):
+    """Rows 0-29: roughness 2^-24 (the smallest binary16 subnormal), so the GGX
+    term and the specular numerators fall below 2^-79; rows 30-49: roughness 0
+    (zero specular numerators); rows 50-59: metals of black albedo (zero
+    diffuse numerators)."""
+    p = inputs.params.reshape(inputs.params.shape[0], -1, 2)
+    p[0:30, :, 0] = 0x0001
+    p[30:50, :, 0] = 0x0000
+    p[50:60, :, 1] = 0x3C00
+    c = inputs.color.reshape(inputs.color.shape[0], -1, 4)
+    c[50:60, :, 0:3] = 0
+
+
+def test_open_loop_division_fallbacks_match_oracle(require_gpu):
+    """The lighting kernels' division shortcuts fall back to the exact form for
+    operands outside their domains (a far light, near-zero and zero
+    roughness, black metals); the FP32 profile's reservoirs must still equal
+    the FP32 restatement's exactly, pass by pass."""
+    log("[parity] open loop fp32 160x100 x3 frames, far light and extreme materials")
+    stats = run_open_loop("fp32", frames=3, scene_hook=far_light, inputs_hook=extreme_materials)
+    check_stats(stats)
+    for key, s in sorted(stats.items(), key=lambda kv: str(kv[0])):
+        if "match" in s:
+            log(key, {k: s[k] for k in ("match", "max_rel", "mismatches")})
+            assert s["match"] == 1.0 and s["max_rel"] == 0.0, (key, s)
 
 
 def test_open_loop_full_size_1920x1200(require_gpu):
