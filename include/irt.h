@@ -567,9 +567,11 @@ int irt_host_audit_selftest(void);
  *                  NaN or normal with |a| in [2^-79, 2^126];
  *   mismatches[2]: the square root shortcut against sqrtf(x), for x NaN or
  *                  in [2^-96, +inf];
- *   mismatches[3]: the ranged quotient against a / b, for every a that is 0,
- *                  infinite, NaN or in [2^-79, 2^110), each over one divisor
- *                  b drawn from a's hash with |b| in [2^-17, 2^40).
+ *   mismatches[3]: the ranged quotient against a / b, for every a, each
+ *                  over one divisor b drawn from a's hash with |b| in
+ *                  [2^-100, 2^100), compared where a is 0, infinite or NaN
+ *                  or where |a| >= 2^-79 and a / b's binary exponent is
+ *                  within [-125, 126].
  * About a second of GPU time.  IRT_ERR_DEVICE (irt_global_error) on a HIP
  * failure. */
 int irt_check_fast_division(uint64_t mismatches[4]);

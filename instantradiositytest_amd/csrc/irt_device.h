@@ -133,16 +133,19 @@ __device__ __forceinline__ float rcp_refined(float b) {
   const float y = __builtin_amdgcn_rcpf(b);
   return fmaf(fmaf(-b, y, 1.0f), y, y);
 }
-// a / b given y = rcp_refined(b) for normal |b| in [2^-17, 2^40]: the
-// correctly rounded quotient when a is 0, infinite, NaN or
-// 2^-79 <= |a| < 2^110 (then a, the quotient and the remainder stay normal,
-// so the significand-level equality scales to them exactly).
+// a / b given y = rcp_refined(b), for normal |b| in [2^-100, 2^100]: the
+// correctly rounded quotient when a is 0, infinite or NaN, or when
+// |a| >= 2^-79 and the binary exponents satisfy -125 <= e(a) - e(b) <= 126
+// (a, the quotient and the remainder then stay normal, so the
+// significand-level equality scales to them exactly).  For |b| in
+// [2^-17, 2^40) every a in [2^-79, 2^110) qualifies (outside_div_ranged).
 __device__ __forceinline__ float div_ranged(float a, float b, float y) {
   const float q = a * y;
   return __builtin_amdgcn_div_fixupf(fmaf(fmaf(-b, q, a), y, q), b, a);
 }
-// True when one of the three numerators lies outside div_ranged's domain
-// (frexp's exponent is 0 for 0, infinities and NaN, which are inside).
+// True when one of the three numerators lies outside [2^-79, 2^110), div_ranged's
+// domain for divisors in [2^-17, 2^40) (frexp's exponent is 0 for 0,
+// infinities and NaN, which are inside).
 __device__ __forceinline__ bool outside_div_ranged(float a0, float a1, float a2) {
   const int e0 = __builtin_amdgcn_frexp_expf(a0), e1 = __builtin_amdgcn_frexp_expf(a1),
             e2 = __builtin_amdgcn_frexp_expf(a2);
@@ -436,8 +439,10 @@ __device__ __forceinline__ v3 brdf_of(const Surface& s, v3 wi, bool& unsafe) {
   float d = a2 / (IRT_PI * q * q);
   float g1l;
   if constexpr (FAST) {
-    // nl >= 2^-40 (and nl <= 1.01) puts 2 nl and its divisor in div_ranged's
-    // argument (both in [2^-40, 4)); the root's argument must reach 2^-96
+    // With nl in [2^-40, 1.01] and the root's argument at least 2^-96,
+    // 2 nl is in [2^-39, 2.02] and its divisor nl + sqrt(...) in [2^-40, 2^33]
+    // (a2 <= 2^64 for binary16 roughness), so the quotient's exponent is in
+    // [-72, 1]: inside div_ranged's contract
     const float g = a2 + (1.0f - a2) * nl * nl;
     unsafe = unsafe || !(nl >= 0x1p-40f) || !(g >= 0x1p-96f);
     const float gb = nl + sqrt_rn(g);
