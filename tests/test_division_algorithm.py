@@ -6,7 +6,15 @@ binary32 quotient RN(a/b).  The GPU check
 over exponents; this test pins the remainder step itself, independent of the
 hardware, on random significand pairs and on quotients chosen to lie within
 a few units in the last place of a rounding midpoint (where a dropped
-correction or a wrong rounding would show)."""
+correction or a wrong rounding would show).
+
+The shortened square root (sqrt_rn) keeps the hardware root s and its two
+neighbour tests: with d and u the binary32 values just below and above s, it
+returns d if x - d s <= 0, u if x - u s > 0, else s (each residual one fused
+multiply-add, whose sign is the exact residual's).  The tests below take s one
+step below, at, or one step above the correctly rounded root and check that
+the two tests land on RN(sqrt(x)), also at powers of four, where the value
+below s is only half a step away."""
 from __future__ import annotations
 
 import random
@@ -105,3 +113,77 @@ def test_rounding_helper_agrees_with_numpy():
     for _ in range(500):
         x = Fraction(rng.getrandbits(60) + 1, rng.getrandbits(40) + 1)
         assert float(rn(x)) == f32(x)
+
+
+def next_up(v: np.float32) -> np.float32:
+    return np.nextafter(v, np.float32(np.inf), dtype=np.float32)
+
+
+def next_down(v: np.float32) -> np.float32:
+    return np.nextafter(v, np.float32(0), dtype=np.float32)
+
+
+def rn_sqrt(x: np.float32) -> np.float32:
+    """RN(sqrt(x)) for a positive binary32 x, decided on exact squares."""
+    X = Fraction(float(x))
+    a = np.float32(np.sqrt(np.float64(x)))
+    while Fraction(float(a)) ** 2 > X:
+        a = next_down(a)
+    while Fraction(float(next_up(a))) ** 2 <= X:
+        a = next_up(a)
+    b = next_up(a)
+    m = (Fraction(float(a)) + Fraction(float(b))) / 2
+    assert m * m != X  # no binary32 square root lies on a midpoint
+    return a if X < m * m else b
+
+
+def sqrt_by_neighbour_tests(x: np.float32, s: np.float32) -> np.float32:
+    X, S = Fraction(float(x)), Fraction(float(s))
+    d, u = next_down(s), next_up(s)
+    r = d if X - Fraction(float(d)) * S <= 0 else s
+    if X - Fraction(float(u)) * S > 0:
+        r = u
+    return r
+
+
+def random_f32(rng: random.Random, lo_exp: int, hi_exp: int) -> np.float32:
+    e = rng.randint(lo_exp, hi_exp)
+    return np.float32(float(sig(rng.getrandbits(23))) * 2.0 ** e)
+
+
+def test_neighbour_tests_round_the_square_root_correctly():
+    rng = random.Random(5)
+    xs = [random_f32(rng, -96, 100) for _ in range(1500)]
+    # powers of four and their neighbours: the root is a power of two
+    for e in range(-48, 50, 7):
+        p = np.float32(4.0 ** e)
+        xs += [p, next_up(p), next_down(p)]
+    for x in xs:
+        r = rn_sqrt(x)
+        for s in (next_down(r), r, next_up(r)):
+            assert sqrt_by_neighbour_tests(x, s) == r, (x, s, r)
+
+
+def test_residual_magnitude_keeps_its_sign_above_the_domain_floor():
+    """Each residual x - v s is a multiple of ulp(s)^2, at least 2^-142 for
+    x >= 2^-96: above binary32's smallest subnormal, so the fused
+    multiply-add's rounded residual has the exact residual's sign."""
+    x = np.float32(2.0 ** -96)
+    s = rn_sqrt(x)
+    ulp = Fraction(float(next_up(s))) - Fraction(float(s))
+    assert ulp * ulp >= Fraction(1, 2 ** 142) > Fraction(1, 2 ** 149)
+
+
+def test_a_skipped_neighbour_test_fails():
+    """Without the upward test, a hardware root one step low stays wrong."""
+    rng = random.Random(9)
+    wrong = 0
+    for _ in range(300):
+        x = random_f32(rng, -20, 20)
+        r = rn_sqrt(x)
+        s = next_down(r)
+        X, S = Fraction(float(x)), Fraction(float(s))
+        d = next_down(s)
+        only_down = d if X - Fraction(float(d)) * S <= 0 else s
+        wrong += only_down != r
+    assert wrong > 0
